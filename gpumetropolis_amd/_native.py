@@ -1,0 +1,298 @@
+"""ctypes binding of the C ABI in include/gpum_hip.h.
+
+The product path goes through this module only.  Loading fails loudly when
+the in-tree library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from .build import LIB
+
+_lib = None
+
+
+class BackendError(RuntimeError):
+    """A failed call into the HIP backend (message from gpum_hip_last_error)."""
+
+
+class ChainResult(C.Structure):
+    _fields_ = [("n_iter", C.c_uint32), ("n_chains", C.c_uint32),
+                ("n_params", C.c_uint32), ("draws", C.POINTER(C.c_float)),
+                ("accept_rate", C.POINTER(C.c_float)),
+                ("last_log_post", C.POINTER(C.c_double))]
+
+
+class RunExt(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("stream_id", C.c_uint32),
+                ("trace_plp", C.POINTER(C.c_double)),
+                ("trace_prop", C.POINTER(C.c_double)),
+                ("trace_accept", C.POINTER(C.c_uint8)),
+                ("final_state", C.POINTER(C.c_double)),
+                ("init_log_post", C.POINTER(C.c_double)),
+                ("chains_per_block", C.c_int32),
+                ("waves_per_chain", C.c_int32),
+                ("precompute_data", C.c_int32), ("fuse", C.c_int32),
+                ("kernel_ms", C.c_double)]
+
+
+class ModelInfo(C.Structure):
+    _fields_ = [("n_vm_instr", C.c_uint32), ("vm_depth", C.c_uint32),
+                ("ref_depth", C.c_uint32), ("n_uniform", C.c_uint32),
+                ("n_cols_total", C.c_uint32), ("n_hoist_words", C.c_uint32),
+                ("w_exec", C.c_double), ("w_bytecode", C.c_double)]
+
+
+_u32p = C.POINTER(C.c_uint32)
+_f64p = C.POINTER(C.c_double)
+
+
+def lib():
+    """The loaded backend library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        raise BackendError(
+            f"HIP backend library not built: {LIB} is missing "
+            "(run `python -m gpumetropolis_amd.build`)")
+    L = C.CDLL(LIB)
+    sz = C.c_size_t
+    L.gpum_hip_available.restype = C.c_int
+    L.gpum_hip_last_error.restype = C.c_char_p
+    run_args = [_u32p, sz, _f64p, sz, _u32p, sz, _f64p, sz, C.c_uint32, _f64p,
+                C.c_uint32, C.c_uint32, _f64p, C.c_uint32, _f64p, _f64p,
+                C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                _f64p]
+    L.gpum_hip_run_chains.argtypes = run_args + [C.POINTER(ChainResult)]
+    L.gpum_hip_run_chains.restype = C.c_int
+    L.gpum_hip_run_chains_ex.argtypes = run_args + [C.POINTER(RunExt),
+                                                    C.POINTER(ChainResult)]
+    L.gpum_hip_run_chains_ex.restype = C.c_int
+    L.gpum_hip_result_free.argtypes = [C.POINTER(ChainResult)]
+    L.gpum_hip_result_free.restype = None
+    L.gpum_hip_loglik_batch.argtypes = [_u32p, sz, _f64p, sz, C.c_uint32,
+                                        _f64p, C.c_uint32, C.c_uint32, _f64p,
+                                        C.c_uint32, _f64p]
+    L.gpum_hip_loglik_batch.restype = C.c_int
+    L.gpum_hip_model_create.argtypes = [_u32p, sz, _f64p, sz, _u32p, sz,
+                                        _f64p, sz, C.c_uint32, _f64p,
+                                        C.c_uint32, C.c_uint32, C.c_int32,
+                                        C.c_int32]
+    L.gpum_hip_model_create.restype = C.c_void_p
+    L.gpum_hip_model_info.argtypes = [C.c_void_p, C.POINTER(ModelInfo)]
+    L.gpum_hip_model_info.restype = C.c_int
+    L.gpum_hip_model_run.argtypes = [C.c_void_p, _f64p, C.c_uint32, _f64p,
+                                     _f64p, C.c_uint32, C.c_uint32,
+                                     C.c_uint32, C.c_double, C.c_double,
+                                     _f64p, C.POINTER(RunExt),
+                                     C.POINTER(ChainResult)]
+    L.gpum_hip_model_run.restype = C.c_int
+    L.gpum_hip_model_loglik.argtypes = [C.c_void_p, _f64p, C.c_uint32, _f64p]
+    L.gpum_hip_model_loglik.restype = C.c_int
+    L.gpum_hip_model_free.argtypes = [C.c_void_p]
+    L.gpum_hip_model_free.restype = None
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().gpum_hip_last_error().decode()
+
+
+def available() -> bool:
+    return bool(lib().gpum_hip_available())
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise BackendError(last_error())
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32).ravel())
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
+
+
+def _p(a: Optional[np.ndarray], t):
+    if a is None or a.size == 0:
+        return C.cast(None, C.POINTER(t))
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+@dataclass
+class RunOutput:
+    """The reference's ChainResult (chain_kernel.rs:360-369) as numpy."""
+    draws: np.ndarray          # (n_iter, n_chains, n_params) float32
+    accept_rate: np.ndarray    # (n_chains,) float32
+    last_log_post: np.ndarray  # (n_chains,) float64
+    final_state: Optional[np.ndarray] = None
+    trace_plp: Optional[np.ndarray] = None
+    trace_prop: Optional[np.ndarray] = None
+    trace_accept: Optional[np.ndarray] = None
+    init_log_post: Optional[np.ndarray] = None
+    kernel_ms: float = 0.0
+
+
+def _collect(res: ChainResult) -> RunOutput:
+    T, Cc, P = res.n_iter, res.n_chains, res.n_params
+    n = T * Cc * P
+    flat = np.ctypeslib.as_array(res.draws, shape=(max(n, 1),))[:n].copy()
+    draws = flat.reshape((P, Cc, T)).transpose(2, 1, 0).copy()
+    acc = np.ctypeslib.as_array(res.accept_rate, shape=(Cc,)).copy()
+    lp = np.ctypeslib.as_array(res.last_log_post, shape=(Cc,)).copy()
+    lib().gpum_hip_result_free(C.byref(res))
+    return RunOutput(draws=draws, accept_rate=acc, last_log_post=lp)
+
+
+def _ext(n_iter, n_chains, n_params, trace, final_state, stream_id, tiling,
+         precompute_data, fuse):
+    ext = RunExt()
+    ext.struct_size = C.sizeof(RunExt)
+    ext.stream_id = stream_id
+    keep = {}
+    if trace:
+        keep["plp"] = np.zeros(n_iter * n_chains, np.float64)
+        keep["prop"] = np.zeros(n_iter * n_chains * n_params, np.float64)
+        keep["acc"] = np.zeros(n_iter * n_chains, np.uint8)
+        ext.trace_plp = _p(keep["plp"], C.c_double)
+        ext.trace_prop = _p(keep["prop"], C.c_double)
+        ext.trace_accept = _p(keep["acc"], C.c_uint8)
+        keep["init"] = np.zeros(n_chains, np.float64)
+        ext.init_log_post = _p(keep["init"], C.c_double)
+    if final_state:
+        keep["final"] = np.zeros(n_chains * n_params, np.float64)
+        ext.final_state = _p(keep["final"], C.c_double)
+    if tiling:
+        ext.chains_per_block, ext.waves_per_chain = tiling
+    ext.precompute_data = -1 if precompute_data is None else int(precompute_data)
+    ext.fuse = -1 if fuse is None else int(fuse)
+    return ext, keep
+
+
+def _finish(out: RunOutput, ext, keep, n_iter, n_chains, n_params):
+    out.kernel_ms = ext.kernel_ms
+    if "plp" in keep:
+        out.trace_plp = keep["plp"].reshape(n_iter, n_chains)
+        out.trace_prop = keep["prop"].reshape(n_iter, n_chains, n_params)
+        out.trace_accept = keep["acc"].reshape(n_iter, n_chains).astype(bool)
+        out.init_log_post = keep["init"]
+    if "final" in keep:
+        out.final_state = keep["final"].reshape(n_chains, n_params)
+    return out
+
+
+def run_chains(code, consts, prior_code, prior_consts, n_params, data, n_cols,
+               n_obs, init, proposal_sd, temperatures, n_iter, seed,
+               proposal_mode=0, gamma=0.0, de_noise=0.0, proposal_l=None,
+               trace=False, final_state=False, stream_id=0, tiling=None,
+               precompute_data=None, fuse=None) -> RunOutput:
+    """gpu_metropolis_chain (chain_kernel.rs:468-488) on the HIP backend."""
+    L = lib()
+    code, prior_code = _u32(code), _u32(prior_code)
+    consts, prior_consts = _f64(consts), _f64(prior_consts)
+    data, init = _f64(data), _f64(init)
+    sd, temps = _f64(proposal_sd), _f64(temperatures)
+    pl = None if proposal_l is None else _f64(proposal_l)
+    n_chains = init.size // n_params
+    ext, keep = _ext(n_iter, n_chains, n_params, trace, final_state, stream_id,
+                     tiling, precompute_data, fuse)
+    res = ChainResult()
+    rc = L.gpum_hip_run_chains_ex(
+        _p(code, C.c_uint32), code.size, _p(consts, C.c_double), consts.size,
+        _p(prior_code, C.c_uint32), prior_code.size,
+        _p(prior_consts, C.c_double), prior_consts.size, n_params,
+        _p(data, C.c_double), n_cols, n_obs, _p(init, C.c_double), n_chains,
+        _p(sd, C.c_double), _p(temps, C.c_double), n_iter, seed & 0xFFFFFFFF,
+        proposal_mode, gamma, de_noise, _p(pl, C.c_double), C.byref(ext),
+        C.byref(res))
+    _check(rc)
+    return _finish(_collect(res), ext, keep, n_iter, n_chains, n_params)
+
+
+def loglik_batch(code, consts, n_params, data, n_cols, n_obs, points):
+    """rust_loglik_batch (lib.rs:131-151) on the HIP backend."""
+    L = lib()
+    code, consts, data, points = _u32(code), _f64(consts), _f64(data), _f64(points)
+    n_points = points.size // n_params if n_params else 0
+    out = np.zeros(max(n_points, 1), np.float64)
+    rc = L.gpum_hip_loglik_batch(_p(code, C.c_uint32), code.size,
+                                 _p(consts, C.c_double), consts.size, n_params,
+                                 _p(data, C.c_double), n_cols, n_obs,
+                                 _p(points, C.c_double), n_points,
+                                 _p(out, C.c_double))
+    _check(rc)
+    return out[:n_points]
+
+
+class DeviceModel:
+    """A model whose lowered program and data stay on the device."""
+
+    def __init__(self, code, consts, prior_code, prior_consts, n_params, data,
+                 n_cols, n_obs, precompute_data=None, fuse=None):
+        L = lib()
+        self._L = L
+        code, prior_code = _u32(code), _u32(prior_code)
+        consts, prior_consts = _f64(consts), _f64(prior_consts)
+        data = _f64(data)
+        self.n_params = int(n_params)
+        h = L.gpum_hip_model_create(
+            _p(code, C.c_uint32), code.size, _p(consts, C.c_double),
+            consts.size, _p(prior_code, C.c_uint32), prior_code.size,
+            _p(prior_consts, C.c_double), prior_consts.size, n_params,
+            _p(data, C.c_double), n_cols, n_obs,
+            -1 if precompute_data is None else int(precompute_data),
+            -1 if fuse is None else int(fuse))
+        if not h:
+            raise BackendError(last_error())
+        self._h = h
+
+    def info(self) -> dict:
+        inf = ModelInfo()
+        _check(self._L.gpum_hip_model_info(self._h, C.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in ModelInfo._fields_}
+
+    def run(self, init, proposal_sd, temperatures, n_iter, seed,
+            proposal_mode=0, gamma=0.0, de_noise=0.0, proposal_l=None,
+            trace=False, final_state=False, stream_id=0,
+            tiling=None) -> RunOutput:
+        np_ = self.n_params
+        init, sd, temps = _f64(init), _f64(proposal_sd), _f64(temperatures)
+        pl = None if proposal_l is None else _f64(proposal_l)
+        n_chains = init.size // np_
+        ext, keep = _ext(n_iter, n_chains, np_, trace, final_state, stream_id,
+                         tiling, None, None)
+        res = ChainResult()
+        rc = self._L.gpum_hip_model_run(
+            self._h, _p(init, C.c_double), n_chains, _p(sd, C.c_double),
+            _p(temps, C.c_double), n_iter, seed & 0xFFFFFFFF, proposal_mode,
+            gamma, de_noise, _p(pl, C.c_double), C.byref(ext), C.byref(res))
+        _check(rc)
+        return _finish(_collect(res), ext, keep, n_iter, n_chains, np_)
+
+    def loglik(self, points) -> np.ndarray:
+        points = _f64(points)
+        n = points.size // self.n_params
+        out = np.zeros(max(n, 1), np.float64)
+        _check(self._L.gpum_hip_model_loglik(self._h, _p(points, C.c_double),
+                                             n, _p(out, C.c_double)))
+        return out[:n]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.gpum_hip_model_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,53 @@
+"""Build the gfx950 backend library in-tree.
+
+``python -m gpumetropolis_amd.build`` compiles ``csrc/`` with hipcc into
+``gpumetropolis_amd/lib/libgpum_hip.so`` (device code for gfx950 only).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libgpum_hip.so")
+
+SOURCES = ["gpum_hip.cpp", "lower.cpp", "chain_kernel.hip"]
+HEADERS = ["vm.h", "lower.h", "chain_kernel.h", "device_vm.h", "math64.h",
+           "philox.h"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Wno-unused-result", "-Wno-unused-value",
+    # Packed FP32 arithmetic stays off (ordinary float code is emitted as
+    # scalar v_*_f32).
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    return "hipcc"
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, *[os.path.join(CSRC, s) for s in SOURCES],
+           "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libgpum_hip.so")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,88 @@
+// chain_kernel.h -- launch arguments of the fused chain kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vm.h"
+
+namespace gpum {
+
+struct ChainArgs {
+  // per-observation VM program (decoded, 4 words per instruction)
+  const VmIns4* vm;
+  uint32_t n_vm;
+  // FP64 hoist program, its constants, and the prior program
+  const uint32_t* hoist;
+  uint32_t n_hoist;  // words
+  const double* consts;
+  const uint32_t* prior;
+  uint32_t n_prior;  // words
+  const double* prior_consts;
+  // uniform table template (constants filled, hoisted slots 0), n_uniform_pad
+  const float* u_init;
+  uint32_t n_uniform_pad;
+  // data: FP32 column-major, column stride ld (floats), padded to n_pad
+  const float* data;
+  uint32_t n_cols;
+  uint32_t n_obs;
+  uint32_t n_pad;
+  uint32_t ld;
+  // chains
+  uint32_t n_chains;
+  uint32_t n_params;
+  const double* init;          // chain-major; also the DE snapshot
+  const double* proposal_sd;   // chain-major
+  const double* temperatures;  // n_chains
+  const double* proposal_l;    // n_chains * np * np or null
+  double gamma;
+  double de_noise;
+  uint32_t mode;
+  uint32_t n_iter;
+  uint32_t seed;
+  uint32_t stream_id;
+  // outputs
+  float* draws;
+  float* accept_rate;
+  double* last_log_post;
+  double* final_state;
+  double* init_log_post;
+  double* trace_plp;
+  double* trace_prop;
+  uint8_t* trace_accept;
+  // tiling
+  uint32_t cpb;  // chains per workgroup
+  uint32_t wpc;  // waves per chain
+  uint32_t tile_len;
+  uint32_t resident;
+  uint32_t lds_tile_offset;
+};
+
+// The values the data pass reads every iteration, passed by value so they
+// sit in scalar registers; everything else is read through the ChainArgs
+// pointer by the lead lane's step code.
+struct ChainHot {
+  const VmIns4* vm;  // decoded VM program: (op, slot, a, b) per instruction
+  const float* u_init;
+  const float* data;
+  uint32_t n_vm;
+  uint32_t n_uniform_pad;
+  uint32_t n_cols;
+  uint32_t n_obs;
+  uint32_t n_pad;
+  uint32_t ld;
+  uint32_t n_chains;
+  uint32_t n_iter;
+  uint32_t cpb;
+  uint32_t wpc;
+  uint32_t tile_len;
+  uint32_t resident;
+  uint32_t lds_tile_offset;
+  uint32_t state_words;
+};
+
+hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
+                               int ns, int opt, size_t lds_bytes,
+                               hipStream_t s);
+hipError_t set_chain_kernel_lds_limit(size_t bytes);
+
+}  // namespace gpum

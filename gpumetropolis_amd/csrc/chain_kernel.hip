@@ -1,0 +1,329 @@
+// chain_kernel.hip -- the fused Metropolis chain kernel for gfx950.
+//
+// Replaces metropolis_block_kernel (src/rust/src/chain_kernel.rs:122-357).
+// Layout of one workgroup: `cpb` chains, `wpc` waves per chain (at most 16
+// waves).  Per iteration:
+//   1. data pass: every wave evaluates its chain's VM program over its share
+//      of each LDS data tile (tiles stream from HBM through LDS and are
+//      shared by all chains of the workgroup, or stay resident when the
+//      whole data set fits), accumulating FP32 chunk sums into an FP64
+//      per-lane accumulator;
+//   2. reduction: FP64 cross-lane butterfly in the wave, then the chain's
+//      wave partials in LDS;
+//   3. fused step on the chain's lead lane, all in FP64: finish iteration
+//      t (prior + Metropolis decision log u < (plp - cur) / T, record the
+//      draw) and prepare t + 1 (Philox uniforms, Box-Muller, diagonal /
+//      Cholesky / DE proposal, hoisted parameter program -> uniform table,
+//      prior of the proposal).
+// The first data pass evaluates the initial state (cur = log-posterior of
+// init), exactly as the reference does before its loop.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "chain_kernel.h"
+#include "device_vm.h"
+#include "math64.h"
+#include "philox.h"
+
+namespace gpum {
+
+// FP64 stack evaluation of a reference program over the parameters (no
+// data), with the hoist extensions writing the uniform table.
+__device__ __noinline__ double eval_params64(const uint32_t* __restrict__ code,
+                                uint32_t n_words,
+                                const double* __restrict__ consts,
+                                const double* params, float* U) {
+  if (n_words < 2) return 0.0;
+  double st[kMaxStack + 1];
+  int sp = 0;
+  for (uint32_t pc = 0; pc < n_words / 2; ++pc) {
+    uint32_t op = code[2 * pc], arg = code[2 * pc + 1];
+    switch (op) {
+      case OP_PUSH_CONST: st[sp++] = consts[arg]; break;
+      case OP_PUSH_PARAM: st[sp++] = params[arg]; break;
+      case OP_ADD: st[sp - 2] += st[sp - 1]; --sp; break;
+      case OP_SUB: st[sp - 2] -= st[sp - 1]; --sp; break;
+      case OP_MUL: st[sp - 2] *= st[sp - 1]; --sp; break;
+      case OP_DIV: st[sp - 2] /= st[sp - 1]; --sp; break;
+      case OP_NEG: st[sp - 1] = -st[sp - 1]; break;
+      case OP_EXP: st[sp - 1] = exp(st[sp - 1]); break;
+      case OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
+      case OP_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
+      case OP_LGAMMA: st[sp - 1] = lgamma64(st[sp - 1]); break;
+      case OP_POW: st[sp - 2] = pow(st[sp - 2], st[sp - 1]); --sp; break;
+      case OP_STORE_U: U[arg] = (float)st[--sp]; break;
+      case OP_STORE_U_RCP: U[arg] = (float)(1.0 / st[--sp]); break;
+      case OP_STORE_U_NEG: U[arg] = (float)(-st[--sp]); break;
+      default: break;
+    }
+  }
+  return sp > 0 ? st[0] : 0.0;
+}
+
+// The uniforms of one chain-iteration, materialised four at a time.
+struct StepUniforms {
+  uint32_t seed, stream, t, chain;
+  U4 cache;
+  int cached = -1;
+  __device__ double get(uint32_t slot) {
+    int blk = (int)(slot >> 2);
+    if (blk != cached) {
+      cache = philox4x32_10(U4{t, chain, (uint32_t)blk, TAG_STEP}, seed, stream);
+      cached = blk;
+    }
+    return u32_to_unit(u4_word(cache, slot & 3u));
+  }
+};
+
+// Per-chain state in LDS (doubles): state[np], prop[np], z[np], cur,
+// prior_lp, accepts.
+__device__ __forceinline__ int state_words(int np) { return 3 * np + 3; }
+
+// The fused step of one chain, run by its lead lane between data passes:
+// finish iteration it-1 with the data term `ll` of the proposal, then
+// prepare iteration it (when it < n_iter).  FP64 throughout; the only FP32
+// values are the uniform-table entries the hoist program writes.
+__device__ __noinline__ void chain_step(const ChainArgs* __restrict__ A,
+                                        uint32_t it, uint32_t chain,
+                                        double* st, double ll, float* U) {
+  const int np = (int)A->n_params;
+  const uint32_t C = A->n_chains;
+  const uint32_t T_iter = A->n_iter;
+  const uint32_t mode = A->mode;
+  const uint32_t spi = mode == 1 ? 2u * np + 4u : 2u * np + 1u;
+  const double two_pi = 6.283185307179586;
+  double* state = st;
+  double* prop = st + np;
+  double* zbuf = st + 2 * np;
+  double& cur = st[3 * np];
+  double& prior_lp = st[3 * np + 1];
+  double& accepts = st[3 * np + 2];
+
+  double plp = ll + prior_lp;
+  if (it == 0) {
+    cur = plp;
+    if (A->init_log_post) A->init_log_post[chain] = plp;
+  } else {
+    // cpu_native.rs:785-799 / chain_kernel.rs:328-347
+    const uint32_t t = it - 1;
+    StepUniforms us{A->seed, A->stream_id, t, chain};
+    double u = us.get(spi - 1);
+    bool acc_ok = log(u) < (plp - cur) / A->temperatures[chain];
+    if (A->trace_plp) A->trace_plp[(size_t)t * C + chain] = plp;
+    if (A->trace_prop)
+      for (int j = 0; j < np; ++j)
+        A->trace_prop[((size_t)t * C + chain) * np + j] = prop[j];
+    if (acc_ok) {
+      for (int j = 0; j < np; ++j) state[j] = prop[j];
+      cur = plp;
+      accepts += 1.0;
+    }
+    if (A->trace_accept) A->trace_accept[(size_t)t * C + chain] = acc_ok ? 1 : 0;
+    for (int j = 0; j < np; ++j)
+      A->draws[t + (size_t)T_iter * (chain + (size_t)C * j)] = (float)state[j];
+  }
+  if (it < T_iter) {
+    // cpu_native.rs:726-784 / chain_kernel.rs:210-296
+    const uint32_t t = it;
+    StepUniforms us{A->seed, A->stream_id, t, chain};
+    const double* sd = A->proposal_sd + (size_t)chain * np;
+    if (mode == 0) {
+      for (int j = 0; j < np; ++j) {
+        double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
+        double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+        prop[j] = state[j] + sd[j] * z;
+      }
+    } else if (mode == 2) {
+      const double* L = A->proposal_l + (size_t)chain * np * np;
+      for (int j = 0; j < np; ++j) {
+        double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
+        zbuf[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+      }
+      for (int k = 0; k < np; ++k) {
+        double a = state[k];
+        for (int j = 0; j <= k; ++j) a += L[k * np + j] * zbuf[j];
+        prop[k] = a;
+      }
+    } else {
+      // DE path A: the pair comes from the frozen batch-start population
+      // in `init`; with probability 0.1 the scale is 1.
+      uint32_t ia = (uint32_t)(us.get(0) * (double)C);
+      if (ia >= C) ia = C - 1;
+      uint32_t ib = (uint32_t)(us.get(1) * (double)C);
+      if (ib >= C) ib = C - 1;
+      if (ib == ia) ib = (ib + 1) % C;
+      double g = us.get(2) < 0.1 ? 1.0 : A->gamma;
+      for (int j = 0; j < np; ++j) {
+        double u1 = us.get(3 + 2 * j), u2 = us.get(4 + 2 * j);
+        double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+        double diff = A->init[(size_t)ia * np + j] - A->init[(size_t)ib * np + j];
+        prop[j] = state[j] + g * diff + A->de_noise * sd[j] * z;
+      }
+    }
+    eval_params64(A->hoist, A->n_hoist, A->consts, prop, U);
+    prior_lp = eval_params64(A->prior, A->n_prior, A->prior_consts, prop, nullptr);
+  }
+}
+
+__device__ __noinline__ void chain_init(const ChainArgs* __restrict__ A,
+                                        uint32_t chain, double* st, float* U) {
+  const int np = (int)A->n_params;
+  for (int j = 0; j < np; ++j) {
+    double v = A->init[(size_t)chain * np + j];
+    st[j] = v;
+    st[np + j] = v;
+  }
+  st[3 * np + 2] = 0.0;
+  eval_params64(A->hoist, A->n_hoist, A->consts, st + np, U);
+  st[3 * np + 1] = eval_params64(A->prior, A->n_prior, A->prior_consts, st + np, nullptr);
+}
+
+__device__ __noinline__ void chain_finish(const ChainArgs* __restrict__ A,
+                                          uint32_t chain, const double* st) {
+  const int np = (int)A->n_params;
+  const uint32_t T_iter = A->n_iter;
+  A->accept_rate[chain] =
+      T_iter == 0 ? 0.0f : (float)st[3 * np + 2] / (float)T_iter;
+  A->last_log_post[chain] = st[3 * np];
+  if (A->final_state)
+    for (int j = 0; j < np; ++j) A->final_state[(size_t)chain * np + j] = st[j];
+}
+
+template <int NS, int OPT>
+__global__ void __launch_bounds__(1024)
+chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int SW = (int)H.state_words;
+  const int cpb = (int)H.cpb, wpc = (int)H.wpc;
+  const int NU = (int)H.n_uniform_pad;
+  double* st_all = reinterpret_cast<double*>(smem);
+  double* partials = st_all + cpb * SW;
+  float* Uall = reinterpret_cast<float*>(partials + cpb * wpc);
+  float* tile = reinterpret_cast<float*>(smem + H.lds_tile_offset);
+
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int c_local = wave / wpc;
+  const int part = wave - c_local * wpc;
+  const uint32_t chain = blockIdx.x * (uint32_t)cpb + (uint32_t)c_local;
+  const bool active = chain < H.n_chains;
+  const bool lead = active && part == 0 && lane == 0;
+  double* st = st_all + c_local * SW;
+  float* U = Uall + c_local * NU;
+
+  for (int i = tid; i < cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
+  __syncthreads();
+  if (lead) chain_init(Ap, chain, st, U);
+
+  const uint32_t chunk = 64u * OPT;
+  const uint32_t tile_len = H.tile_len;
+  const uint32_t n4 = tile_len / 4;
+  const uint32_t ld4 = H.ld / 4;
+  const float4* src = reinterpret_cast<const float4*>(H.data);
+  float4* dst = reinterpret_cast<float4*>(tile);
+  if (H.resident) {
+    // The whole (padded) data set stays in LDS for every iteration.
+    for (uint32_t i = tid; i < H.n_cols * n4; i += blockDim.x) {
+      uint32_t col = i / n4, off = i - col * n4;
+      dst[i] = src[(size_t)col * ld4 + off];
+    }
+  }
+  __syncthreads();
+
+  for (uint32_t it = 0; it <= H.n_iter; ++it) {
+    // ---- data pass for the chain's current proposal -------------------
+    double acc = 0.0;
+    if (H.n_vm > 0) {
+      for (uint32_t t0 = 0; t0 < H.n_pad; t0 += tile_len) {
+        if (!H.resident) {
+          __syncthreads();
+          for (uint32_t i = tid; i < H.n_cols * n4; i += blockDim.x) {
+            uint32_t col = i / n4, off = i - col * n4;
+            dst[i] = src[(size_t)col * ld4 + t0 / 4 + off];
+          }
+          __syncthreads();
+        }
+        if (active) {
+          const uint32_t nchunks = tile_len / chunk;
+          for (uint32_t ck = part; ck < nchunks; ck += wpc) {
+            const uint32_t base = ck * chunk;
+            float T[OPT];
+            vm_run<NS, OPT>((vm_prog_t)H.vm, H.n_vm, U, tile + base, tile_len, lane, T);
+            const uint32_t first = t0 + base;
+            float s = 0.0f;
+            if (first + chunk <= H.n_obs) {
+#pragma unroll
+              for (int k = 0; k < OPT; ++k) s += T[k];
+            } else {
+#pragma unroll
+              for (int k = 0; k < OPT; ++k) {
+                uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
+                s += obs < H.n_obs ? T[k] : 0.0f;
+              }
+            }
+            acc += (double)s;
+          }
+        }
+      }
+    }
+    // ---- reduction ----------------------------------------------------
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) partials[wave] = acc;
+    __syncthreads();
+    // ---- fused step ---------------------------------------------------
+    if (lead) {
+      double ll = 0.0;
+      for (int p = 0; p < wpc; ++p) ll += partials[c_local * wpc + p];
+      chain_step(Ap, it, chain, st, ll, U);
+    }
+    __syncthreads();
+  }
+  if (lead) chain_finish(Ap, chain, st);
+}
+
+#define GPUM_INST(NS, OPT) template __global__ void chain_kernel<NS, OPT>(const ChainArgs*, ChainHot);
+GPUM_INST(1, 16)
+GPUM_INST(3, 16)
+GPUM_INST(7, 8)
+GPUM_INST(15, 4)
+GPUM_INST(31, 4)
+#undef GPUM_INST
+
+hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
+                               int ns, int opt, size_t lds_bytes,
+                               hipStream_t s) {
+  dim3 grid((h.n_chains + h.cpb - 1) / h.cpb);
+  dim3 block(h.cpb * h.wpc * 64);
+#define GPUM_CASE(NS_, OPT_)                                                  \
+  if (ns == NS_ && opt == OPT_) {                                             \
+    hipLaunchKernelGGL((chain_kernel<NS_, OPT_>), grid, block, lds_bytes, s, dev_args, h); \
+    return hipGetLastError();                                                 \
+  }
+  GPUM_CASE(1, 16)
+  GPUM_CASE(3, 16)
+  GPUM_CASE(7, 8)
+  GPUM_CASE(15, 4)
+  GPUM_CASE(31, 4)
+#undef GPUM_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t set_chain_kernel_lds_limit(size_t bytes) {
+  hipError_t e = hipSuccess;
+#define GPUM_ATTR(NS_, OPT_)                                                       \
+  e = hipFuncSetAttribute((const void*)chain_kernel<NS_, OPT_>,                   \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes); \
+  if (e != hipSuccess) return e;
+  GPUM_ATTR(1, 16)
+  GPUM_ATTR(3, 16)
+  GPUM_ATTR(7, 8)
+  GPUM_ATTR(15, 4)
+  GPUM_ATTR(31, 4)
+#undef GPUM_ATTR
+  return e;
+}
+
+}  // namespace gpum

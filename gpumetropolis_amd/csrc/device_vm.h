@@ -1,0 +1,265 @@
+// device_vm.h -- the wavefront-uniform interpreter of the per-observation
+// VM (vm.h) for gfx950.
+//
+// One wave evaluates the program of one chain over a chunk of 64 * OPT
+// observations: lane l holds observations [4*l + 256*g + j] (g < OPT/4,
+// j < 4) of the chunk, read from the LDS data tile with 128-bit loads.  The
+// instruction stream is read with scalar loads (the program pointer and pc
+// are uniform), the operands of every case are statically indexed
+// registers: T[OPT] is the cached top of stack, S[NS][OPT] the spill slots.
+// Uniform operands (hoisted parameter values and constants) come from the
+// chain's table in LDS as broadcast reads.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "vm.h"
+
+namespace gpum {
+
+// ln(2) in FP32 for natural logs from the hardware log2.
+#define GPUM_LN2F 0.69314718055994531f
+#define GPUM_LOG2EF 1.4426950408889634f
+
+__device__ __forceinline__ float vm_exp(float x) {
+  return __builtin_amdgcn_exp2f(x * GPUM_LOG2EF);
+}
+__device__ __forceinline__ float vm_log(float x) {
+  return __builtin_amdgcn_logf(x) * GPUM_LN2F;
+}
+__device__ __forceinline__ float vm_rcp(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+__device__ __forceinline__ float vm_sqrt(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+
+// pow with the sign rules of C pow for a negative base and an integral
+// exponent; a negative base with a non-integral exponent gives NaN.
+__device__ __forceinline__ float vm_pow(float b, float e) {
+  float ab = fabsf(b);
+  float r = __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(ab));
+  float et = truncf(e);
+  bool e_int = et == e;
+  // odd integral exponent: e/2 is not integral (exact for |e| < 2^24)
+  bool e_odd = e_int && (truncf(e * 0.5f) != e * 0.5f);
+  r = (b < 0.0f && e_odd) ? -r : r;
+  r = (b < 0.0f && !e_int) ? __builtin_nanf("") : r;
+  r = (e == 0.0f || b == 1.0f) ? 1.0f : r;
+  return r;
+}
+
+// FP32 log-gamma for the data-dependent lgamma opcode: shift the argument up
+// past 8 with lgamma(x) = lgamma(x + 1) - ln x (accumulated as one product
+// and one log), then Stirling's series with three correction terms
+// (truncation error below 3e-10 at x >= 8); reflection below 0.5.
+__device__ __forceinline__ float vm_lgamma(float xin) {
+  bool refl = xin < 0.5f;
+  float x = refl ? 1.0f - xin : xin;
+  float prod = 1.0f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    bool sh = x < 8.0f;
+    prod = sh ? prod * x : prod;
+    x = sh ? x + 1.0f : x;
+  }
+  float inv = vm_rcp(x);
+  float inv2 = inv * inv;
+  float series =
+      inv * (0.083333333333333333f -
+             inv2 * (0.0027777777777777778f - inv2 * 0.00079365079365079365f));
+  float lx = vm_log(x);
+  float r = (x - 0.5f) * lx - x + 0.91893853320467274f + series - vm_log(prod);
+  if (refl) {
+    const float pi = 3.14159265358979f;
+    float s = __builtin_amdgcn_sinf(xin * 0.5f);  // sin(2*pi*t) with t = xin/2
+    r = vm_log(fabsf(pi * vm_rcp(s))) - r;
+  }
+  return r;
+}
+
+// log(1 + exp(x)) without overflow.
+__device__ __forceinline__ float vm_softplus(float x) {
+  return fmaxf(x, 0.0f) + vm_log(1.0f + vm_exp(-fabsf(x)));
+}
+
+template <int OPT>
+__device__ __forceinline__ void vm_load_col(const float* __restrict__ col,
+                                            int lane, float (&v)[OPT]) {
+  const float4* p = reinterpret_cast<const float4*>(col) + lane;
+#pragma unroll
+  for (int g = 0; g < OPT / 4; ++g) {
+    float4 x = p[g * 64];
+    v[4 * g + 0] = x.x;
+    v[4 * g + 1] = x.y;
+    v[4 * g + 2] = x.z;
+    v[4 * g + 3] = x.w;
+  }
+}
+
+template <int I, int NS, int OPT>
+__device__ __forceinline__ void vm_spill(uint32_t slot, float (&S)[NS][OPT],
+                                         const float (&T)[OPT]) {
+  if constexpr (I < NS) {
+    if (slot == I) {
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) S[I][k] = T[k];
+    } else {
+      vm_spill<I + 1, NS, OPT>(slot, S, T);
+    }
+  }
+}
+
+template <int I, int NS, int OPT, typename F>
+__device__ __forceinline__ void vm_bin_s(uint32_t slot, const float (&S)[NS][OPT],
+                                         float (&T)[OPT], F f) {
+  if constexpr (I < NS) {
+    if (slot == I) {
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) T[k] = f(S[I][k], T[k]);
+    } else {
+      vm_bin_s<I + 1, NS, OPT>(slot, S, T, f);
+    }
+  }
+}
+
+// Evaluate the program on one chunk; `tile` points at the chunk's first
+// observation of column 0 and `ld` is the column stride of the tile (floats).
+// The program lives in the constant address space: its address and the pc
+// are wave-uniform, so every fetch is a scalar load into SGPRs.
+typedef const __attribute__((address_space(4))) VmIns4* vm_prog_t;
+
+template <int NS, int OPT>
+__device__ __forceinline__ void vm_run(vm_prog_t prog,
+                                       uint32_t n_instr,
+                                       const float* __restrict__ U,
+                                       const float* __restrict__ tile,
+                                       uint32_t ld, int lane, float (&T)[OPT]) {
+  float S[NS][OPT];
+  float D[OPT];
+#pragma unroll
+  for (int k = 0; k < OPT; ++k) T[k] = 0.0f;
+  for (uint32_t pc = 0; pc < n_instr; ++pc) {
+    const uint32_t op = prog[pc].op, slot = prog[pc].slot, a = prog[pc].a,
+                   b = prog[pc].b;
+    switch (op) {
+      case VM_LOAD_U: {
+        float u = U[a];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[k] = u;
+        break;
+      }
+      case VM_LOAD_D:
+        vm_load_col<OPT>(tile + a * ld, lane, T);
+        break;
+      case VM_PUSH_U: {
+        vm_spill<0, NS, OPT>(slot, S, T);
+        float u = U[a];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[k] = u;
+        break;
+      }
+      case VM_PUSH_D:
+        vm_spill<0, NS, OPT>(slot, S, T);
+        vm_load_col<OPT>(tile + a * ld, lane, T);
+        break;
+      case VM_ADD_S:
+        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s + t; });
+        break;
+      case VM_SUB_S:
+        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s - t; });
+        break;
+      case VM_MUL_S:
+        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s * t; });
+        break;
+      case VM_DIV_S:
+        vm_bin_s<0, NS, OPT>(slot, S, T,
+                             [](float s, float t) { return s * vm_rcp(t); });
+        break;
+      case VM_POW_S:
+        vm_bin_s<0, NS, OPT>(slot, S, T,
+                             [](float s, float t) { return vm_pow(s, t); });
+        break;
+#define GPUM_UOP(NAME, EXPR)                          \
+  case NAME: {                                        \
+    float u = U[a];                                   \
+    _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
+      float t = T[k];                                 \
+      T[k] = (EXPR);                                  \
+    }                                                 \
+    break;                                            \
+  }
+      GPUM_UOP(VM_ADD_U, t + u)
+      GPUM_UOP(VM_SUB_U, t - u)
+      GPUM_UOP(VM_MUL_U, t * u)
+      GPUM_UOP(VM_POW_U, vm_pow(t, u))
+      GPUM_UOP(VM_RSUB_U, u - t)
+      GPUM_UOP(VM_RDIV_U, u * vm_rcp(t))
+      GPUM_UOP(VM_RPOW_U, vm_pow(u, t))
+#undef GPUM_UOP
+#define GPUM_DOP(NAME, EXPR)                          \
+  case NAME: {                                        \
+    vm_load_col<OPT>(tile + a * ld, lane, D);         \
+    _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
+      float t = T[k];                                 \
+      float d = D[k];                                 \
+      T[k] = (EXPR);                                  \
+    }                                                 \
+    break;                                            \
+  }
+      GPUM_DOP(VM_ADD_D, t + d)
+      GPUM_DOP(VM_SUB_D, t - d)
+      GPUM_DOP(VM_MUL_D, t * d)
+      GPUM_DOP(VM_DIV_D, t * vm_rcp(d))
+      GPUM_DOP(VM_POW_D, vm_pow(t, d))
+      GPUM_DOP(VM_RSUB_D, d - t)
+      GPUM_DOP(VM_RDIV_D, d * vm_rcp(t))
+      GPUM_DOP(VM_RPOW_D, vm_pow(d, t))
+#undef GPUM_DOP
+#define GPUM_UNOP(NAME, EXPR)                         \
+  case NAME: {                                        \
+    _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
+      float t = T[k];                                 \
+      T[k] = (EXPR);                                  \
+    }                                                 \
+    break;                                            \
+  }
+      GPUM_UNOP(VM_NEG, -t)
+      GPUM_UNOP(VM_EXP, vm_exp(t))
+      GPUM_UNOP(VM_LOG, vm_log(t))
+      GPUM_UNOP(VM_SQRT, vm_sqrt(t))
+      GPUM_UNOP(VM_LGAMMA, vm_lgamma(t))
+      GPUM_UNOP(VM_SQR, t * t)
+      GPUM_UNOP(VM_CUBE, t * t * t)
+      GPUM_UNOP(VM_POW4, (t * t) * (t * t))
+      GPUM_UNOP(VM_RCP, vm_rcp(t))
+      GPUM_UNOP(VM_SOFTPLUS, vm_softplus(t))
+#undef GPUM_UNOP
+      case VM_FMA_UU: {
+        float ua = U[a], ub = U[b];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], ua, ub);
+        break;
+      }
+      case VM_FMA_UD: {
+        float ua = U[a];
+        vm_load_col<OPT>(tile + b * ld, lane, D);
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[k] = fmaf(ua, D[k], T[k]);
+        break;
+      }
+      case VM_FMA_DU: {
+        float ub = U[b];
+        vm_load_col<OPT>(tile + a * ld, lane, D);
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], D[k], ub);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+}
+
+}  // namespace gpum

@@ -1,0 +1,743 @@
+// gpum_hip.cpp -- host runtime behind the C ABI in include/gpum_hip.h.
+//
+// The HIP replacement of the reference's CubeCL host launcher
+// (src/rust/src/chain_kernel.rs:371-532): validate and lower the bytecode,
+// lay the data out for the device (FP32, column-major, padded per tile,
+// precomputed data-only columns appended), pick the tiling, launch the fused
+// chain kernel on the library's stream, and return the reference's result
+// layout.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpum_hip.h"
+#include "chain_kernel.h"
+#include "lower.h"
+
+using namespace gpum;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+// A HIP failure as a message carrying the phrases the R guard matches.
+void set_hip_err(const char* what, hipError_t e) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    set_err("%s: out of memory on the device (%s)", what, hipGetErrorString(e));
+  else if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+    set_err("%s: no usable device (%s)", what, hipGetErrorString(e));
+  else
+    set_err("%s: kernel/device error (%s)", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(what, expr)            \
+  do {                                 \
+    hipError_t e_ = (expr);            \
+    if (e_ != hipSuccess) {            \
+      set_hip_err(what, e_);           \
+      return false;                    \
+    }                                  \
+  } while (0)
+
+struct Device {
+  bool ready = false;
+  bool ok = false;
+  int id = 0;
+  int n_cu = 0;
+  size_t lds_max = 0;
+  hipStream_t stream = nullptr;
+  std::string why;
+};
+
+std::mutex g_mu;  // serialises device work of the library
+Device g_dev;
+
+bool device_init() {
+  if (g_dev.ready) {
+    if (!g_dev.ok) set_err("%s", g_dev.why.c_str());
+    return g_dev.ok;
+  }
+  g_dev.ready = true;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    g_dev.why = "no HIP device available";
+    set_err("%s", g_dev.why.c_str());
+    return false;
+  }
+  int id = 0;
+  hipGetDevice(&id);
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, id) != hipSuccess) {
+    g_dev.why = "cannot query the HIP device";
+    set_err("%s", g_dev.why.c_str());
+    return false;
+  }
+  if (strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+    g_dev.why = std::string("device ") + p.gcnArchName +
+                " is not gfx950 (MI355X); this backend is built for gfx950 only";
+    set_err("%s", g_dev.why.c_str());
+    return false;
+  }
+  g_dev.id = id;
+  g_dev.n_cu = p.multiProcessorCount;
+  g_dev.lds_max = p.sharedMemPerBlock;
+  if (g_dev.lds_max < 65536) g_dev.lds_max = 65536;
+  if (g_dev.lds_max > 160 * 1024) g_dev.lds_max = 160 * 1024;
+  if (set_chain_kernel_lds_limit(g_dev.lds_max) != hipSuccess) {
+    g_dev.lds_max = 65536;
+  }
+  if (hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking) != hipSuccess) {
+    g_dev.why = "cannot create a HIP stream on the device";
+    set_err("%s", g_dev.why.c_str());
+    return false;
+  }
+  g_dev.ok = true;
+  return true;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  bool alloc(size_t count, const char* what) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_hip_err(what, e);
+      return false;
+    }
+    n = count;
+    return true;
+  }
+  bool upload(const T* src, size_t count, const char* what) {
+    if (!alloc(count, what)) return false;
+    if (count == 0 || src == nullptr) return true;
+    hipError_t e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      set_hip_err(what, e);
+      return false;
+    }
+    return true;
+  }
+};
+
+struct Variant {
+  int ns, opt;
+};
+
+Variant pick_variant(int vm_depth) {
+  int need = vm_depth > 1 ? vm_depth - 1 : 0;
+  if (need <= 1) return {1, 16};
+  if (need <= 3) return {3, 16};
+  if (need <= 7) return {7, 8};
+  if (need <= 15) return {15, 4};
+  return {31, 4};
+}
+
+uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct gpum_hip_model {
+  Lowered L;
+  uint32_t n_params = 0;
+  uint32_t n_cols_total = 0;
+  uint32_t n_obs = 0;
+  // host copies needed to lay the data out per tiling
+  std::vector<float> cols;  // column-major, n_obs per column, unpadded
+  // device program
+  DevBuf<uint32_t> vm, hoist, prior;
+  DevBuf<double> consts, prior_consts;
+  DevBuf<float> u_init;
+  uint32_t n_uniform_pad = 1;
+  std::vector<uint32_t> prior_code;
+  // device data, laid out for `data_ld`
+  DevBuf<float> data;
+  uint32_t data_ld = 0;
+  uint32_t data_chunk = 0;  // layout granularity the data was padded to
+};
+
+namespace {
+
+bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
+                 const double* consts, size_t n_consts,
+                 const uint32_t* prior_code, size_t prior_len,
+                 const double* prior_consts, size_t n_prior_consts,
+                 uint32_t n_params, const double* data, uint32_t n_cols,
+                 uint32_t n_obs, int32_t precompute, int32_t fuse) {
+  if (n_params == 0 || n_params > (uint32_t)kMaxParams) {
+    set_err("n_params must be in [1, %d] on the hip backend (got %u)",
+            kMaxParams, n_params);
+    return false;
+  }
+  if (n_cols > 0 && n_obs > 0 && data == nullptr) {
+    set_err("data is NULL but n_obs * n_cols > 0");
+    return false;
+  }
+  if (n_consts > 0 && consts == nullptr) {
+    set_err("consts is NULL");
+    return false;
+  }
+  if (code_len > 0 && code == nullptr) {
+    set_err("code is NULL");
+    return false;
+  }
+  int pdepth = 0;
+  std::string e = validate_program(prior_code, prior_len, n_prior_consts,
+                                   n_params, 0, false, &pdepth);
+  if (!e.empty()) {
+    set_err("prior: %s", e.c_str());
+    return false;
+  }
+  LowerOptions opt;
+  opt.precompute_data = precompute != 0;
+  opt.fuse = fuse != 0;
+  e = lower_program(code, code_len, consts, n_consts, n_params, n_cols, opt,
+                    &m->L);
+  if (!e.empty()) {
+    set_err("log-likelihood: %s", e.c_str());
+    return false;
+  }
+  if (m->L.n_uniform > kMaxUniform) {
+    set_err("too many uniform operands");
+    return false;
+  }
+  m->n_params = n_params;
+  m->n_obs = n_obs;
+  // Columns: originals cast to FP32 (as the reference's GPU path casts,
+  // chain_kernel.rs:501-505), then precomputed data-only subtrees evaluated
+  // in FP64 on the original f64 values and rounded once.
+  const uint32_t nx = (uint32_t)m->L.extra_cols.size();
+  m->n_cols_total = n_cols + nx;
+  if (m->L.empty) m->n_cols_total = 0;
+  m->cols.assign((size_t)m->n_cols_total * n_obs, 0.0f);
+  if (!m->L.empty) {
+    for (uint32_t i = 0; i < n_obs; ++i) {
+      const double* row = data + (size_t)i * n_cols;
+      for (uint32_t c = 0; c < n_cols; ++c)
+        m->cols[(size_t)c * n_obs + i] = (float)row[c];
+      for (uint32_t k = 0; k < nx; ++k) {
+        const std::vector<uint32_t>& pc = m->L.extra_cols[k].code;
+        double v = eval_ref64(pc.data(), pc.size(), m->L.consts.data(),
+                              nullptr, row);
+        m->cols[(size_t)(n_cols + k) * n_obs + i] = (float)v;
+      }
+    }
+  }
+  // Device program.
+  std::vector<uint32_t> vm4;
+  for (const VmInstr& ins : m->L.vm) {
+    vm4.push_back(vm_op(ins));
+    vm4.push_back(vm_slot(ins));
+    vm4.push_back(vm_a(ins));
+    vm4.push_back(vm_b(ins));
+  }
+  m->n_uniform_pad = (uint32_t)std::max(1, m->L.n_uniform);
+  std::vector<float> u0(m->n_uniform_pad, 0.0f);
+  for (auto& kv : m->L.const_uniforms) u0[kv.first] = kv.second;
+  m->prior_code.assign(prior_code, prior_code + prior_len);
+  if (!m->vm.upload(vm4.data(), vm4.size(), "upload program")) return false;
+  if (!m->hoist.upload(m->L.hoist.data(), m->L.hoist.size(), "upload program"))
+    return false;
+  if (!m->consts.upload(m->L.consts.data(), m->L.consts.size(), "upload consts"))
+    return false;
+  if (!m->prior.upload(prior_code, prior_len, "upload prior")) return false;
+  if (!m->prior_consts.upload(prior_consts, n_prior_consts, "upload prior"))
+    return false;
+  if (!m->u_init.upload(u0.data(), u0.size(), "upload uniforms")) return false;
+  return true;
+}
+
+struct Tiling {
+  Variant v;
+  uint32_t cpb, wpc, chunk, tile_len, n_pad, resident, lds_tile_offset;
+  size_t lds_bytes;
+};
+
+bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
+                 const gpum_run_ext* ext, Tiling* t) {
+  t->v = pick_variant(m->L.vm_depth);
+  const uint32_t opt = (uint32_t)t->v.opt;
+  t->chunk = 64u * opt;
+  const uint32_t ncu = (uint32_t)g_dev.n_cu;
+  uint32_t wpc, cpb;
+  if (n_chains >= 16u * ncu) {
+    wpc = 1;
+  } else {
+    uint32_t w = 16u * ncu / std::max(1u, n_chains);
+    wpc = 1;
+    while (wpc * 2 <= w && wpc * 2 <= 16) wpc *= 2;
+  }
+  // No more waves per chain than there are chunks to share.
+  uint32_t nchunks = std::max(1u, (m->n_obs + t->chunk - 1) / t->chunk);
+  while (wpc > 1 && wpc > nchunks) wpc /= 2;
+  cpb = std::max(1u, 16u / wpc);
+  if (cpb > n_chains) cpb = std::max(1u, n_chains);
+  if (ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4) {
+    if (ext->waves_per_chain > 0) wpc = (uint32_t)ext->waves_per_chain;
+    if (ext->chains_per_block > 0) cpb = (uint32_t)ext->chains_per_block;
+  }
+  if (cpb * wpc > 16 || cpb == 0 || wpc == 0) {
+    set_err("invalid tiling: chains_per_block * waves_per_chain must be <= 16");
+    return false;
+  }
+  t->cpb = cpb;
+  t->wpc = wpc;
+  const uint32_t ncol = std::max(1u, m->n_cols_total);
+  const size_t state_bytes = (size_t)cpb * (3 * m->n_params + 3) * 8;
+  const size_t part_bytes = (size_t)cpb * wpc * 8;
+  const size_t u_bytes = (size_t)cpb * m->n_uniform_pad * 4;
+  size_t head = (state_bytes + part_bytes + u_bytes + 15) / 16 * 16;
+  t->lds_tile_offset = (uint32_t)head;
+  const size_t budget = g_dev.lds_max - head;
+  const uint32_t step = t->chunk * wpc;  // one chunk per wave of a chain
+  uint32_t n_res = round_up(std::max(1u, m->n_obs), step);
+  if ((size_t)n_res * ncol * 4 <= std::min(budget, (size_t)96 * 1024)) {
+    t->resident = 1;
+    t->tile_len = n_res;
+    t->n_pad = n_res;
+  } else {
+    t->resident = 0;
+    uint32_t mult = (uint32_t)std::max<size_t>(
+        1, (size_t)16384 / ((size_t)ncol * step));
+    t->tile_len = step * mult;
+    if ((size_t)t->tile_len * ncol * 4 > budget) {
+      set_err("data rows too wide for the LDS tile (%u columns)", ncol);
+      return false;
+    }
+    t->n_pad = round_up(std::max(1u, m->n_obs), t->tile_len);
+  }
+  t->lds_bytes = head + (size_t)t->tile_len * ncol * 4;
+  return true;
+}
+
+bool ensure_data(gpum_hip_model* m, uint32_t n_pad) {
+  if (m->data.p && m->data_ld == n_pad) return true;
+  const uint32_t ncol = std::max(1u, m->n_cols_total);
+  std::vector<float> h((size_t)ncol * n_pad, 0.0f);
+  for (uint32_t c = 0; c < m->n_cols_total; ++c)
+    memcpy(&h[(size_t)c * n_pad], &m->cols[(size_t)c * m->n_obs],
+           (size_t)m->n_obs * 4);
+  if (!m->data.upload(h.data(), h.size(), "upload data")) return false;
+  m->data_ld = n_pad;
+  return true;
+}
+
+bool validate_run(const gpum_hip_model* m, const double* init,
+                  uint32_t n_chains, const double* proposal_sd,
+                  const double* temperatures, uint32_t proposal_mode,
+                  const double* proposal_l) {
+  if (n_chains == 0) {
+    set_err("n_chains must be positive");
+    return false;
+  }
+  if (!init || !proposal_sd || !temperatures) {
+    set_err("init, proposal_sd and temperatures must not be NULL");
+    return false;
+  }
+  if (proposal_mode == 3) {
+    set_err("proposal_mode 3 (MALA) runs on the cpu backend; the hip "
+            "backend implements modes 0, 1 and 2");
+    return false;
+  }
+  if (proposal_mode > 3) {
+    set_err("unknown proposal_mode %u", proposal_mode);
+    return false;
+  }
+  if (proposal_mode == 2 && proposal_l == nullptr) {
+    set_err("proposal_mode 2 needs proposal_l");
+    return false;
+  }
+  if (proposal_mode == 1 && n_chains < 2) {
+    set_err("proposal_mode 1 (DE) needs at least 2 chains");
+    return false;
+  }
+  for (uint32_t c = 0; c < n_chains; ++c) {
+    if (!(temperatures[c] > 0.0) || !isfinite(temperatures[c])) {
+      set_err("temperatures must be positive and finite");
+      return false;
+    }
+  }
+  (void)m;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpum_hip_available(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return device_init() ? 1 : 0;
+}
+
+const char* gpum_hip_last_error(void) { return g_err.c_str(); }
+
+void gpum_hip_result_free(gpum_chain_result* r) {
+  if (!r) return;
+  free(r->draws);
+  free(r->accept_rate);
+  free(r->last_log_post);
+  r->draws = nullptr;
+  r->accept_rate = nullptr;
+  r->last_log_post = nullptr;
+}
+
+gpum_hip_model* gpum_hip_model_create(
+    const uint32_t* code, size_t code_len, const double* consts,
+    size_t n_consts, const uint32_t* prior_code, size_t prior_len,
+    const double* prior_consts, size_t n_prior_consts, uint32_t n_params,
+    const double* data, uint32_t n_cols, uint32_t n_obs,
+    int32_t precompute_data, int32_t fuse) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!device_init()) return nullptr;
+  hipSetDevice(g_dev.id);
+  gpum_hip_model* m = new gpum_hip_model();
+  if (!model_build(m, code, code_len, consts, n_consts, prior_code, prior_len,
+                   prior_consts, n_prior_consts, n_params, data, n_cols, n_obs,
+                   precompute_data, fuse)) {
+    delete m;
+    return nullptr;
+  }
+  return m;
+}
+
+void gpum_hip_model_free(gpum_hip_model* m) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  delete m;
+}
+
+int gpum_hip_model_info(const gpum_hip_model* m, gpum_model_info* info) {
+  if (!m || !info) {
+    set_err("null model or info");
+    return 1;
+  }
+  info->n_vm_instr = (uint32_t)m->L.vm.size();
+  info->vm_depth = (uint32_t)m->L.vm_depth;
+  info->ref_depth = (uint32_t)m->L.ref_depth;
+  info->n_uniform = (uint32_t)m->L.n_uniform;
+  info->n_cols_total = m->n_cols_total;
+  info->n_hoist_words = (uint32_t)m->L.hoist.size();
+  info->w_exec = m->L.w_exec;
+  info->w_bytecode = m->L.w_bytecode;
+  return 0;
+}
+
+static int model_run_locked(gpum_hip_model* m, const double* init,
+                            uint32_t n_chains, const double* proposal_sd,
+                            const double* temperatures, uint32_t n_iter,
+                            uint32_t seed, uint32_t proposal_mode,
+                            double gamma, double de_noise,
+                            const double* proposal_l, gpum_run_ext* ext,
+                            gpum_chain_result* out, double* loglik_out,
+                            bool with_prior = true) {
+  if (!out && !loglik_out) {
+    set_err("null result");
+    return 1;
+  }
+  if (!validate_run(m, init, n_chains, proposal_sd, temperatures,
+                    proposal_mode, proposal_l))
+    return 1;
+  hipSetDevice(g_dev.id);
+  const uint32_t np = m->n_params;
+  Tiling t;
+  if (!pick_tiling(m, n_chains, ext, &t)) return 1;
+  if (!ensure_data(m, t.n_pad)) return 1;
+
+  const size_t ncp = (size_t)n_chains * np;
+  const size_t n_draws = (size_t)n_iter * ncp;
+  DevBuf<double> d_init, d_sd, d_temp, d_l, d_lp, d_final, d_tplp, d_tprop, d_tinit;
+  DevBuf<float> d_draws, d_acc;
+  DevBuf<uint8_t> d_tacc;
+  DevBuf<ChainArgs> d_args;
+  auto fail = [&]() { return 1; };
+  if (!d_init.upload(init, ncp, "upload init")) return fail();
+  if (!d_sd.upload(proposal_sd, ncp, "upload proposal_sd")) return fail();
+  if (!d_temp.upload(temperatures, n_chains, "upload temperatures")) return fail();
+  if (proposal_mode == 2 &&
+      !d_l.upload(proposal_l, ncp * np, "upload proposal_l"))
+    return fail();
+  if (!d_draws.alloc(n_draws, "allocate draws")) return fail();
+  if (!d_acc.alloc(n_chains, "allocate accept_rate")) return fail();
+  if (!d_lp.alloc(n_chains, "allocate last_log_post")) return fail();
+  const bool want_final = ext && ext->final_state;
+  const bool want_tplp = ext && ext->trace_plp;
+  const bool want_tprop = ext && ext->trace_prop;
+  const bool want_tacc = ext && ext->trace_accept;
+  const bool want_tinit = ext && ext->init_log_post;
+  if (want_final && !d_final.alloc(ncp, "allocate final_state")) return fail();
+  if (want_tplp && !d_tplp.alloc((size_t)n_iter * n_chains, "allocate trace")) return fail();
+  if (want_tprop && !d_tprop.alloc(n_draws, "allocate trace")) return fail();
+  if (want_tacc && !d_tacc.alloc((size_t)n_iter * n_chains, "allocate trace")) return fail();
+  if (want_tinit && !d_tinit.alloc(n_chains, "allocate trace")) return fail();
+
+  ChainArgs a;
+  memset(&a, 0, sizeof a);
+  a.vm = reinterpret_cast<const VmIns4*>(m->vm.p);
+  a.n_vm = (uint32_t)m->L.vm.size();
+  a.hoist = m->hoist.p;
+  a.n_hoist = (uint32_t)m->L.hoist.size();
+  a.consts = m->consts.p;
+  a.prior = m->prior.p;
+  a.n_prior = with_prior ? (uint32_t)m->prior_code.size() : 0u;
+  a.prior_consts = m->prior_consts.p;
+  a.u_init = m->u_init.p;
+  a.n_uniform_pad = m->n_uniform_pad;
+  a.data = m->data.p;
+  a.n_cols = m->n_cols_total;
+  a.n_obs = m->n_obs;
+  a.n_pad = t.n_pad;
+  a.ld = m->data_ld;
+  a.n_chains = n_chains;
+  a.n_params = np;
+  a.init = d_init.p;
+  a.proposal_sd = d_sd.p;
+  a.temperatures = d_temp.p;
+  a.proposal_l = proposal_mode == 2 ? d_l.p : nullptr;
+  a.gamma = gamma;
+  a.de_noise = de_noise;
+  a.mode = proposal_mode;
+  a.n_iter = n_iter;
+  a.seed = seed;
+  a.stream_id = ext ? ext->stream_id : 0;
+  a.draws = d_draws.p;
+  a.accept_rate = d_acc.p;
+  a.last_log_post = d_lp.p;
+  a.final_state = want_final ? d_final.p : nullptr;
+  a.trace_plp = want_tplp ? d_tplp.p : nullptr;
+  a.trace_prop = want_tprop ? d_tprop.p : nullptr;
+  a.trace_accept = want_tacc ? d_tacc.p : nullptr;
+  a.init_log_post = want_tinit ? d_tinit.p : nullptr;
+  a.cpb = t.cpb;
+  a.wpc = t.wpc;
+  a.tile_len = t.tile_len;
+  a.resident = t.resident;
+  a.lds_tile_offset = t.lds_tile_offset;
+  if (!d_args.upload(&a, 1, "upload launch arguments")) return fail();
+
+  ChainHot h;
+  memset(&h, 0, sizeof h);
+  h.vm = a.vm;
+  h.u_init = a.u_init;
+  h.data = a.data;
+  h.n_vm = m->L.empty ? 0 : a.n_vm;
+  h.n_uniform_pad = a.n_uniform_pad;
+  h.n_cols = a.n_cols;
+  h.n_obs = a.n_obs;
+  h.n_pad = a.n_pad;
+  h.ld = a.ld;
+  h.n_chains = n_chains;
+  h.n_iter = n_iter;
+  h.cpb = t.cpb;
+  h.wpc = t.wpc;
+  h.tile_len = t.tile_len;
+  h.resident = t.resident;
+  h.lds_tile_offset = t.lds_tile_offset;
+  h.state_words = 3 * np + 3;
+
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, g_dev.stream);
+  hipError_t e = launch_chain_kernel(d_args.p, h, t.v.ns, t.v.opt, t.lds_bytes,
+                                     g_dev.stream);
+  hipEventRecord(e1, g_dev.stream);
+  if (e != hipSuccess) {
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    set_hip_err("chain kernel launch", e);
+    return 1;
+  }
+  e = hipStreamSynchronize(g_dev.stream);
+  if (e != hipSuccess) {
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    set_hip_err("chain kernel", e);
+    return 1;
+  }
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ext && ext->struct_size >= offsetof(gpum_run_ext, kernel_ms) + sizeof(double))
+    ext->kernel_ms = ms;
+
+  if (loglik_out) {
+    e = hipMemcpy(loglik_out, d_lp.p, n_chains * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      set_hip_err("read results", e);
+      return 1;
+    }
+    return 0;
+  }
+  out->n_iter = n_iter;
+  out->n_chains = n_chains;
+  out->n_params = np;
+  out->draws = (float*)malloc(std::max<size_t>(1, n_draws) * sizeof(float));
+  out->accept_rate = (float*)malloc(n_chains * sizeof(float));
+  out->last_log_post = (double*)malloc(n_chains * sizeof(double));
+  if (!out->draws || !out->accept_rate || !out->last_log_post) {
+    gpum_hip_result_free(out);
+    set_err("host out of memory for results");
+    return 1;
+  }
+  bool ok = hipMemcpy(out->draws, d_draws.p, n_draws * sizeof(float),
+                      hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(out->accept_rate, d_acc.p, n_chains * sizeof(float),
+                      hipMemcpyDeviceToHost) == hipSuccess &&
+            hipMemcpy(out->last_log_post, d_lp.p, n_chains * sizeof(double),
+                      hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_final)
+    ok = hipMemcpy(ext->final_state, d_final.p, ncp * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_tplp)
+    ok = hipMemcpy(ext->trace_plp, d_tplp.p, (size_t)n_iter * n_chains * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_tprop)
+    ok = hipMemcpy(ext->trace_prop, d_tprop.p, n_draws * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_tinit)
+    ok = hipMemcpy(ext->init_log_post, d_tinit.p, n_chains * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_tacc)
+    ok = hipMemcpy(ext->trace_accept, d_tacc.p, (size_t)n_iter * n_chains,
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) {
+    gpum_hip_result_free(out);
+    set_err("read results: device error");
+    return 1;
+  }
+  return 0;
+}
+
+int gpum_hip_model_run(gpum_hip_model* m, const double* init,
+                       uint32_t n_chains, const double* proposal_sd,
+                       const double* temperatures, uint32_t n_iter,
+                       uint32_t seed, uint32_t proposal_mode, double gamma,
+                       double de_noise, const double* proposal_l,
+                       gpum_run_ext* ext, gpum_chain_result* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!m) {
+    set_err("null model");
+    return 1;
+  }
+  if (!device_init()) return 1;
+  return model_run_locked(m, init, n_chains, proposal_sd, temperatures, n_iter,
+                          seed, proposal_mode, gamma, de_noise, proposal_l,
+                          ext, out, nullptr);
+}
+
+int gpum_hip_model_loglik(gpum_hip_model* m, const double* points,
+                          uint32_t n_points, double* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!m || !out) {
+    set_err("null model or output");
+    return 1;
+  }
+  if (!device_init()) return 1;
+  if (n_points == 0) return 0;
+  // A zero-iteration run evaluates the data term at the initial states;
+  // the prior is left out (the batch is the data term only).
+  std::vector<double> sd((size_t)n_points * m->n_params, 1.0);
+  std::vector<double> temps(n_points, 1.0);
+  return model_run_locked(m, points, n_points, sd.data(), temps.data(), 0, 0,
+                          0, 0.0, 0.0, nullptr, nullptr, nullptr, out, false);
+}
+
+int gpum_hip_run_chains_ex(const uint32_t* code, size_t code_len,
+                           const double* consts, size_t n_consts,
+                           const uint32_t* prior_code, size_t prior_len,
+                           const double* prior_consts, size_t n_prior_consts,
+                           uint32_t n_params, const double* data,
+                           uint32_t n_cols, uint32_t n_obs, const double* init,
+                           uint32_t n_chains, const double* proposal_sd,
+                           const double* temperatures, uint32_t n_iter,
+                           uint32_t seed, uint32_t proposal_mode, double gamma,
+                           double de_noise, const double* proposal_l,
+                           gpum_run_ext* ext, gpum_chain_result* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!device_init()) return 1;
+  hipSetDevice(g_dev.id);
+  int32_t pre = -1, fu = -1;
+  if (ext && ext->struct_size >= offsetof(gpum_run_ext, fuse) + 4) {
+    pre = ext->precompute_data;
+    fu = ext->fuse;
+  }
+  gpum_hip_model m;
+  if (!model_build(&m, code, code_len, consts, n_consts, prior_code, prior_len,
+                   prior_consts, n_prior_consts, n_params, data, n_cols, n_obs,
+                   pre, fu))
+    return 1;
+  return model_run_locked(&m, init, n_chains, proposal_sd, temperatures,
+                          n_iter, seed, proposal_mode, gamma, de_noise,
+                          proposal_l, ext, out, nullptr);
+}
+
+int gpum_hip_run_chains(const uint32_t* code, size_t code_len,
+                        const double* consts, size_t n_consts,
+                        const uint32_t* prior_code, size_t prior_len,
+                        const double* prior_consts, size_t n_prior_consts,
+                        uint32_t n_params, const double* data, uint32_t n_cols,
+                        uint32_t n_obs, const double* init, uint32_t n_chains,
+                        const double* proposal_sd, const double* temperatures,
+                        uint32_t n_iter, uint32_t seed, uint32_t proposal_mode,
+                        double gamma, double de_noise,
+                        const double* proposal_l, gpum_chain_result* out) {
+  return gpum_hip_run_chains_ex(code, code_len, consts, n_consts, prior_code,
+                                prior_len, prior_consts, n_prior_consts,
+                                n_params, data, n_cols, n_obs, init, n_chains,
+                                proposal_sd, temperatures, n_iter, seed,
+                                proposal_mode, gamma, de_noise, proposal_l,
+                                nullptr, out);
+}
+
+int gpum_hip_loglik_batch(const uint32_t* code, size_t code_len,
+                          const double* consts, size_t n_consts,
+                          uint32_t n_params, const double* data,
+                          uint32_t n_cols, uint32_t n_obs,
+                          const double* points, uint32_t n_points,
+                          double* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!device_init()) return 1;
+  hipSetDevice(g_dev.id);
+  if (n_points == 0) return 0;
+  if (!points || !out) {
+    set_err("points and out must not be NULL");
+    return 1;
+  }
+  gpum_hip_model m;
+  if (!model_build(&m, code, code_len, consts, n_consts, nullptr, 0, nullptr,
+                   0, n_params, data, n_cols, n_obs, -1, -1))
+    return 1;
+  std::vector<double> sd((size_t)n_points * n_params, 1.0);
+  std::vector<double> temps(n_points, 1.0);
+  return model_run_locked(&m, points, n_points, sd.data(), temps.data(), 0, 0,
+                          0, 0.0, 0.0, nullptr, nullptr, nullptr, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""Seeded synthetic workloads for the sampler.
+
+Each workload is a model written in the DSL plus a deterministic data
+generator (numpy PCG64 from a fixed seed).  Stored data are float32 where
+the workload says FP32; the sampler receives them promoted to f64 exactly
+as the reference's R layer hands data to its backends.
+
+  gaussian_mean   y ~ N(3.4, sd 2), n = 20 000; loglik -((y - mu)^2) / 8,
+                  flat prior; 8 chains x 3000 iterations; closed-form
+                  posterior N(mean(y), 4 / n).
+  gamma           y ~ Gamma(shape 2.5, rate 0.7), n = 1 000 000 (FP32);
+                  params (log_a, log_b), N(0, 10) priors on the logs;
+                  4096 chains x 5000 iterations.
+  mixture         y from 0.5 N(-3, 1) + 0.5 N(3, 1), n = 200 000 (FP32);
+                  params mu1, mu2; 16 temperatures x 256 replicas.
+  logistic16      16-dim Bayesian logistic regression, n = 4 000 000 (FP32),
+                  equicorrelated (0.3) N(0, 1) covariates, N(0, 5) priors.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .dsl import Model, gpum_model
+
+
+@dataclass
+class Workload:
+    name: str
+    model: Model
+    data: np.ndarray  # (n_obs, n_cols) float64 (FP32-rounded when the spec says FP32)
+    n_chains: int
+    n_iter: int
+    meta: Dict = field(default_factory=dict)
+
+    @property
+    def n_obs(self) -> int:
+        return self.data.shape[0]
+
+    @property
+    def n_cols(self) -> int:
+        return self.data.shape[1]
+
+
+GAUSSIAN_MEAN = "~ -((y - mu)^2) / 8"
+GAMMA = ("~ exp(log_a) * log_b - lgamma(exp(log_a)) + (exp(log_a) - 1) * log(y)"
+         " - exp(log_b) * y")
+GAMMA_PRIOR = "~ -0.5 * (log_a / 10)^2 - 0.5 * (log_b / 10)^2"
+MIXTURE = ("~ log(0.5 * exp(-((y - mu1)^2) / 2) + 0.5 * exp(-((y - mu2)^2) / 2))")
+
+
+def _logistic_formula(d: int) -> str:
+    eta = " + ".join(f"b{j} * x{j}" for j in range(1, d + 1))
+    return f"~ y * ({eta}) - log(1 + exp({eta}))"
+
+
+def _logistic_prior(d: int) -> str:
+    return "~ " + " + ".join(f"-0.5 * (b{j} / 5)^2" for j in range(1, d + 1))
+
+
+def gaussian_mean(n: int = 20000, seed: int = 11, n_chains: int = 8,
+                  n_iter: int = 3000) -> Workload:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y = rng.normal(3.4, 2.0, size=n)
+    m = gpum_model(GAUSSIAN_MEAN, params=["mu"], data=["y"])
+    return Workload("gaussian_mean", m, y.reshape(-1, 1), n_chains, n_iter,
+                    meta={"post_mean": float(np.mean(y)),
+                          "post_sd": float(np.sqrt(4.0 / n))})
+
+
+def gamma(n: int = 1_000_000, seed: int = 12, n_chains: int = 4096,
+          n_iter: int = 5000) -> Workload:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y = rng.gamma(shape=2.5, scale=1.0 / 0.7, size=n).astype(np.float32)
+    m = gpum_model(GAMMA, params=["log_a", "log_b"], data=["y"],
+                   prior=GAMMA_PRIOR)
+    return Workload("gamma", m, y.astype(np.float64).reshape(-1, 1), n_chains,
+                    n_iter, meta={"true": [np.log(2.5), np.log(0.7)]})
+
+
+def mixture(n: int = 200_000, seed: int = 13, n_temps: int = 16,
+            n_replicas: int = 256, n_iter: int = 4000) -> Workload:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    comp = rng.random(n) < 0.5
+    y = np.where(comp, rng.normal(-3.0, 1.0, n), rng.normal(3.0, 1.0, n))
+    y = y.astype(np.float32)
+    m = gpum_model(MIXTURE, params=["mu1", "mu2"], data=["y"])
+    betas = np.geomspace(1.0, 0.01, n_temps)
+    return Workload("mixture", m, y.astype(np.float64).reshape(-1, 1),
+                    n_temps * n_replicas, n_iter,
+                    meta={"temperatures": 1.0 / betas, "n_temps": n_temps,
+                          "n_replicas": n_replicas, "swap_every": 10})
+
+
+def logistic(n: int = 4_000_000, d: int = 16, seed: int = 15,
+             n_chains: int = 1024, n_iter: int = 2000) -> Workload:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    z = rng.standard_normal((n, d), dtype=np.float32)
+    w = rng.standard_normal((n, 1), dtype=np.float32)
+    X = (np.sqrt(0.7) * z + np.sqrt(0.3) * w).astype(np.float32)
+    beta = rng.normal(0.0, 0.5, size=d)
+    eta = X.astype(np.float64) @ beta
+    y = (rng.random(n) < 1.0 / (1.0 + np.exp(-eta))).astype(np.float32)
+    data = np.empty((n, d + 1), np.float64)
+    data[:, 0] = y
+    data[:, 1:] = X
+    m = gpum_model(_logistic_formula(d), params=[f"b{j}" for j in range(1, d + 1)],
+                   data=["y"] + [f"x{j}" for j in range(1, d + 1)],
+                   prior=_logistic_prior(d))
+    return Workload("logistic16", m, data, n_chains, n_iter,
+                    meta={"true": beta})
+
+
+def by_name(name: str, **kw) -> Workload:
+    return {"gaussian_mean": gaussian_mean, "gamma": gamma, "mixture": mixture,
+            "logistic16": logistic}[name](**kw)

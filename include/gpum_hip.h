@@ -1,0 +1,174 @@
+/*
+ * gpum_hip.h -- C ABI of the MI355X (gfx950) sampling backend.
+ *
+ * The entry points mirror the reference's Rust backend interface so that a
+ * `Backend::Hip` arm can call them with the arguments it already has:
+ *
+ *   gpum_hip_run_chains    <- gpu_metropolis_chain(...) -> ChainResult
+ *                             src/rust/src/chain_kernel.rs:468-488 (result
+ *                             struct :360-369), reached from
+ *                             rust_gpu_metropolis src/rust/src/lib.rs:69-123
+ *   gpum_hip_available     <- rust_available_backends src/rust/src/lib.rs:51-61
+ *   gpum_hip_loglik_batch  <- loglik_batch src/rust/src/cpu_native.rs:339-360
+ *                             via rust_loglik_batch src/rust/src/lib.rs:131-151
+ *
+ * Conventions (as in the reference):
+ *   - `code` is the DSL bytecode, (op, arg) u32 pairs (R/dsl.R:13-17);
+ *     `code_len` counts u32 words.  An empty program evaluates to 0.
+ *   - `data` is row-major n_obs x n_cols f64 (R/api.R:288-290).
+ *   - `init`, `proposal_sd` are chain-major (n_chains x n_params);
+ *     `proposal_l` is n_chains x n_params x n_params row-major lower
+ *     triangular, or NULL.
+ *   - proposal_mode: 0 diagonal random walk, 1 Differential Evolution against
+ *     the `init` snapshot, 2 Cholesky random walk, 3 MALA (not available on
+ *     this backend: rejected with an error, as the reference's GPU backends
+ *     refuse it, R/api.R:373-377).
+ *   - draws are f32, column-major (n_iter, n_chains, n_params):
+ *     draws[t + n_iter * (c + n_chains * j)].
+ *   - Uniforms come from a counter-based Philox4x32-10 stream keyed on
+ *     (seed, stream id) and addressed by (iteration, chain, slot); the slot
+ *     layout per iteration is the reference's consumption order.  The
+ *     numbers differ from the reference's triple32 stream.
+ *
+ * Return values: 0 on success; nonzero on failure with a message available
+ * from gpum_hip_last_error() (thread-local).  Messages about allocation
+ * failures contain "out of memory"; failed launches contain "kernel"; a
+ * missing or unsupported GPU contains "device" -- the phrases the
+ * reference's R error guard matches (R/api.R:12-34).
+ *
+ * Threading: calls are synchronous and may be made from any thread; the
+ * library serialises its own device work per process.
+ */
+#ifndef GPUM_HIP_H
+#define GPUM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPUM_HIP_ABI_VERSION 1
+
+/* chain_kernel.rs:360-369 (ChainResult).  Buffers are owned by the library
+ * until gpum_hip_result_free(). */
+typedef struct gpum_chain_result {
+  uint32_t n_iter;
+  uint32_t n_chains;
+  uint32_t n_params;
+  float* draws;          /* n_iter * n_chains * n_params */
+  float* accept_rate;    /* n_chains */
+  double* last_log_post; /* n_chains: untempered log-posterior of the final state */
+} gpum_chain_result;
+
+/* Optional extension of a run.  Zero-initialise and set struct_size;
+ * fields beyond struct_size are treated as zero. */
+typedef struct gpum_run_ext {
+  uint32_t struct_size;
+  uint32_t stream_id;     /* Philox key word 1 (model index); default 0 */
+  /* Host buffers filled when non-NULL: */
+  double* trace_plp;      /* n_iter * n_chains: log-posterior of each proposal */
+  double* trace_prop;     /* n_iter * n_chains * n_params: each proposal */
+  uint8_t* trace_accept;  /* n_iter * n_chains: 1 if accepted */
+  double* final_state;    /* n_chains * n_params: state after the last iteration */
+  double* init_log_post;  /* n_chains: log-posterior of `init` as evaluated */
+  /* Tiling overrides (0 = autotuned). */
+  int32_t chains_per_block;
+  int32_t waves_per_chain;
+  /* Lowering switches: -1 = default (on), 0 = off, 1 = on. */
+  int32_t precompute_data;
+  int32_t fuse;
+  /* Out: device time of the chain kernel in milliseconds. */
+  double kernel_ms;
+} gpum_run_ext;
+
+/* Static description of a lowered model. */
+typedef struct gpum_model_info {
+  uint32_t n_vm_instr;     /* per-observation VM program length */
+  uint32_t vm_depth;       /* peak stack depth of the VM program */
+  uint32_t ref_depth;      /* peak stack depth of the DSL bytecode */
+  uint32_t n_uniform;      /* hoisted values + constants */
+  uint32_t n_cols_total;   /* data columns streamed (incl. precomputed) */
+  uint32_t n_hoist_words;  /* FP64 hoist program length (u32 words) */
+  double w_exec;           /* nominal VALU slots per observation evaluation */
+  double w_bytecode;       /* same weights on the program as compiled */
+} gpum_model_info;
+
+typedef struct gpum_hip_model gpum_hip_model;
+
+/* 1 if a gfx950 device is visible and usable, else 0. */
+int gpum_hip_available(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gpum_hip_last_error(void);
+
+/* One batch of whole-chain Metropolis iterations; the HIP replacement of
+ * gpu_metropolis_chain (chain_kernel.rs:468-532). */
+int gpum_hip_run_chains(const uint32_t* code, size_t code_len,
+                        const double* consts, size_t n_consts,
+                        const uint32_t* prior_code, size_t prior_len,
+                        const double* prior_consts, size_t n_prior_consts,
+                        uint32_t n_params, const double* data, uint32_t n_cols,
+                        uint32_t n_obs, const double* init, uint32_t n_chains,
+                        const double* proposal_sd, const double* temperatures,
+                        uint32_t n_iter, uint32_t seed, uint32_t proposal_mode,
+                        double gamma, double de_noise,
+                        const double* proposal_l, gpum_chain_result* out);
+
+/* Same, with the extension struct (may be NULL). */
+int gpum_hip_run_chains_ex(const uint32_t* code, size_t code_len,
+                           const double* consts, size_t n_consts,
+                           const uint32_t* prior_code, size_t prior_len,
+                           const double* prior_consts, size_t n_prior_consts,
+                           uint32_t n_params, const double* data,
+                           uint32_t n_cols, uint32_t n_obs, const double* init,
+                           uint32_t n_chains, const double* proposal_sd,
+                           const double* temperatures, uint32_t n_iter,
+                           uint32_t seed, uint32_t proposal_mode, double gamma,
+                           double de_noise, const double* proposal_l,
+                           gpum_run_ext* ext, gpum_chain_result* out);
+
+/* Free the buffers of a result filled by a run call. */
+void gpum_hip_result_free(gpum_chain_result* r);
+
+/* Log-likelihood summed over observations at each of n_points parameter
+ * vectors (point-major), cpu_native.rs:339-360.  out: n_points doubles. */
+int gpum_hip_loglik_batch(const uint32_t* code, size_t code_len,
+                          const double* consts, size_t n_consts,
+                          uint32_t n_params, const double* data,
+                          uint32_t n_cols, uint32_t n_obs,
+                          const double* points, uint32_t n_points,
+                          double* out);
+
+/* A model with its lowered program and data resident on the device, for
+ * repeated batches without re-uploading the data (the reference uploads
+ * every call, chain_kernel.rs:393-413).  precompute_data / fuse: -1
+ * default, 0 off, 1 on.  Returns NULL on failure. */
+gpum_hip_model* gpum_hip_model_create(
+    const uint32_t* code, size_t code_len, const double* consts,
+    size_t n_consts, const uint32_t* prior_code, size_t prior_len,
+    const double* prior_consts, size_t n_prior_consts, uint32_t n_params,
+    const double* data, uint32_t n_cols, uint32_t n_obs,
+    int32_t precompute_data, int32_t fuse);
+
+int gpum_hip_model_info(const gpum_hip_model* m, gpum_model_info* info);
+
+/* One batch on a resident model; arguments as gpum_hip_run_chains_ex. */
+int gpum_hip_model_run(gpum_hip_model* m, const double* init,
+                       uint32_t n_chains, const double* proposal_sd,
+                       const double* temperatures, uint32_t n_iter,
+                       uint32_t seed, uint32_t proposal_mode, double gamma,
+                       double de_noise, const double* proposal_l,
+                       gpum_run_ext* ext, gpum_chain_result* out);
+
+/* Loglik batch on a resident model. */
+int gpum_hip_model_loglik(gpum_hip_model* m, const double* points,
+                          uint32_t n_points, double* out);
+
+void gpum_hip_model_free(gpum_hip_model* m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUM_HIP_H */
