@@ -1,0 +1,212 @@
+"""Headline benchmark: plain HIP random-walk Metropolis on the gamma model.
+
+Workload: y = 1 000 000 FP32 draws ~ Gamma(shape 2.5, rate 0.7); params
+(log_a, log_b) with N(0, 10) priors; 4096 chains x 5000 iterations.  The
+data are synthetic and seeded; there are no model weights.
+
+One step = one batch of `--iters-per-step` Metropolis iterations of all 4096
+chains over the full data set on the GPU (one launch of the fused chain
+kernel).  The W warmup steps run the reference's adaptive warmup
+(Welford + Robbins-Monro + pooled Cholesky, R/adaptive.R:101-254) untimed;
+the K timed steps continue the chains from their exact FP64 states with the
+frozen per-chain Cholesky proposal.  Defaults: W = K = 10, 250 iterations per
+step: 2500 warmup + 2500 sampling iterations = the 5000-iteration workload.
+
+value  = observation-loglik evaluations per second over the timed region
+         (chains x observations x iterations / wall time, inputs resident in
+         HBM, draws copied back to the host inside the region).
+Also reported: ESS/s (min over parameters of the summed per-chain Geyer ESS
+of the sampling draws / timed wall) with the max split R-hat, the roofline
+of the chain kernel, and the oracle timed on the host's cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+VALU_PEAK = 256 * 64 * 2.4e9  # FP32 lane-ops/s, non-packed (one slot per op)
+HBM_PEAK = 8.0e12             # B/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--iters-per-step", type=int, default=250)
+    p.add_argument("--chains", type=int, default=4096)
+    p.add_argument("--n-obs", type=int, default=1_000_000)
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
+                   help="target duration of the CPU-baseline sample")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(w, target_s):
+    """The oracle (C restatement of the reference's f64 CPU sampler with the
+    per-model compiled log-likelihood mirroring its JIT) on the host cores,
+    on a bounded sample of the same workload: `cores` chains (one per
+    thread) over the full data set for a calibrated number of iterations."""
+    from oracle import codegen, oracle
+
+    m = w.model
+    cores = min(16, os.cpu_count() or 1)
+    _, _, addr = codegen.compile_loglik(m.loglik.code, m.loglik.consts)
+    C = cores
+    init = np.tile([math.log(2.5), math.log(0.7)], (C, 1))
+    sd = np.full((C, 2), 0.001)
+
+    def timed(iters):
+        t = time.perf_counter()
+        oracle.run(m.loglik.code, m.loglik.consts, m.prior.code,
+                   m.prior.consts, 2, w.data, 1, w.n_obs, init, sd,
+                   np.ones(C), iters, 1, n_threads=cores, loglik_fn=addr)
+        return time.perf_counter() - t
+
+    t1 = timed(1)
+    iters = max(1, int(target_s / max(t1, 1e-3)))
+    dt = timed(iters)
+    evals = C * w.n_obs * (iters + 1)  # + the initial evaluation
+    return {"value": evals / dt, "unit": "obs-loglik evals/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{C} chains x {iters} iterations x {w.n_obs} obs "
+                      f"(compiled f64 loglik, {cores} threads, {dt:.1f} s)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must "
+                          "be 1 (multi-GPU sharding is described in "
+                          "DESIGN.md, not launched)"}))
+        sys.exit(2)
+    import torch
+
+    from gpumetropolis_amd import _native, adaptive as ad, workloads
+    from gpumetropolis_amd.api import adaptive_warmup
+    from gpumetropolis_amd.diagnostics import summarize
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    torch.cuda.set_device(0)
+    if not _native.available():
+        raise RuntimeError(_native.last_error())
+
+    w = workloads.gamma(n=a.n_obs, n_chains=a.chains)
+    m = w.model
+    C, P, N = a.chains, m.n_params, w.n_obs
+    dm = _native.DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
+                             m.prior.consts, P, w.data, 1, N)
+    info = dm.info()
+    print(f"[bench] model lowered: {info}", file=sys.stderr, flush=True)
+    state = {"kernel_ms": []}
+
+    def call(init_m, sd_m, iters, s, temps=None, mode=0, g=0.0, noise=0.0,
+             L=None):
+        r = dm.run(init_m, sd_m, np.ones(C) if temps is None else temps,
+                   iters, int(s) & 0xFFFFFFFF, proposal_mode=mode, gamma=g,
+                   de_noise=noise, proposal_l=L, final_state=True)
+        state["kernel_ms"].append(r.kernel_ms)
+        return r
+
+    # Initial values drawn around the prior scale, as gpu_metropolis does.
+    from gpumetropolis_amd.philox_host import normals
+    init = normals(a.seed, C * P).reshape(C, P)
+    sd0 = np.full((C, P), 0.1)
+    t0 = time.perf_counter()
+    wu = adaptive_warmup(call, P, C, init, sd0, [a.iters_per_step] * a.warmup,
+                         a.seed)
+    warm_s = time.perf_counter() - t0
+    print(f"[bench] warmup {a.warmup} x {a.iters_per_step} it done in "
+          f"{warm_s:.1f} s, accept {wu.accept_history[:, -1].mean():.3f}",
+          file=sys.stderr, flush=True)
+
+    state["kernel_ms"].clear()
+    draws = []
+    cur = wu.init
+    # final_state of the last warmup batch is the exact FP64 state; the
+    # warmup hands back the f32 draws, so continue from them as R does.
+    mode = 2 if wu.L is not None else 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        r = call(cur, wu.sd, a.iters_per_step, a.seed + wu.batches_run + k,
+                 mode=mode, L=wu.L)
+        cur = r.final_state
+        draws.append(r.draws)
+        print(f"[bench] step {k + 1}/{a.steps} kernel {r.kernel_ms:.1f} ms",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    iters = a.steps * a.iters_per_step
+    evals = C * N * iters
+    value = evals / wall
+    d = np.concatenate(draws).astype(np.float64)
+    summ = summarize(d)
+    ess_min = summ["min_ess"]
+    kms = np.array(state["kernel_ms"])
+    avg_launch_s = kms.mean() / 1e3
+    evals_per_launch = C * N * a.iters_per_step
+    w_exec = info["w_exec"]
+    achieved_slots = w_exec * evals_per_launch / avg_launch_s
+    # Data bytes one launch streams through the chips' caches into LDS:
+    # every workgroup reads every column once per iteration.
+    cpb = 16
+    n_wg = math.ceil(C / cpb)
+    bytes_per_launch = (a.iters_per_step * n_wg * N * info["n_cols_total"] * 4)
+    out = {
+        "metric": "ESS/s at R-hat<1.01 + obs-loglik evals/s; plain vs "
+                  "surrogate-DA, 1 MI355X",
+        "value": value,
+        "unit": "obs-loglik evals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 terms, fp64 sums",
+        "data": "synthetic (seeded Gamma(2.5, 0.7) draws, FP32)",
+        "config": {"workload": "plain RWM, gamma likelihood, 1e6 FP32 obs, "
+                               f"{C} chains x {iters} sampling + "
+                               f"{a.warmup * a.iters_per_step} warmup iterations",
+                   "n_obs": N, "chains": C,
+                   "iters_per_step": a.iters_per_step,
+                   "parallelism": "single GPU"},
+        "ess_per_s": ess_min / wall,
+        "rhat_max": summ["max_rhat"],
+        "ess_min": ess_min,
+        "posterior": [{"mean": p["mean"], "sd": p["sd"], "mcse": p["mcse"]}
+                      for p in summ["params"]],
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_slots / 1e12,
+            "peak": VALU_PEAK / 1e12,
+            "unit": "T VALU-slot/s",
+            "frac": achieved_slots / VALU_PEAK,
+            "traffic": None,
+            "w_exec": w_exec,
+            "w_bytecode": info["w_bytecode"],
+            "kernel": "gpum::chain_kernel",
+            "avg_launch_ms": kms.mean(),
+            "stream_bytes_per_launch": bytes_per_launch,
+            "stream_GBps": bytes_per_launch / avg_launch_s / 1e9,
+        },
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    dm.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,377 @@
+"""``gpu_metropolis`` on the HIP backend.
+
+Host-side mirror of the reference's user entry point (R/api.R:247-599) and
+its batch orchestrators (R/adaptive.R:101-524), driving the C ABI one batch
+at a time exactly where the reference calls ``rust_gpu_metropolis``:
+
+  method="rwm", adapt=True   .am_orchestrate_warmup, then one sampling batch
+                             with the frozen per-chain Cholesky factor
+                             (R/api.R:529-572, R/adaptive.R:101-254)
+  method="rwm", adapt=False  one run, warmup rows trimmed (R/api.R:574-598)
+  method="pt"                .pt_orchestrate (R/adaptive.R:272-413)
+  method="de"                .de_orchestrate path A (R/adaptive.R:427-524)
+
+The model's data are uploaded once (a device-resident model) instead of
+once per batch.  Initial values and PT swap uniforms come from the Philox
+stream keyed on ``seed`` (the reference draws them from R's generator), so
+runs are reproducible from ``seed`` alone.  ``method="mala"`` and
+``de_sync=True`` are refused on this backend, as the reference refuses them
+on its GPU backends (R/api.R:373-377, :468-472).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import adaptive as ad
+from ._native import DeviceModel
+from .dsl import Model
+from .philox_host import normals, uniforms
+
+
+@dataclass
+class Fit:
+    draws: np.ndarray  # (n_iter, n_chains, n_params) float64 of the f32 draws
+    accept_rate: np.ndarray
+    method: str
+    n_iter: int
+    n_iter_total: int
+    warmup: int
+    n_chains: int
+    n_params: int
+    params: list
+    seed: int
+    backend: str = "hip"
+    temperatures: Optional[np.ndarray] = None
+    adaptation: Dict = field(default_factory=dict)
+    kernel_ms: float = 0.0
+
+
+def _data_matrix(model: Model, data) -> np.ndarray:
+    if model.n_cols == 0:
+        return np.zeros((0, 0))
+    if isinstance(data, dict):
+        missing = [c for c in model.data if c not in data]
+        if missing:
+            raise ValueError("missing data columns: " + ", ".join(missing))
+        cols = [np.asarray(data[c], dtype=np.float64).ravel() for c in model.data]
+        return np.stack(cols, axis=1)
+    a = np.asarray(data, dtype=np.float64)
+    return a.reshape(-1, model.n_cols)
+
+
+def _sd_matrix(proposal_sd, n_chains, np_) -> np.ndarray:
+    """R/api.R:326-346: scalar, per-parameter vector or chain matrix."""
+    sd = np.asarray(proposal_sd, dtype=np.float64)
+    if sd.ndim == 2:
+        if sd.shape != (n_chains, np_):
+            raise ValueError(f"`proposal_sd` matrix must be `n_chains` by "
+                             f"`n_params` ({n_chains} by {np_}).")
+        m = sd.copy()
+    else:
+        v = np.resize(sd.ravel(), np_)
+        m = np.tile(v, (n_chains, 1))
+    if not np.all(np.isfinite(m)) or np.any(m <= 0):
+        raise ValueError("`proposal_sd` must be positive and finite.")
+    return m
+
+
+def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
+                   n_iter: int = 2000, n_chains: int = 4, warmup=None,
+                   adapt: bool = True, method: str = "rwm",
+                   temperatures=None, swap_every=None, gamma=None,
+                   de_noise: float = 1e-3, de_every=None, de_sync=False,
+                   seed: int = 1, backend: str = "hip",
+                   tiling=None) -> Fit:
+    if backend not in ("hip", "auto"):
+        raise ValueError(f"backend '{backend}' is not available here; this "
+                         "package is the hip backend")
+    if method not in ("rwm", "pt", "de", "mala"):
+        raise ValueError(f"unknown method '{method}'")
+    if method == "mala":
+        raise ValueError('`method = "mala"` runs on `backend = "cpu"` in this '
+                         "release; the hip backend implements rwm, pt and de.")
+    X = _data_matrix(model, data)
+    n_obs = X.shape[0]
+    np_ = model.n_params
+    if init is None:
+        n_chains = int(n_chains)
+        init_mat = normals(seed, n_chains * np_).reshape(n_chains, np_)
+    else:
+        init_mat = np.asarray(init, dtype=np.float64)
+        if init_mat.ndim == 1:
+            init_mat = init_mat.reshape(-1, np_)
+        if init_mat.shape[1] != np_:
+            raise ValueError(f"`init` must have one column per parameter ({np_}).")
+        n_chains = init_mat.shape[0]
+    n_iter = int(n_iter)
+    warmup_auto = warmup == "auto"
+    if warmup is None or warmup_auto:
+        warmup = n_iter // 2
+    warmup = int(warmup)
+    if warmup < 0 or warmup >= n_iter:
+        raise ValueError("`warmup` must be a non-negative integer strictly "
+                         "less than `n_iter`.")
+    sd_mat = _sd_matrix(proposal_sd, n_chains, np_)
+
+    dm = DeviceModel(model.loglik.code, model.loglik.consts, model.prior.code,
+                     model.prior.consts, np_, X, model.n_cols, n_obs)
+    kernel_ms = [0.0]
+
+    def call(init_m, sd_m, iters, s, temps=None, mode=0, g=0.0, noise=0.0,
+             L=None):
+        t = np.ones(n_chains) if temps is None else temps
+        r = dm.run(init_m, sd_m, t, iters, int(s) & 0xFFFFFFFF,
+                   proposal_mode=mode, gamma=g, de_noise=noise, proposal_l=L,
+                   final_state=True, tiling=tiling)
+        kernel_ms[0] += r.kernel_ms
+        return r
+
+    try:
+        if method == "pt":
+            fit = _pt(call, model, init_mat, sd_mat, temperatures, n_iter,
+                      warmup, swap_every, adapt, seed, n_chains)
+        elif method == "de":
+            if n_chains < 4:
+                raise ValueError('`method = "de"` needs at least 4 chains.')
+            if de_sync:
+                raise ValueError('`de_sync = TRUE` runs on `backend = "cpu"` '
+                                 "in this release.")
+            fit = _de(call, model, init_mat, sd_mat, gamma, de_noise, n_iter,
+                      warmup, de_every, adapt, seed, n_chains)
+        elif adapt and warmup > 0:
+            fit = _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup,
+                                seed, warmup_auto, n_chains)
+        else:
+            r = call(init_mat, sd_mat, n_iter, seed)
+            d = r.draws[warmup:].astype(np.float64)
+            fit = Fit(d, r.accept_rate.astype(np.float64), "rwm", d.shape[0],
+                      n_iter, warmup, n_chains, np_, model.params, seed)
+    finally:
+        dm.close()
+    fit.kernel_ms = kernel_ms[0]
+    return fit
+
+
+@dataclass
+class WarmupResult:
+    init: np.ndarray        # (C, d) chain states to start sampling from
+    sd: np.ndarray          # (C, d) per-chain diagonal proposal scale
+    L: Optional[np.ndarray]  # (C * d * d,) per-chain Cholesky factor or None
+    scales: np.ndarray
+    used: int
+    batches_run: int
+    batch_sizes: list
+    accept_history: np.ndarray
+
+
+def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
+                    auto_stop=False) -> WarmupResult:
+    """.am_orchestrate_warmup (R/adaptive.R:101-254): per-chain and pooled
+    Welford, Robbins-Monro scale per chain, mid-warmup reset, per-chain
+    L = scale * chol(pooled covariance), optional early freeze."""
+    np_ = n_params
+    nb = len(sizes)
+    states = ad.WelfordState.init(n_chains, np_, sd_mat ** 2)
+    scale0 = 2.38 / math.sqrt(np_)
+    scales = np.full(n_chains, scale0)
+    cur_init = init_mat.copy()
+    cur_sd = sd_mat.copy()
+    acc_hist = np.full((n_chains, nb), np.nan)
+    cur_L = None
+    L_pool = None
+    pool = ad.PooledWelford.init(np_, sd_mat[0] ** 2)
+    target = ad.target_accept(np_)
+    reset_batch = nb // 2
+    prev_scales = scales.copy()
+    streak = 0
+    batches_run = 0
+    used = 0
+    for b in range(1, nb + 1):
+        use_L = np_ > 1 and cur_L is not None
+        r = call(cur_init, cur_sd, sizes[b - 1], seed + b - 1,
+                 mode=2 if use_L else 0, L=cur_L if use_L else None)
+        X = r.draws.astype(np.float64)  # the f32 draws, as R receives them
+        states.update_batch(X)
+        pool.update_batch(X)
+        bi = b - reset_batch if b > reset_batch else b
+        scales = ad.robbins_monro_scale(scales, r.accept_rate.astype(np.float64),
+                                        bi, np_, target)
+        cur_init = X[-1].copy()
+        cov = states.covariance()
+        diag_sd = np.sqrt(np.maximum(np.diagonal(cov, axis1=1, axis2=2), 1e-12))
+        cur_sd = scales[:, None] * diag_sd
+        acc_hist[:, b - 1] = r.accept_rate
+        if b == reset_batch:
+            diag_now = np.maximum(np.diag(pool.covariance()), 1e-12)
+            pool = ad.PooledWelford.init(np_, diag_now)
+            states = ad.WelfordState.init(n_chains, np_, diag_now)
+            scales = np.full(n_chains, scale0)
+        if np_ > 1:
+            if pool.n >= 2:
+                L_pool = ad.regularised_cholesky(pool.covariance())
+            if L_pool is not None:
+                cur_L = (scales[:, None, None] * L_pool[None]).reshape(-1)
+        batches_run = b
+        used += sizes[b - 1]
+        if auto_stop and b >= max(10, reset_batch + 5):
+            drift = float(np.max(np.abs(scales / prev_scales - 1)))
+            if np.mean(np.abs(r.accept_rate - target)) < 0.05 and drift < 0.03:
+                streak += 1
+            else:
+                streak = 0
+            if streak >= 2:
+                break
+        prev_scales = scales.copy()
+    return WarmupResult(cur_init, cur_sd, cur_L if np_ > 1 else None, scales,
+                        used, batches_run, sizes[:batches_run],
+                        acc_hist[:, :batches_run])
+
+
+def _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup, seed,
+                  auto_stop, n_chains) -> Fit:
+    """R/adaptive.R:101-254 then R/api.R:537-571."""
+    np_ = model.n_params
+    w = adaptive_warmup(call, np_, n_chains, init_mat, sd_mat,
+                        ad.warmup_batches(warmup), seed, auto_stop)
+    sampling = max(1, n_iter - w.used)
+    use_L = w.L is not None
+    r = call(w.init, w.sd, sampling, seed + w.batches_run,
+             mode=2 if use_L else 0, L=w.L if use_L else None)
+    d = r.draws.astype(np.float64)
+    return Fit(d, r.accept_rate.astype(np.float64), "rwm", d.shape[0],
+               w.used + d.shape[0], w.used, n_chains, np_, model.params, seed,
+               adaptation={"final_proposal_sd": w.sd, "final_scales": w.scales,
+                           "n_batches": w.batches_run,
+                           "batch_sizes": w.batch_sizes,
+                           "accept_history": w.accept_history,
+                           "final_L": w.L})
+
+
+def _pt(call, model, init_mat, sd_mat, temperatures, n_iter, warmup,
+        swap_every, adapt, seed, n_chains) -> Fit:
+    """R/api.R:395-440 and R/adaptive.R:272-413."""
+    np_ = model.n_params
+    if temperatures is None:
+        temps = ad.default_ladder(n_chains)
+    else:
+        temps = np.asarray(temperatures, dtype=np.float64).ravel()
+        if temps.size != n_chains:
+            raise ValueError("`temperatures` must have one value per chain.")
+        if not np.all(np.isfinite(temps)) or np.any(temps <= 0):
+            raise ValueError("`temperatures` must be positive and finite.")
+    if swap_every is None:
+        swap_every = max(n_chains, 10)
+    swap_every = int(swap_every)
+    if swap_every < 1:
+        raise ValueError("`swap_every` must be a positive integer.")
+    sizes, nb = ad.split_batches(n_iter, swap_every)
+    wb = max(0, min(nb, warmup // swap_every))
+    warmup_used = sum(sizes[:wb])
+    states = ad.WelfordState.init(n_chains, np_, sd_mat ** 2)
+    scales = np.ones(n_chains)
+    cur_init = init_mat.copy()
+    cur_sd = sd_mat.copy()
+    kept = []
+    acc_acc = np.zeros(n_chains)
+    iters_acc = 0
+    pairs = n_chains - 1
+    swap_hist = np.full((max(pairs, 0), nb), np.nan)
+    acc_hist = np.full((n_chains, nb), np.nan)
+    r = None
+    for b in range(1, nb + 1):
+        bsize = sizes[b - 1]
+        r = call(cur_init, cur_sd, bsize, seed + b - 1, temps=temps)
+        X = r.draws.astype(np.float64)
+        lp = r.last_log_post.copy()
+        acc_hist[:, b - 1] = r.accept_rate
+        in_warmup = b <= wb
+        if not in_warmup:
+            kept.append(X)
+            acc_acc += bsize * r.accept_rate
+            iters_acc += bsize
+        if in_warmup and adapt:
+            states.update_batch(X)
+            scales = ad.robbins_monro_scale(scales, r.accept_rate.astype(np.float64),
+                                            b, np_)
+            cov = states.covariance()
+            cur_sd = scales[:, None] * np.sqrt(
+                np.maximum(np.diagonal(cov, axis1=1, axis2=2), 1e-12))
+        cur_init = X[-1].copy()
+        if pairs > 0:
+            order = (list(range(0, pairs, 2)) if b % 2 == 1
+                     else list(range(min(1, pairs - 1), pairs, 2)))
+            u = uniforms(seed, b, len(order))
+            att = np.zeros(pairs)
+            acc = np.zeros(pairs)
+            for k, i in enumerate(order):
+                delta = (lp[i + 1] - lp[i]) * (1 / temps[i] - 1 / temps[i + 1])
+                att[i] += 1
+                if math.isfinite(delta) and math.log(u[k]) < delta:
+                    cur_init[[i, i + 1]] = cur_init[[i + 1, i]]
+                    lp[[i, i + 1]] = lp[[i + 1, i]]
+                    acc[i] += 1
+            with np.errstate(invalid="ignore", divide="ignore"):
+                swap_hist[:, b - 1] = np.where(att > 0, acc / np.maximum(att, 1), np.nan)
+    d = np.concatenate(kept) if kept else np.zeros((0, n_chains, np_))
+    rate = acc_acc / iters_acc if iters_acc > 0 else r.accept_rate
+    return Fit(d, rate, "pt", d.shape[0], warmup_used + d.shape[0], warmup_used,
+               n_chains, np_, model.params, seed, temperatures=temps,
+               adaptation={"final_proposal_sd": cur_sd, "final_scales": scales,
+                           "n_batches": nb, "batch_sizes": sizes,
+                           "accept_history": acc_hist,
+                           "swap_history": swap_hist})
+
+
+def _de(call, model, init_mat, sd_mat, gamma, de_noise, n_iter, warmup,
+        de_every, adapt, seed, n_chains) -> Fit:
+    """R/api.R:442-527 and R/adaptive.R:427-524 (path A)."""
+    np_ = model.n_params
+    if gamma is None:
+        gamma = 2.38 / math.sqrt(2 * np_)
+    gamma = float(gamma)
+    if not math.isfinite(gamma) or gamma <= 0:
+        raise ValueError("`gamma` must be positive and finite.")
+    if not math.isfinite(de_noise) or de_noise < 0:
+        raise ValueError("`de_noise` must be non-negative and finite.")
+    if de_every is None:
+        de_every = max(n_chains, 10)
+    sizes, nb = ad.split_batches(n_iter, int(de_every))
+    wb = max(0, min(nb, warmup // int(de_every)))
+    warmup_used = sum(sizes[:wb])
+    states = ad.WelfordState.init(n_chains, np_, sd_mat ** 2)
+    cur_init = init_mat.copy()
+    cur_sd = sd_mat.copy()
+    kept = []
+    acc_acc = np.zeros(n_chains)
+    iters_acc = 0
+    acc_hist = np.full((n_chains, nb), np.nan)
+    disp = np.full((np_, nb), np.nan)
+    r = None
+    for b in range(1, nb + 1):
+        bsize = sizes[b - 1]
+        r = call(cur_init, cur_sd, bsize, seed + b - 1, mode=1, g=gamma,
+                 noise=de_noise)
+        X = r.draws.astype(np.float64)
+        acc_hist[:, b - 1] = r.accept_rate
+        in_warmup = b <= wb
+        if not in_warmup:
+            kept.append(X)
+            acc_acc += bsize * r.accept_rate
+            iters_acc += bsize
+        if in_warmup and adapt:
+            states.update_batch(X)
+            cov = states.covariance()
+            cur_sd = np.sqrt(np.maximum(np.diagonal(cov, axis1=1, axis2=2), 1e-12))
+        cur_init = X[-1].copy()
+        disp[:, b - 1] = cur_init.std(axis=0, ddof=1)
+    d = np.concatenate(kept) if kept else np.zeros((0, n_chains, np_))
+    rate = acc_acc / iters_acc if iters_acc > 0 else r.accept_rate
+    return Fit(d, rate, "de", d.shape[0], warmup_used + d.shape[0], warmup_used,
+               n_chains, np_, model.params, seed,
+               adaptation={"final_proposal_sd": cur_sd, "n_batches": nb,
+                           "batch_sizes": sizes, "accept_history": acc_hist,
+                           "disp_history": disp})

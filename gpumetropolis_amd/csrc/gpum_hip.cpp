@@ -302,6 +302,16 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
     if (ext->waves_per_chain > 0) wpc = (uint32_t)ext->waves_per_chain;
     if (ext->chains_per_block > 0) cpb = (uint32_t)ext->chains_per_block;
   }
+  // Wide rows: fewer waves per chain so one chunk per wave fits in LDS.
+  {
+    const uint32_t ncol0 = std::max(1u, m->n_cols_total);
+    const size_t tile_cap = g_dev.lds_max / 2;
+    while (wpc > 1 && (size_t)t->chunk * wpc * ncol0 * 4 > tile_cap) {
+      wpc /= 2;
+      if (!(ext && ext->chains_per_block > 0))
+        cpb = std::min(std::max(1u, 16u / wpc), std::max(1u, n_chains));
+    }
+  }
   if (cpb * wpc > 16 || cpb == 0 || wpc == 0) {
     set_err("invalid tiling: chains_per_block * waves_per_chain must be <= 16");
     return false;

@@ -231,7 +231,7 @@ def run(code, consts, prior_code, prior_consts, n_params, data, n_cols, n_obs,
     a.stream = stream
     a.stream_id = stream_id
     a.n_threads = n_threads or os.cpu_count() or 1
-    a.loglik_fn = C.cast(loglik_fn, C.c_void_p) if loglik_fn else None
+    a.loglik_fn = int(loglik_fn) if loglik_fn else None
     draws = np.zeros(max(n_iter * n_chains * n_params, 1), np.float32)
     acc = np.zeros(n_chains, np.float32)
     lp = np.zeros(n_chains, np.float64)
