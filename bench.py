@@ -47,7 +47,21 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target duration of the CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="only run a plain device-to-device copy of MIB MiB "
+                        "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4}), flush=True)
 
 
 def cpu_baseline(w, target_s):
@@ -83,6 +97,9 @@ def cpu_baseline(w, target_s):
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must "
                           "be 1 (multi-GPU sharding is described in "

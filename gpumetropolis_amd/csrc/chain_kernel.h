@@ -75,10 +75,18 @@ struct ChainHot {
   uint32_t cpb;
   uint32_t wpc;
   uint32_t tile_len;
-  uint32_t resident;
+  // 0: whole data set resident in LDS; 1: double-buffered streaming through
+  // LDS DMA copies; 2: synchronous single-buffer streaming.
+  uint32_t stream_mode;
   uint32_t lds_tile_offset;
   uint32_t state_words;
 };
+
+// Streamed tiles: two static LDS buffers of kTileF4 float4s each.
+constexpr uint32_t kTileF4 = 4800;
+constexpr size_t kStreamStaticLds = 2 * (size_t)kTileF4 * 16;
+// Copies per wave that fill one buffer with 1024 threads.
+constexpr uint32_t kTileIssues = (kTileF4 + 1023) / 1024;
 
 hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
                                int ns, int opt, size_t lds_bytes,

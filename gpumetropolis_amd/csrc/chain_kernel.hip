@@ -190,7 +190,47 @@ __device__ __noinline__ void chain_finish(const ChainArgs* __restrict__ A,
     for (int j = 0; j < np; ++j) A->final_state[(size_t)chain * np + j] = st[j];
 }
 
+// Streamed tiles live in two static LDS buffers.  Being distinct LDS
+// objects, the compiler can tell that the waves' reads of one buffer do not
+// depend on the asynchronous copy into the other, so the copy overlaps the
+// evaluation instead of being waited for at the first LDS read.
+__shared__ float4 g_tile_a[kTileF4];
+__shared__ float4 g_tile_b[kTileF4];
+
+// The chain's share of one tile: chunks part, part + wpc, ... of 64*OPT
+// observations; FP32 sum per chunk, FP64 across chunks.
 template <int NS, int OPT>
+__device__ __forceinline__ double tile_pass(const ChainHot& H,
+                                            const float* __restrict__ tile,
+                                            uint32_t t0, uint32_t part,
+                                            uint32_t wpc, const float* U,
+                                            int lane) {
+  const uint32_t chunk = 64u * OPT;
+  const uint32_t nchunks = H.tile_len / chunk;
+  double acc = 0.0;
+  for (uint32_t ck = part; ck < nchunks; ck += wpc) {
+    const uint32_t base = ck * chunk;
+    float T[OPT];
+    vm_run<NS, OPT>((vm_prog_t)H.vm, H.n_vm, U, tile + base, H.tile_len, lane,
+                    T);
+    const uint32_t first = t0 + base;
+    float s = 0.0f;
+    if (first + chunk <= H.n_obs) {
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) s += T[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) {
+        uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
+        s += obs < H.n_obs ? T[k] : 0.0f;
+      }
+    }
+    acc += (double)s;
+  }
+  return acc;
+}
+
+template <int NS, int OPT, bool STREAM>
 __global__ void __launch_bounds__(1024)
 chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -200,11 +240,12 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   double* st_all = reinterpret_cast<double*>(smem);
   double* partials = st_all + cpb * SW;
   float* Uall = reinterpret_cast<float*>(partials + cpb * wpc);
-  float* tile = reinterpret_cast<float*>(smem + H.lds_tile_offset);
 
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  // Wave-uniform indices: readfirstlane tells the compiler so, and the
+  // chunk loops become scalar loops instead of exec-masked ones.
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int c_local = wave / wpc;
   const int part = wave - c_local * wpc;
   const uint32_t chain = blockIdx.x * (uint32_t)cpb + (uint32_t)c_local;
@@ -217,18 +258,47 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   __syncthreads();
   if (lead) chain_init(Ap, chain, st, U);
 
-  const uint32_t chunk = 64u * OPT;
   const uint32_t tile_len = H.tile_len;
-  const uint32_t n4 = tile_len / 4;
+  const uint32_t n4 = tile_len / 4;      // float4s per column of a tile
+  const uint32_t tile4 = H.n_cols * n4;  // float4s per tile
   const uint32_t ld4 = H.ld / 4;
+  const uint32_t n_tiles = H.n_pad / tile_len;
   const float4* src = reinterpret_cast<const float4*>(H.data);
-  float4* dst = reinterpret_cast<float4*>(tile);
-  if (H.resident) {
-    // The whole (padded) data set stays in LDS for every iteration.
-    for (uint32_t i = tid; i < H.n_cols * n4; i += blockDim.x) {
-      uint32_t col = i / n4, off = i - col * n4;
-      dst[i] = src[(size_t)col * ld4 + off];
+  const uint32_t wave_base = (uint32_t)(wave * 64);
+  float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
+
+  // Asynchronous tile copy: global_load_lds moves 16 bytes per lane from
+  // HBM straight into LDS (no VGPR staging); lane l of wave w writes float4
+  // i0 + l where i0 = k*blockDim + 64*w is wave-uniform.  A tile holds a
+  // multiple of 64 float4s per column, so a wave's 64 slots are all inside
+  // the tile or all outside.
+  // Streaming launches use 1024 threads, so a tile takes at most
+  // kTileIssues copies per wave; the loop is unrolled so that every copy in
+  // flight keeps its own address registers.
+  auto issue_tile = [&](uint32_t ti, float4* dst) {
+#pragma unroll
+    for (uint32_t k = 0; k < kTileIssues; ++k) {
+      // Slots past the tile repeat its last 64 float4s (identical bytes to
+      // identical addresses), which keeps the sequence free of branches.
+      // n4 is a multiple of 64, so the column is uniform across the wave and
+      // the address is a scalar base plus the lane's 16-byte offset.
+      const uint32_t i0 = min(wave_base + k * 1024u, tile4 - 64u);
+      const uint32_t col = i0 / n4, off0 = i0 - col * n4;
+      const float4* base = src + ((size_t)col * ld4 + (size_t)ti * n4 + off0);
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(base + lane),
+          (__attribute__((address_space(3))) void*)(dst + i0), 16, 0, 0);
     }
+  };
+
+  if constexpr (!STREAM) {
+    // The whole (padded) data set stays in LDS for every iteration.
+    for (uint32_t i = tid; i < tile4; i += blockDim.x) {
+      uint32_t col = i / n4, off = i - col * n4;
+      res[i] = src[(size_t)col * ld4 + off];
+    }
+  } else {
+    if (H.stream_mode == 1 && H.n_vm > 0) issue_tile(0, g_tile_a);
   }
   __syncthreads();
 
@@ -236,35 +306,42 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
     // ---- data pass for the chain's current proposal -------------------
     double acc = 0.0;
     if (H.n_vm > 0) {
-      for (uint32_t t0 = 0; t0 < H.n_pad; t0 += tile_len) {
-        if (!H.resident) {
+      if constexpr (!STREAM) {
+        if (active)
+          acc = tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(res), 0,
+                                   part, wpc, U, lane);
+      } else if (H.stream_mode == 1) {
+        // Double buffering over an even number of tiles: tile ti is in A
+        // when the pair starts; the copy of ti+1 into B overlaps the pass
+        // over A, and the copy of ti+2 (or tile 0 of the next iteration,
+        // the data never change) into A overlaps the pass over B.
+        for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
+          issue_tile(ti + 1, g_tile_b);
+          if (active)
+            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
+                                      ti * tile_len, part, wpc, U, lane);
           __syncthreads();
-          for (uint32_t i = tid; i < H.n_cols * n4; i += blockDim.x) {
-            uint32_t col = i / n4, off = i - col * n4;
-            dst[i] = src[(size_t)col * ld4 + t0 / 4 + off];
-          }
+          if (ti + 2 < n_tiles)
+            issue_tile(ti + 2, g_tile_a);
+          else if (it < H.n_iter)
+            issue_tile(0, g_tile_a);
+          if (active)
+            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_b),
+                                      (ti + 1) * tile_len, part, wpc, U, lane);
           __syncthreads();
         }
-        if (active) {
-          const uint32_t nchunks = tile_len / chunk;
-          for (uint32_t ck = part; ck < nchunks; ck += wpc) {
-            const uint32_t base = ck * chunk;
-            float T[OPT];
-            vm_run<NS, OPT>((vm_prog_t)H.vm, H.n_vm, U, tile + base, tile_len, lane, T);
-            const uint32_t first = t0 + base;
-            float s = 0.0f;
-            if (first + chunk <= H.n_obs) {
-#pragma unroll
-              for (int k = 0; k < OPT; ++k) s += T[k];
-            } else {
-#pragma unroll
-              for (int k = 0; k < OPT; ++k) {
-                uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
-                s += obs < H.n_obs ? T[k] : 0.0f;
-              }
-            }
-            acc += (double)s;
+      } else {
+        // Rows too wide for two buffers: synchronous copy into A.
+        for (uint32_t ti = 0; ti < n_tiles; ++ti) {
+          __syncthreads();
+          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
+            uint32_t col = i / n4, off = i - col * n4;
+            g_tile_a[i] = src[(size_t)col * ld4 + (size_t)ti * n4 + off];
           }
+          __syncthreads();
+          if (active)
+            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
+                                      ti * tile_len, part, wpc, U, lane);
         }
       }
     }
@@ -284,7 +361,11 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   if (lead) chain_finish(Ap, chain, st);
 }
 
-#define GPUM_INST(NS, OPT) template __global__ void chain_kernel<NS, OPT>(const ChainArgs*, ChainHot);
+#define GPUM_INST(NS, OPT)                                                   \
+  template __global__ void chain_kernel<NS, OPT, false>(const ChainArgs*,   \
+                                                        ChainHot);          \
+  template __global__ void chain_kernel<NS, OPT, true>(const ChainArgs*, ChainHot);
+GPUM_INST(0, 16)
 GPUM_INST(1, 16)
 GPUM_INST(3, 16)
 GPUM_INST(7, 8)
@@ -297,11 +378,18 @@ hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
                                hipStream_t s) {
   dim3 grid((h.n_chains + h.cpb - 1) / h.cpb);
   dim3 block(h.cpb * h.wpc * 64);
-#define GPUM_CASE(NS_, OPT_)                                                  \
-  if (ns == NS_ && opt == OPT_) {                                             \
-    hipLaunchKernelGGL((chain_kernel<NS_, OPT_>), grid, block, lds_bytes, s, dev_args, h); \
-    return hipGetLastError();                                                 \
+  const bool stream = h.stream_mode != 0;
+#define GPUM_CASE(NS_, OPT_)                                                 \
+  if (ns == NS_ && opt == OPT_) {                                            \
+    if (stream)                                                              \
+      hipLaunchKernelGGL((chain_kernel<NS_, OPT_, true>), grid, block,       \
+                         lds_bytes, s, dev_args, h);                         \
+    else                                                                     \
+      hipLaunchKernelGGL((chain_kernel<NS_, OPT_, false>), grid, block,      \
+                         lds_bytes, s, dev_args, h);                         \
+    return hipGetLastError();                                                \
   }
+  GPUM_CASE(0, 16)
   GPUM_CASE(1, 16)
   GPUM_CASE(3, 16)
   GPUM_CASE(7, 8)
@@ -313,10 +401,16 @@ hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
 
 hipError_t set_chain_kernel_lds_limit(size_t bytes) {
   hipError_t e = hipSuccess;
-#define GPUM_ATTR(NS_, OPT_)                                                       \
-  e = hipFuncSetAttribute((const void*)chain_kernel<NS_, OPT_>,                   \
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes); \
+#define GPUM_ATTR(NS_, OPT_)                                                 \
+  e = hipFuncSetAttribute((const void*)chain_kernel<NS_, OPT_, false>,       \
+                          hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                          (int)bytes);                                       \
+  if (e != hipSuccess) return e;                                             \
+  e = hipFuncSetAttribute((const void*)chain_kernel<NS_, OPT_, true>,        \
+                          hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                          (int)(bytes - kStreamStaticLds));                  \
   if (e != hipSuccess) return e;
+  GPUM_ATTR(0, 16)
   GPUM_ATTR(1, 16)
   GPUM_ATTR(3, 16)
   GPUM_ATTR(7, 8)

@@ -130,19 +130,34 @@ __device__ __forceinline__ void vm_bin_s(uint32_t slot, const float (&S)[NS][OPT
 // are wave-uniform, so every fetch is a scalar load into SGPRs.
 typedef const __attribute__((address_space(4))) VmIns4* vm_prog_t;
 
+__device__ __forceinline__ VmIns4 vm_fetch(vm_prog_t prog, uint32_t pc) {
+  VmIns4 r;
+  r.op = prog[pc].op;
+  r.slot = prog[pc].slot;
+  r.a = prog[pc].a;
+  r.b = prog[pc].b;
+  return r;
+}
+
 template <int NS, int OPT>
 __device__ __forceinline__ void vm_run(vm_prog_t prog,
                                        uint32_t n_instr,
                                        const float* __restrict__ U,
                                        const float* __restrict__ tile,
                                        uint32_t ld, int lane, float (&T)[OPT]) {
-  float S[NS][OPT];
+  // NS == 0: a program of depth 1 (no PUSH, no binary op on the stack);
+  // those cases compile to nothing and the spill registers do not exist.
+  float S[NS > 0 ? NS : 1][OPT];
   float D[OPT];
-#pragma unroll
-  for (int k = 0; k < OPT; ++k) T[k] = 0.0f;
+  // The first instruction of every program is a LOAD (the host lowering
+  // starts from an empty stack), so T needs no initialisation.
+  // Software-pipelined fetch: the next instruction's scalar load is issued
+  // before the current one executes (the program carries one padding
+  // instruction past its end).
+  VmIns4 nxt = vm_fetch(prog, 0);
   for (uint32_t pc = 0; pc < n_instr; ++pc) {
-    const uint32_t op = prog[pc].op, slot = prog[pc].slot, a = prog[pc].a,
-                   b = prog[pc].b;
+    const uint32_t op = nxt.op, slot = nxt.slot, a = nxt.a, b = nxt.b;
+    nxt = vm_fetch(prog, pc + 1);
     switch (op) {
       case VM_LOAD_U: {
         float u = U[a];
@@ -153,33 +168,41 @@ __device__ __forceinline__ void vm_run(vm_prog_t prog,
       case VM_LOAD_D:
         vm_load_col<OPT>(tile + a * ld, lane, T);
         break;
-      case VM_PUSH_U: {
-        vm_spill<0, NS, OPT>(slot, S, T);
-        float u = U[a];
+      case VM_PUSH_U:
+        if constexpr (NS > 0) {
+          vm_spill<0, NS, OPT>(slot, S, T);
+          float u = U[a];
 #pragma unroll
-        for (int k = 0; k < OPT; ++k) T[k] = u;
+          for (int k = 0; k < OPT; ++k) T[k] = u;
+        }
         break;
-      }
       case VM_PUSH_D:
-        vm_spill<0, NS, OPT>(slot, S, T);
-        vm_load_col<OPT>(tile + a * ld, lane, T);
+        if constexpr (NS > 0) {
+          vm_spill<0, NS, OPT>(slot, S, T);
+          vm_load_col<OPT>(tile + a * ld, lane, T);
+        }
         break;
       case VM_ADD_S:
-        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s + t; });
+        if constexpr (NS > 0)
+          vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s + t; });
         break;
       case VM_SUB_S:
-        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s - t; });
+        if constexpr (NS > 0)
+          vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s - t; });
         break;
       case VM_MUL_S:
-        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s * t; });
+        if constexpr (NS > 0)
+          vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s * t; });
         break;
       case VM_DIV_S:
-        vm_bin_s<0, NS, OPT>(slot, S, T,
-                             [](float s, float t) { return s * vm_rcp(t); });
+        if constexpr (NS > 0)
+          vm_bin_s<0, NS, OPT>(slot, S, T,
+                               [](float s, float t) { return s * vm_rcp(t); });
         break;
       case VM_POW_S:
-        vm_bin_s<0, NS, OPT>(slot, S, T,
-                             [](float s, float t) { return vm_pow(s, t); });
+        if constexpr (NS > 0)
+          vm_bin_s<0, NS, OPT>(slot, S, T,
+                               [](float s, float t) { return vm_pow(s, t); });
         break;
 #define GPUM_UOP(NAME, EXPR)                          \
   case NAME: {                                        \

@@ -152,6 +152,7 @@ struct Variant {
 
 Variant pick_variant(int vm_depth) {
   int need = vm_depth > 1 ? vm_depth - 1 : 0;
+  if (need == 0) return {0, 16};
   if (need <= 1) return {1, 16};
   if (need <= 3) return {3, 16};
   if (need <= 7) return {7, 8};
@@ -257,6 +258,8 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
     vm4.push_back(vm_a(ins));
     vm4.push_back(vm_b(ins));
   }
+  // One padding instruction: the kernel prefetches pc + 1.
+  for (int k = 0; k < 4; ++k) vm4.push_back(k == 0 ? (uint32_t)VM_NUM_OPS : 0u);
   m->n_uniform_pad = (uint32_t)std::max(1, m->L.n_uniform);
   std::vector<float> u0(m->n_uniform_pad, 0.0f);
   for (auto& kv : m->L.const_uniforms) u0[kv.first] = kv.second;
@@ -275,7 +278,8 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
 
 struct Tiling {
   Variant v;
-  uint32_t cpb, wpc, chunk, tile_len, n_pad, resident, lds_tile_offset;
+  uint32_t cpb, wpc, chunk, tile_len, n_pad, resident, stream_mode,
+      lds_tile_offset;
   size_t lds_bytes;
 };
 
@@ -305,7 +309,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   // Wide rows: fewer waves per chain so one chunk per wave fits in LDS.
   {
     const uint32_t ncol0 = std::max(1u, m->n_cols_total);
-    const size_t tile_cap = g_dev.lds_max / 2;
+    const size_t tile_cap = (size_t)kTileF4 * 16;
     while (wpc > 1 && (size_t)t->chunk * wpc * ncol0 * 4 > tile_cap) {
       wpc /= 2;
       if (!(ext && ext->chains_per_block > 0))
@@ -324,25 +328,43 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   const size_t u_bytes = (size_t)cpb * m->n_uniform_pad * 4;
   size_t head = (state_bytes + part_bytes + u_bytes + 15) / 16 * 16;
   t->lds_tile_offset = (uint32_t)head;
-  const size_t budget = g_dev.lds_max - head;
   const uint32_t step = t->chunk * wpc;  // one chunk per wave of a chain
+  const size_t tile_floats = (size_t)kTileF4 * 4;  // one static buffer
   uint32_t n_res = round_up(std::max(1u, m->n_obs), step);
-  if ((size_t)n_res * ncol * 4 <= std::min(budget, (size_t)96 * 1024)) {
+  if ((size_t)n_res * ncol * 4 <= (size_t)96 * 1024 &&
+      head + (size_t)n_res * ncol * 4 <= g_dev.lds_max) {
+    // Resident: the whole padded data set in dynamic LDS.
     t->resident = 1;
+    t->stream_mode = 0;
     t->tile_len = n_res;
     t->n_pad = n_res;
-  } else {
-    t->resident = 0;
-    uint32_t mult = (uint32_t)std::max<size_t>(
-        1, (size_t)16384 / ((size_t)ncol * step));
-    t->tile_len = step * mult;
-    if ((size_t)t->tile_len * ncol * 4 > budget) {
-      set_err("data rows too wide for the LDS tile (%u columns)", ncol);
-      return false;
-    }
-    t->n_pad = round_up(std::max(1u, m->n_obs), t->tile_len);
+    t->lds_bytes = head + (size_t)n_res * ncol * 4;
+    return true;
   }
-  t->lds_bytes = head + (size_t)t->tile_len * ncol * 4;
+  if (head + kStreamStaticLds > g_dev.lds_max) {
+    set_err("per-chain state too large for the LDS budget (%u chains per "
+            "block, %u parameters)", cpb, m->n_params);
+    return false;
+  }
+  if ((size_t)step * ncol > tile_floats) {
+    set_err("data rows too wide for the LDS tile (%u columns)", ncol);
+    return false;
+  }
+  t->resident = 0;
+  // The double-buffered copy is laid out for 1024-thread workgroups; other
+  // shapes copy synchronously.
+  t->stream_mode = cpb * wpc == 16 ? 1 : 2;
+  // Tiles of as many steps as one static buffer holds, an even number of
+  // them (the double buffer alternates in pairs), as little padding as
+  // that allows.
+  uint32_t max_mult = (uint32_t)(tile_floats / ((size_t)step * ncol));
+  uint32_t n_steps = (std::max(1u, m->n_obs) + step - 1) / step;
+  uint32_t n_tiles = (n_steps + max_mult - 1) / max_mult;
+  if (t->stream_mode == 1) n_tiles += n_tiles & 1u;
+  uint32_t mult = (n_steps + n_tiles - 1) / n_tiles;
+  t->tile_len = step * std::max(1u, mult);
+  t->n_pad = t->tile_len * n_tiles;
+  t->lds_bytes = head;
   return true;
 }
 
@@ -566,7 +588,7 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   h.cpb = t.cpb;
   h.wpc = t.wpc;
   h.tile_len = t.tile_len;
-  h.resident = t.resident;
+  h.stream_mode = t.stream_mode;
   h.lds_tile_offset = t.lds_tile_offset;
   h.state_words = 3 * np + 3;
 
