@@ -38,7 +38,15 @@ class RunExt(C.Structure):
                 ("chains_per_block", C.c_int32),
                 ("waves_per_chain", C.c_int32),
                 ("precompute_data", C.c_int32), ("fuse", C.c_int32),
-                ("kernel_ms", C.c_double)]
+                ("kernel_ms", C.c_double),
+                ("iter_offset", C.c_uint32), ("swap_every", C.c_uint32),
+                ("swap_group", C.c_uint32), ("adapt_batches", C.c_uint32),
+                ("batch0", C.c_uint32),
+                ("init_scale", C.POINTER(C.c_double)),
+                ("swap_hist", C.POINTER(C.c_uint8)),
+                ("acc_hist", C.POINTER(C.c_float)),
+                ("final_sd", C.POINTER(C.c_double)),
+                ("final_scale", C.POINTER(C.c_double))]
 
 
 class ModelInfo(C.Structure):
@@ -140,6 +148,10 @@ class RunOutput:
     trace_accept: Optional[np.ndarray] = None
     init_log_post: Optional[np.ndarray] = None
     kernel_ms: float = 0.0
+    swap_hist: Optional[np.ndarray] = None    # (n_batches, n_chains) uint8
+    acc_hist: Optional[np.ndarray] = None     # (n_batches, n_chains)
+    final_sd: Optional[np.ndarray] = None
+    final_scale: Optional[np.ndarray] = None
 
 
 def _collect(res: ChainResult) -> RunOutput:
@@ -154,11 +166,31 @@ def _collect(res: ChainResult) -> RunOutput:
 
 
 def _ext(n_iter, n_chains, n_params, trace, final_state, stream_id, tiling,
-         precompute_data, fuse):
+         precompute_data, fuse, batches=None):
     ext = RunExt()
     ext.struct_size = C.sizeof(RunExt)
     ext.stream_id = stream_id
     keep = {}
+    if batches:
+        ext.iter_offset = batches.get("iter_offset", 0)
+        ext.swap_every = batches.get("swap_every", 0)
+        ext.swap_group = batches.get("swap_group", 0)
+        ext.adapt_batches = batches.get("adapt_batches", 0)
+        ext.batch0 = batches.get("batch0", 0)
+        if batches.get("init_scale") is not None:
+            keep["iscale"] = _f64(batches["init_scale"])
+            ext.init_scale = _p(keep["iscale"], C.c_double)
+        if ext.swap_every:
+            nb = max(1, n_iter // ext.swap_every)
+            keep["nb"] = nb
+            keep["swaph"] = np.zeros(nb * n_chains, np.uint8)
+            keep["acch"] = np.zeros(nb * n_chains, np.float32)
+            ext.swap_hist = _p(keep["swaph"], C.c_uint8)
+            ext.acc_hist = _p(keep["acch"], C.c_float)
+        keep["fsd"] = np.zeros(n_chains * n_params, np.float64)
+        keep["fscale"] = np.zeros(n_chains, np.float64)
+        ext.final_sd = _p(keep["fsd"], C.c_double)
+        ext.final_scale = _p(keep["fscale"], C.c_double)
     if trace:
         keep["plp"] = np.zeros(n_iter * n_chains, np.float64)
         keep["prop"] = np.zeros(n_iter * n_chains * n_params, np.float64)
@@ -187,6 +219,12 @@ def _finish(out: RunOutput, ext, keep, n_iter, n_chains, n_params):
         out.init_log_post = keep["init"]
     if "final" in keep:
         out.final_state = keep["final"].reshape(n_chains, n_params)
+    if "swaph" in keep:
+        out.swap_hist = keep["swaph"].reshape(keep["nb"], n_chains)
+        out.acc_hist = keep["acch"].reshape(keep["nb"], n_chains)
+    if "fsd" in keep:
+        out.final_sd = keep["fsd"].reshape(n_chains, n_params)
+        out.final_scale = keep["fscale"]
     return out
 
 
@@ -194,8 +232,10 @@ def run_chains(code, consts, prior_code, prior_consts, n_params, data, n_cols,
                n_obs, init, proposal_sd, temperatures, n_iter, seed,
                proposal_mode=0, gamma=0.0, de_noise=0.0, proposal_l=None,
                trace=False, final_state=False, stream_id=0, tiling=None,
-               precompute_data=None, fuse=None) -> RunOutput:
-    """gpu_metropolis_chain (chain_kernel.rs:468-488) on the HIP backend."""
+               precompute_data=None, fuse=None, batches=None) -> RunOutput:
+    """gpu_metropolis_chain (chain_kernel.rs:468-488) on the HIP backend.
+    `batches` (dict: swap_every, swap_group, adapt_batches, iter_offset,
+    batch0, init_scale) fuses the PT / adaptation batches into the launch."""
     L = lib()
     code, prior_code = _u32(code), _u32(prior_code)
     consts, prior_consts = _f64(consts), _f64(prior_consts)
@@ -204,7 +244,7 @@ def run_chains(code, consts, prior_code, prior_consts, n_params, data, n_cols,
     pl = None if proposal_l is None else _f64(proposal_l)
     n_chains = init.size // n_params
     ext, keep = _ext(n_iter, n_chains, n_params, trace, final_state, stream_id,
-                     tiling, precompute_data, fuse)
+                     tiling, precompute_data, fuse, batches)
     res = ChainResult()
     rc = L.gpum_hip_run_chains_ex(
         _p(code, C.c_uint32), code.size, _p(consts, C.c_double), consts.size,
@@ -263,13 +303,13 @@ class DeviceModel:
     def run(self, init, proposal_sd, temperatures, n_iter, seed,
             proposal_mode=0, gamma=0.0, de_noise=0.0, proposal_l=None,
             trace=False, final_state=False, stream_id=0,
-            tiling=None) -> RunOutput:
+            tiling=None, batches=None) -> RunOutput:
         np_ = self.n_params
         init, sd, temps = _f64(init), _f64(proposal_sd), _f64(temperatures)
         pl = None if proposal_l is None else _f64(proposal_l)
         n_chains = init.size // np_
         ext, keep = _ext(n_iter, n_chains, np_, trace, final_state, stream_id,
-                         tiling, None, None)
+                         tiling, None, None, batches)
         res = ChainResult()
         rc = self._L.gpum_hip_model_run(
             self._h, _p(init, C.c_double), n_chains, _p(sd, C.c_double),

@@ -85,7 +85,7 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                    temperatures=None, swap_every=None, gamma=None,
                    de_noise: float = 1e-3, de_every=None, de_sync=False,
                    seed: int = 1, backend: str = "hip",
-                   tiling=None) -> Fit:
+                   tiling=None, ladder_size=None) -> Fit:
     if backend not in ("hip", "auto"):
         raise ValueError(f"backend '{backend}' is not available here; this "
                          "package is the hip backend")
@@ -119,21 +119,13 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
 
     dm = DeviceModel(model.loglik.code, model.loglik.consts, model.prior.code,
                      model.prior.consts, np_, X, model.n_cols, n_obs)
-    kernel_ms = [0.0]
-
-    def call(init_m, sd_m, iters, s, temps=None, mode=0, g=0.0, noise=0.0,
-             L=None):
-        t = np.ones(n_chains) if temps is None else temps
-        r = dm.run(init_m, sd_m, t, iters, int(s) & 0xFFFFFFFF,
-                   proposal_mode=mode, gamma=g, de_noise=noise, proposal_l=L,
-                   final_state=True, tiling=tiling)
-        kernel_ms[0] += r.kernel_ms
-        return r
+    call = _Caller(dm, n_chains, tiling)
 
     try:
         if method == "pt":
             fit = _pt(call, model, init_mat, sd_mat, temperatures, n_iter,
-                      warmup, swap_every, adapt, seed, n_chains)
+                      warmup, swap_every, adapt, seed, n_chains,
+                      ladder_size=ladder_size, dm=dm, tiling=tiling)
         elif method == "de":
             if n_chains < 4:
                 raise ValueError('`method = "de"` needs at least 4 chains.')
@@ -152,8 +144,28 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                       n_iter, warmup, n_chains, np_, model.params, seed)
     finally:
         dm.close()
-    fit.kernel_ms = kernel_ms[0]
+    fit.kernel_ms = call.kernel_ms
     return fit
+
+
+class _Caller:
+    """One batch on the device-resident model: the reference's `rust_call`
+    closure (R/api.R:379-393)."""
+
+    def __init__(self, dm, n_chains, tiling):
+        self.dm, self.n_chains, self.tiling = dm, n_chains, tiling
+        self.kernel_ms = 0.0
+
+    def __call__(self, init_m, sd_m, iters, s, temps=None, mode=0, g=0.0,
+                 noise=0.0, L=None, iter_offset=0):
+        t = np.ones(self.n_chains) if temps is None else temps
+        batches = {"iter_offset": iter_offset} if iter_offset else None
+        r = self.dm.run(init_m, sd_m, t, iters, int(s) & 0xFFFFFFFF,
+                        proposal_mode=mode, gamma=g, de_noise=noise,
+                        proposal_l=L, final_state=True, tiling=self.tiling,
+                        batches=batches)
+        self.kernel_ms += r.kernel_ms
+        return r
 
 
 @dataclass
@@ -252,78 +264,112 @@ def _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup, seed,
 
 
 def _pt(call, model, init_mat, sd_mat, temperatures, n_iter, warmup,
-        swap_every, adapt, seed, n_chains) -> Fit:
-    """R/api.R:395-440 and R/adaptive.R:272-413."""
+        swap_every, adapt, seed, n_chains, ladder_size=None, dm=None,
+        tiling=None) -> Fit:
+    """R/api.R:395-440 and R/adaptive.R:272-413.
+
+    With ladders of at most 16 chains the whole orchestration -- batches,
+    per-chain adaptation during the warmup batches, the alternating swap
+    sweeps -- runs inside one kernel launch.  Longer ladders fall back to one
+    launch per batch with the sweep on the host; both draw the same Philox
+    stream (one stream for the run, iteration offsets per batch)."""
     np_ = model.n_params
+    G = n_chains if ladder_size is None else int(ladder_size)
+    if G < 1 or n_chains % G != 0:
+        raise ValueError("`ladder_size` must divide the number of chains.")
     if temperatures is None:
-        temps = ad.default_ladder(n_chains)
+        temps = np.tile(ad.default_ladder(G), n_chains // G)
     else:
         temps = np.asarray(temperatures, dtype=np.float64).ravel()
+        if temps.size == G and G != n_chains:
+            temps = np.tile(temps, n_chains // G)
         if temps.size != n_chains:
             raise ValueError("`temperatures` must have one value per chain.")
         if not np.all(np.isfinite(temps)) or np.any(temps <= 0):
             raise ValueError("`temperatures` must be positive and finite.")
     if swap_every is None:
-        swap_every = max(n_chains, 10)
+        swap_every = max(G, 10)
     swap_every = int(swap_every)
     if swap_every < 1:
         raise ValueError("`swap_every` must be a positive integer.")
     sizes, nb = ad.split_batches(n_iter, swap_every)
     wb = max(0, min(nb, warmup // swap_every))
     warmup_used = sum(sizes[:wb])
+    if G <= 16 and dm is not None:
+        r = dm.run(init_mat, sd_mat, temps, n_iter, int(seed) & 0xFFFFFFFF,
+                   final_state=True, tiling=tiling,
+                   batches={"swap_every": swap_every, "swap_group": G,
+                            "adapt_batches": wb if adapt else 0})
+        call.kernel_ms += r.kernel_ms
+        draws = r.draws.astype(np.float64)
+        acc_hist = r.acc_hist.astype(np.float64)
+        swap_hist = r.swap_hist
+        cur_sd, scales = r.final_sd, r.final_scale
+    else:
+        draws, acc_hist, swap_hist, cur_sd, scales = _pt_batched(
+            call, np_, n_chains, init_mat, sd_mat, temps, sizes, wb, adapt,
+            seed, G)
+    kept = draws[warmup_used:]
+    w = np.array(sizes[wb:], dtype=np.float64)
+    if w.sum() > 0:
+        rate = (acc_hist[wb:] * w[:, None]).sum(axis=0) / w.sum()
+    else:
+        rate = acc_hist[-1]
+    pairs = [i for i in range(n_chains) if (i + 1) % G != 0]
+    sh = swap_hist[:, pairs].T.astype(np.float64)
+    sh[sh == 255] = np.nan
+    return Fit(kept, rate, "pt", kept.shape[0], warmup_used + kept.shape[0],
+               warmup_used, n_chains, np_, model.params, seed,
+               temperatures=temps,
+               adaptation={"final_proposal_sd": cur_sd, "final_scales": scales,
+                           "n_batches": nb, "batch_sizes": sizes,
+                           "accept_history": acc_hist.T,
+                           "swap_history": sh})
+
+
+def _pt_batched(call, np_, n_chains, init_mat, sd_mat, temps, sizes, wb,
+                adapt, seed, G):
+    """One launch per batch; adaptation and the swap sweep on the host."""
+    nb = len(sizes)
     states = ad.WelfordState.init(n_chains, np_, sd_mat ** 2)
     scales = np.ones(n_chains)
-    cur_init = init_mat.copy()
+    cur = init_mat.copy()
     cur_sd = sd_mat.copy()
-    kept = []
-    acc_acc = np.zeros(n_chains)
-    iters_acc = 0
-    pairs = n_chains - 1
-    swap_hist = np.full((max(pairs, 0), nb), np.nan)
-    acc_hist = np.full((n_chains, nb), np.nan)
-    r = None
+    draws = []
+    acc_hist = np.zeros((nb, n_chains))
+    swap_hist = np.full((nb, n_chains), 255, np.uint8)
+    t0 = 0
     for b in range(1, nb + 1):
-        bsize = sizes[b - 1]
-        r = call(cur_init, cur_sd, bsize, seed + b - 1, temps=temps)
+        r = call(cur, cur_sd, sizes[b - 1], seed, temps=temps, iter_offset=t0)
         X = r.draws.astype(np.float64)
-        lp = r.last_log_post.copy()
-        acc_hist[:, b - 1] = r.accept_rate
-        in_warmup = b <= wb
-        if not in_warmup:
-            kept.append(X)
-            acc_acc += bsize * r.accept_rate
-            iters_acc += bsize
-        if in_warmup and adapt:
+        draws.append(X)
+        acc = r.accept_rate.astype(np.float64)
+        acc_hist[b - 1] = acc
+        if b <= wb and adapt:
             states.update_batch(X)
-            scales = ad.robbins_monro_scale(scales, r.accept_rate.astype(np.float64),
-                                            b, np_)
+            scales = ad.robbins_monro_scale(scales, acc, b, np_)
             cov = states.covariance()
             cur_sd = scales[:, None] * np.sqrt(
                 np.maximum(np.diagonal(cov, axis1=1, axis2=2), 1e-12))
-        cur_init = X[-1].copy()
-        if pairs > 0:
-            order = (list(range(0, pairs, 2)) if b % 2 == 1
-                     else list(range(min(1, pairs - 1), pairs, 2)))
-            u = uniforms(seed, b, len(order))
-            att = np.zeros(pairs)
-            acc = np.zeros(pairs)
+        cur = r.final_state.copy()
+        lp = r.last_log_post.copy()
+        for g0 in range(0, n_chains, G):
+            pairs = G - 1
+            if pairs < 1:
+                continue
+            start = 0 if b % 2 == 1 else (1 if pairs >= 2 else 0)
+            order = list(range(start, pairs, 2))
+            u = uniforms(seed, b, len(order), group=g0 // G)
             for k, i in enumerate(order):
-                delta = (lp[i + 1] - lp[i]) * (1 / temps[i] - 1 / temps[i + 1])
-                att[i] += 1
-                if math.isfinite(delta) and math.log(u[k]) < delta:
-                    cur_init[[i, i + 1]] = cur_init[[i + 1, i]]
-                    lp[[i, i + 1]] = lp[[i + 1, i]]
-                    acc[i] += 1
-            with np.errstate(invalid="ignore", divide="ignore"):
-                swap_hist[:, b - 1] = np.where(att > 0, acc / np.maximum(att, 1), np.nan)
-    d = np.concatenate(kept) if kept else np.zeros((0, n_chains, np_))
-    rate = acc_acc / iters_acc if iters_acc > 0 else r.accept_rate
-    return Fit(d, rate, "pt", d.shape[0], warmup_used + d.shape[0], warmup_used,
-               n_chains, np_, model.params, seed, temperatures=temps,
-               adaptation={"final_proposal_sd": cur_sd, "final_scales": scales,
-                           "n_batches": nb, "batch_sizes": sizes,
-                           "accept_history": acc_hist,
-                           "swap_history": swap_hist})
+                lo, hi = g0 + i, g0 + i + 1
+                delta = (lp[hi] - lp[lo]) * (1 / temps[lo] - 1 / temps[hi])
+                ok = math.isfinite(delta) and math.log(u[k]) < delta
+                if ok:
+                    cur[[lo, hi]] = cur[[hi, lo]]
+                    lp[[lo, hi]] = lp[[hi, lo]]
+                swap_hist[b - 1, lo] = 1 if ok else 0
+        t0 += sizes[b - 1]
+    return np.concatenate(draws), acc_hist, swap_hist, cur_sd, scales
 
 
 def _de(call, model, init_mat, sd_mat, gamma, de_noise, n_iter, warmup,

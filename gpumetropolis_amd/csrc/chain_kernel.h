@@ -46,6 +46,19 @@ struct ChainArgs {
   double* last_log_post;
   double* final_state;
   double* init_log_post;
+  // Batches inside one launch (parallel tempering and per-chain adaptation,
+  // R/adaptive.R:272-413): swap_every = 0 disables both.
+  uint32_t iter_offset;    // Philox iteration counter of it = 0
+  uint32_t swap_every;
+  uint32_t n_batches;
+  uint32_t swap_group;     // chains per temperature ladder (<= chains per block)
+  uint32_t adapt_batches;  // leading batches with Robbins-Monro adaptation
+  uint32_t batch0;         // global index of batch 0 (for resumed runs)
+  const double* init_scale;  // per-chain starting RM scale or null (1.0)
+  uint8_t* swap_hist;      // n_batches x n_chains: 1/0 per attempted pair
+  float* acc_hist;         // n_batches x n_chains: batch acceptance
+  double* final_sd;        // n_chains x np
+  double* final_scale;     // n_chains
   double* trace_plp;
   double* trace_prop;
   uint8_t* trace_accept;
@@ -80,10 +93,13 @@ struct ChainHot {
   uint32_t stream_mode;
   uint32_t lds_tile_offset;
   uint32_t state_words;
+  uint32_t swap_every;
+  uint32_t n_batches;
+  uint32_t swap_group;
 };
 
 // Streamed tiles: two static LDS buffers of kTileF4 float4s each.
-constexpr uint32_t kTileF4 = 4800;
+constexpr uint32_t kTileF4 = 4608;
 constexpr size_t kStreamStaticLds = 2 * (size_t)kTileF4 * 16;
 // Copies per wave that fill one buffer with 1024 threads.
 constexpr uint32_t kTileIssues = (kTileF4 + 1023) / 1024;

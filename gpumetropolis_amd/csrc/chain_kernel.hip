@@ -76,118 +76,259 @@ struct StepUniforms {
   }
 };
 
-// Per-chain state in LDS (doubles): state[np], prop[np], z[np], cur,
-// prior_lp, accepts.
-__device__ __forceinline__ int state_words(int np) { return 3 * np + 3; }
+// Per-chain record in LDS (doubles):
+//   state[np] prop[np] z[np] sd[np] wmean[np] wm2[np]
+//   cur prior_lp accepts wn scale batch_accepts
+struct Rec {
+  double* state;
+  double* prop;
+  double* z;
+  double* sd;
+  double* wmean;
+  double* wm2;
+  double* tail;  // cur, prior_lp, accepts, wn, scale, batch_accepts
+  __device__ double& cur() { return tail[0]; }
+  __device__ double& prior_lp() { return tail[1]; }
+  __device__ double& accepts() { return tail[2]; }
+  __device__ double& wn() { return tail[3]; }
+  __device__ double& scale() { return tail[4]; }
+  __device__ double& batch_acc() { return tail[5]; }
+};
 
-// The fused step of one chain, run by its lead lane between data passes:
-// finish iteration it-1 with the data term `ll` of the proposal, then
-// prepare iteration it (when it < n_iter).  FP64 throughout; the only FP32
-// values are the uniform-table entries the hoist program writes.
-__device__ __noinline__ void chain_step(const ChainArgs* __restrict__ A,
-                                        uint32_t it, uint32_t chain,
-                                        double* st, double ll, float* U) {
+__device__ __forceinline__ Rec make_rec(double* st, int np) {
+  Rec r;
+  r.state = st;
+  r.prop = st + np;
+  r.z = st + 2 * np;
+  r.sd = st + 3 * np;
+  r.wmean = st + 4 * np;
+  r.wm2 = st + 5 * np;
+  r.tail = st + 6 * np;
+  return r;
+}
+
+// Batch structure of a run split into batches of `every` iterations, the
+// last absorbing the remainder (R/adaptive.R:284-289).  Returns the 1-based
+// batch that ends after iteration t, or 0.
+__device__ __forceinline__ uint32_t batch_ending_at(uint32_t t, uint32_t n_iter,
+                                                    uint32_t every,
+                                                    uint32_t n_batches) {
+  if (every == 0) return 0;
+  if (t + 1 == n_iter) return n_batches;
+  if ((t + 1) % every == 0 && (t + 1) / every < n_batches) return (t + 1) / every;
+  return 0;
+}
+
+// Finish iteration it-1 with the data term `ll` of the proposal
+// (cpu_native.rs:785-799): tempered Metropolis decision in FP64, record the
+// f32 draw; during adaptation batches, fold the draw into the chain's
+// Welford accumulator (R/adaptive.R:31-42, diagonal part).
+__device__ __noinline__ void chain_finish_iter(const ChainArgs* __restrict__ A,
+                                               uint32_t it, uint32_t chain,
+                                               double* st, double ll) {
   const int np = (int)A->n_params;
   const uint32_t C = A->n_chains;
   const uint32_t T_iter = A->n_iter;
-  const uint32_t mode = A->mode;
-  const uint32_t spi = mode == 1 ? 2u * np + 4u : 2u * np + 1u;
-  const double two_pi = 6.283185307179586;
-  double* state = st;
-  double* prop = st + np;
-  double* zbuf = st + 2 * np;
-  double& cur = st[3 * np];
-  double& prior_lp = st[3 * np + 1];
-  double& accepts = st[3 * np + 2];
-
-  double plp = ll + prior_lp;
+  const uint32_t spi = A->mode == 1 ? 2u * np + 4u : 2u * np + 1u;
+  Rec r = make_rec(st, np);
+  double plp = ll + r.prior_lp();
   if (it == 0) {
-    cur = plp;
+    r.cur() = plp;
     if (A->init_log_post) A->init_log_post[chain] = plp;
-  } else {
-    // cpu_native.rs:785-799 / chain_kernel.rs:328-347
-    const uint32_t t = it - 1;
-    StepUniforms us{A->seed, A->stream_id, t, chain};
-    double u = us.get(spi - 1);
-    bool acc_ok = log(u) < (plp - cur) / A->temperatures[chain];
-    if (A->trace_plp) A->trace_plp[(size_t)t * C + chain] = plp;
-    if (A->trace_prop)
-      for (int j = 0; j < np; ++j)
-        A->trace_prop[((size_t)t * C + chain) * np + j] = prop[j];
-    if (acc_ok) {
-      for (int j = 0; j < np; ++j) state[j] = prop[j];
-      cur = plp;
-      accepts += 1.0;
-    }
-    if (A->trace_accept) A->trace_accept[(size_t)t * C + chain] = acc_ok ? 1 : 0;
+    return;
+  }
+  const uint32_t t = it - 1;
+  StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, chain};
+  double u = us.get(spi - 1);
+  bool acc_ok = log(u) < (plp - r.cur()) / A->temperatures[chain];
+  if (A->trace_plp) A->trace_plp[(size_t)t * C + chain] = plp;
+  if (A->trace_prop)
     for (int j = 0; j < np; ++j)
-      A->draws[t + (size_t)T_iter * (chain + (size_t)C * j)] = (float)state[j];
+      A->trace_prop[((size_t)t * C + chain) * np + j] = r.prop[j];
+  if (acc_ok) {
+    for (int j = 0; j < np; ++j) r.state[j] = r.prop[j];
+    r.cur() = plp;
+    r.accepts() += 1.0;
+    r.batch_acc() += 1.0;
   }
-  if (it < T_iter) {
-    // cpu_native.rs:726-784 / chain_kernel.rs:210-296
-    const uint32_t t = it;
-    StepUniforms us{A->seed, A->stream_id, t, chain};
-    const double* sd = A->proposal_sd + (size_t)chain * np;
-    if (mode == 0) {
+  if (A->trace_accept) A->trace_accept[(size_t)t * C + chain] = acc_ok ? 1 : 0;
+  for (int j = 0; j < np; ++j)
+    A->draws[t + (size_t)T_iter * (chain + (size_t)C * j)] = (float)r.state[j];
+  if (A->adapt_batches > 0 && A->swap_every > 0) {
+    const uint32_t b = min(t / A->swap_every + 1, A->n_batches);
+    if (b <= A->adapt_batches) {
+      // Welford over the recorded f32 draws, as the reference's host sees
+      // them (R/adaptive.R:31-42).
+      r.wn() += 1.0;
       for (int j = 0; j < np; ++j) {
-        double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-        double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-        prop[j] = state[j] + sd[j] * z;
-      }
-    } else if (mode == 2) {
-      const double* L = A->proposal_l + (size_t)chain * np * np;
-      for (int j = 0; j < np; ++j) {
-        double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-        zbuf[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-      }
-      for (int k = 0; k < np; ++k) {
-        double a = state[k];
-        for (int j = 0; j <= k; ++j) a += L[k * np + j] * zbuf[j];
-        prop[k] = a;
-      }
-    } else {
-      // DE path A: the pair comes from the frozen batch-start population
-      // in `init`; with probability 0.1 the scale is 1.
-      uint32_t ia = (uint32_t)(us.get(0) * (double)C);
-      if (ia >= C) ia = C - 1;
-      uint32_t ib = (uint32_t)(us.get(1) * (double)C);
-      if (ib >= C) ib = C - 1;
-      if (ib == ia) ib = (ib + 1) % C;
-      double g = us.get(2) < 0.1 ? 1.0 : A->gamma;
-      for (int j = 0; j < np; ++j) {
-        double u1 = us.get(3 + 2 * j), u2 = us.get(4 + 2 * j);
-        double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-        double diff = A->init[(size_t)ia * np + j] - A->init[(size_t)ib * np + j];
-        prop[j] = state[j] + g * diff + A->de_noise * sd[j] * z;
+        double x = (double)(float)r.state[j];
+        double d1 = x - r.wmean[j];
+        r.wmean[j] += d1 / r.wn();
+        r.wm2[j] += d1 * (x - r.wmean[j]);
       }
     }
-    eval_params64(A->hoist, A->n_hoist, A->consts, prop, U);
-    prior_lp = eval_params64(A->prior, A->n_prior, A->prior_consts, prop, nullptr);
   }
+}
+
+// Target acceptance by dimension (R/adaptive.R:69-72).
+__device__ __forceinline__ double target_accept(int d) {
+  const double low[8] = {0.44, 0.35, 0.32, 0.30, 0.28, 0.27, 0.26, 0.25};
+  return (d >= 1 && d <= 8) ? low[d - 1] : 0.234;
+}
+
+// End of batch b for one chain: Robbins-Monro on the batch acceptance and
+// the per-dimension scale sqrt(diag(cov)) (R/adaptive.R:342-356), while b is
+// an adaptation batch; then reset the batch counter.
+__device__ __noinline__ void chain_adapt(const ChainArgs* __restrict__ A,
+                                         uint32_t b, uint32_t bsize,
+                                         uint32_t chain, double* st) {
+  const int np = (int)A->n_params;
+  Rec r = make_rec(st, np);
+  // accept_rate crosses the reference's FFI as f32 (chain_kernel.rs:353).
+  const double acc = (double)((float)r.batch_acc() / (float)bsize);
+  if (A->acc_hist) A->acc_hist[(size_t)(b - 1) * A->n_chains + chain] = (float)acc;
+  r.batch_acc() = 0.0;
+  if (b > A->adapt_batches) return;
+  const double g = pow((double)(A->batch0 + b), -2.0 / 3.0);
+  r.scale() = exp(log(r.scale()) + g * (acc - target_accept(np)));
+  for (int j = 0; j < np; ++j) {
+    // .am_covariance: the seed Sigma (initial sd^2) until two draws.
+    double var = r.wn() >= 2.0 ? r.wm2[j] / (r.wn() - 1.0)
+                               : A->proposal_sd[(size_t)chain * np + j] *
+                                     A->proposal_sd[(size_t)chain * np + j];
+    r.sd[j] = r.scale() * sqrt(fmax(var, 1e-12));
+  }
+}
+
+// Replica-exchange sweep of batch b over the ladder groups of this
+// workgroup (R/adaptive.R:362-391): odd batches try pairs (0,1), (2,3), ...,
+// even batches (1,2), (3,4), ...; a pair swaps states and log-posteriors
+// when log u < (lp[i+1] - lp[i]) (1/T_i - 1/T_{i+1}).  The uniform for the
+// k-th pair of the sweep of group g is word 0 of Philox block
+// (b, k, g, TAG_SWAP).
+__device__ __noinline__ void pt_sweep(const ChainArgs* __restrict__ A,
+                                      uint32_t b, uint32_t chain0, int n_local,
+                                      double* st_all, int SW) {
+  const int np = (int)A->n_params;
+  const uint32_t G = A->swap_group;
+  const uint32_t bg = A->batch0 + b;  // global batch index
+  for (int g0 = 0; g0 + (int)G <= n_local; g0 += (int)G) {
+    const uint32_t gchain0 = chain0 + (uint32_t)g0;
+    if (gchain0 + G > A->n_chains) break;
+    const uint32_t group = gchain0 / G;
+    const int pairs = (int)G - 1;
+    if (pairs < 1) continue;
+    int start = (bg % 2u == 1u) ? 0 : (pairs >= 2 ? 1 : 0);
+    int k = 0;
+    for (int i = start; i < pairs; i += 2, ++k) {
+      U4 w = philox4x32_10(U4{bg, (uint32_t)k, group, TAG_SWAP}, A->seed,
+                           A->stream_id);
+      double u = u32_to_unit(w.x);
+      Rec lo = make_rec(st_all + (g0 + i) * SW, np);
+      Rec hi = make_rec(st_all + (g0 + i + 1) * SW, np);
+      const double t_lo = A->temperatures[gchain0 + i];
+      const double t_hi = A->temperatures[gchain0 + i + 1];
+      double delta = (hi.cur() - lo.cur()) * (1.0 / t_lo - 1.0 / t_hi);
+      bool ok = isfinite(delta) && log(u) < delta;
+      if (ok) {
+        for (int j = 0; j < np; ++j) {
+          double x = lo.state[j];
+          lo.state[j] = hi.state[j];
+          hi.state[j] = x;
+        }
+        double c = lo.cur();
+        lo.cur() = hi.cur();
+        hi.cur() = c;
+      }
+      if (A->swap_hist)
+        A->swap_hist[(size_t)(b - 1) * A->n_chains + gchain0 + i] = ok ? 1 : 0;
+    }
+  }
+}
+
+// Prepare iteration it: Philox uniforms, Box-Muller, the proposal
+// (cpu_native.rs:726-784), the hoisted parameter program into the uniform
+// table and the prior of the proposal.
+__device__ __noinline__ void chain_prepare_iter(const ChainArgs* __restrict__ A,
+                                                uint32_t it, uint32_t chain,
+                                                double* st, float* U) {
+  const int np = (int)A->n_params;
+  const uint32_t C = A->n_chains;
+  const uint32_t mode = A->mode;
+  const double two_pi = 6.283185307179586;
+  Rec r = make_rec(st, np);
+  StepUniforms us{A->seed, A->stream_id, A->iter_offset + it, chain};
+  const double* sd = r.sd;
+  if (mode == 0) {
+    for (int j = 0; j < np; ++j) {
+      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
+      double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+      r.prop[j] = r.state[j] + sd[j] * z;
+    }
+  } else if (mode == 2) {
+    const double* L = A->proposal_l + (size_t)chain * np * np;
+    for (int j = 0; j < np; ++j) {
+      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
+      r.z[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+    }
+    for (int k = 0; k < np; ++k) {
+      double a = r.state[k];
+      for (int j = 0; j <= k; ++j) a += L[k * np + j] * r.z[j];
+      r.prop[k] = a;
+    }
+  } else {
+    // DE path A: the pair comes from the frozen batch-start population in
+    // `init`; with probability 0.1 the scale is 1.
+    uint32_t ia = (uint32_t)(us.get(0) * (double)C);
+    if (ia >= C) ia = C - 1;
+    uint32_t ib = (uint32_t)(us.get(1) * (double)C);
+    if (ib >= C) ib = C - 1;
+    if (ib == ia) ib = (ib + 1) % C;
+    double g = us.get(2) < 0.1 ? 1.0 : A->gamma;
+    for (int j = 0; j < np; ++j) {
+      double u1 = us.get(3 + 2 * j), u2 = us.get(4 + 2 * j);
+      double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+      double diff = A->init[(size_t)ia * np + j] - A->init[(size_t)ib * np + j];
+      r.prop[j] = r.state[j] + g * diff + A->de_noise * sd[j] * z;
+    }
+  }
+  eval_params64(A->hoist, A->n_hoist, A->consts, r.prop, U);
+  r.prior_lp() = eval_params64(A->prior, A->n_prior, A->prior_consts, r.prop, nullptr);
 }
 
 __device__ __noinline__ void chain_init(const ChainArgs* __restrict__ A,
                                         uint32_t chain, double* st, float* U) {
   const int np = (int)A->n_params;
+  Rec r = make_rec(st, np);
   for (int j = 0; j < np; ++j) {
     double v = A->init[(size_t)chain * np + j];
-    st[j] = v;
-    st[np + j] = v;
+    r.state[j] = v;
+    r.prop[j] = v;
+    r.sd[j] = A->proposal_sd[(size_t)chain * np + j];
+    r.wmean[j] = 0.0;
+    r.wm2[j] = 0.0;
   }
-  st[3 * np + 2] = 0.0;
-  eval_params64(A->hoist, A->n_hoist, A->consts, st + np, U);
-  st[3 * np + 1] = eval_params64(A->prior, A->n_prior, A->prior_consts, st + np, nullptr);
+  r.accepts() = 0.0;
+  r.wn() = 0.0;
+  r.scale() = A->init_scale ? A->init_scale[chain] : 1.0;
+  r.batch_acc() = 0.0;
+  eval_params64(A->hoist, A->n_hoist, A->consts, r.prop, U);
+  r.prior_lp() = eval_params64(A->prior, A->n_prior, A->prior_consts, r.prop, nullptr);
 }
 
 __device__ __noinline__ void chain_finish(const ChainArgs* __restrict__ A,
-                                          uint32_t chain, const double* st) {
+                                          uint32_t chain, double* st) {
   const int np = (int)A->n_params;
+  Rec r = make_rec(st, np);
   const uint32_t T_iter = A->n_iter;
   A->accept_rate[chain] =
-      T_iter == 0 ? 0.0f : (float)st[3 * np + 2] / (float)T_iter;
-  A->last_log_post[chain] = st[3 * np];
+      T_iter == 0 ? 0.0f : (float)r.accepts() / (float)T_iter;
+  A->last_log_post[chain] = r.cur();
   if (A->final_state)
-    for (int j = 0; j < np; ++j) A->final_state[(size_t)chain * np + j] = st[j];
+    for (int j = 0; j < np; ++j) A->final_state[(size_t)chain * np + j] = r.state[j];
+  if (A->final_sd)
+    for (int j = 0; j < np; ++j) A->final_sd[(size_t)chain * np + j] = r.sd[j];
+  if (A->final_scale) A->final_scale[chain] = r.scale();
 }
 
 // Streamed tiles live in two static LDS buffers.  Being distinct LDS
@@ -354,8 +495,29 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
     if (lead) {
       double ll = 0.0;
       for (int p = 0; p < wpc; ++p) ll += partials[c_local * wpc + p];
-      chain_step(Ap, it, chain, st, ll, U);
+      chain_finish_iter(Ap, it, chain, st, ll);
     }
+    if (it > 0 && H.swap_every > 0) {
+      const uint32_t b = batch_ending_at(it - 1, H.n_iter, H.swap_every,
+                                         H.n_batches);
+      if (b > 0) {
+        // End of a batch: per-chain adaptation, then the replica-exchange
+        // sweep of the workgroup's ladder groups, before any proposal of
+        // the next batch is drawn.
+        const uint32_t bsize =
+            b == H.n_batches ? H.n_iter - (H.n_batches - 1) * H.swap_every
+                             : H.swap_every;
+        __syncthreads();
+        if (lead) chain_adapt(Ap, b, bsize, chain, st);
+        if (H.swap_group > 1) {
+          __syncthreads();
+          if (tid == 0)
+            pt_sweep(Ap, b, blockIdx.x * (uint32_t)cpb, cpb, st_all, SW);
+        }
+        __syncthreads();
+      }
+    }
+    if (lead && it < H.n_iter) chain_prepare_iter(Ap, it, chain, st, U);
     __syncthreads();
   }
   if (lead) chain_finish(Ap, chain, st);

@@ -316,6 +316,13 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
         cpb = std::min(std::max(1u, 16u / wpc), std::max(1u, n_chains));
     }
   }
+  // A temperature ladder lives inside one workgroup.
+  if (ext && ext->swap_every > 0 && ext->swap_group > 1) {
+    const uint32_t G = ext->swap_group;
+    while (wpc > 1 && G * wpc > 16) wpc /= 2;
+    cpb = G * std::max(1u, (16u / wpc) / G);
+    while (cpb > G && cpb > n_chains) cpb -= G;
+  }
   if (cpb * wpc > 16 || cpb == 0 || wpc == 0) {
     set_err("invalid tiling: chains_per_block * waves_per_chain must be <= 16");
     return false;
@@ -323,10 +330,21 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   t->cpb = cpb;
   t->wpc = wpc;
   const uint32_t ncol = std::max(1u, m->n_cols_total);
-  const size_t state_bytes = (size_t)cpb * (3 * m->n_params + 3) * 8;
-  const size_t part_bytes = (size_t)cpb * wpc * 8;
-  const size_t u_bytes = (size_t)cpb * m->n_uniform_pad * 4;
-  size_t head = (state_bytes + part_bytes + u_bytes + 15) / 16 * 16;
+  auto head_bytes = [&](uint32_t c, uint32_t w) {
+    const size_t state_bytes = (size_t)c * (6 * m->n_params + 6) * 8;
+    const size_t part_bytes = (size_t)c * w * 8;
+    const size_t u_bytes = (size_t)c * m->n_uniform_pad * 4;
+    return (state_bytes + part_bytes + u_bytes + 15) / 16 * 16;
+  };
+  // Per-chain records and uniform tables must leave room for the two
+  // streamed tile buffers.
+  const bool pinned = ext && ext->chains_per_block > 0;
+  while (!pinned && cpb > 1 &&
+         head_bytes(cpb, wpc) + kStreamStaticLds > g_dev.lds_max) {
+    cpb /= 2;
+  }
+  t->cpb = cpb;
+  size_t head = head_bytes(cpb, wpc);
   t->lds_tile_offset = (uint32_t)head;
   const uint32_t step = t->chunk * wpc;  // one chunk per wave of a chain
   const size_t tile_floats = (size_t)kTileF4 * 4;  // one static buffer
@@ -492,6 +510,26 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
     set_err("null result");
     return 1;
   }
+  // Fields past the caller's struct_size read as zero (ABI growth).
+  gpum_run_ext X;
+  memset(&X, 0, sizeof X);
+  gpum_run_ext* caller_ext = ext;
+  if (ext) {
+    memcpy(&X, ext, std::min<size_t>(ext->struct_size, sizeof X));
+    ext = &X;
+  }
+  uint32_t n_batches = 0;
+  if (ext && ext->swap_every > 0) {
+    n_batches = std::max(1u, n_iter / ext->swap_every);
+    if (ext->swap_group > 1) {
+      if (ext->swap_group > 16 || n_chains % ext->swap_group != 0) {
+        set_err("swap_group must divide n_chains and be at most 16 (got %u "
+                "for %u chains)", ext->swap_group, n_chains);
+        return 1;
+      }
+    }
+    if (ext->adapt_batches > n_batches) ext->adapt_batches = n_batches;
+  }
   if (!validate_run(m, init, n_chains, proposal_sd, temperatures,
                     proposal_mode, proposal_l))
     return 1;
@@ -504,8 +542,9 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   const size_t ncp = (size_t)n_chains * np;
   const size_t n_draws = (size_t)n_iter * ncp;
   DevBuf<double> d_init, d_sd, d_temp, d_l, d_lp, d_final, d_tplp, d_tprop, d_tinit;
-  DevBuf<float> d_draws, d_acc;
-  DevBuf<uint8_t> d_tacc;
+  DevBuf<double> d_iscale, d_fsd, d_fscale;
+  DevBuf<float> d_draws, d_acc, d_acch;
+  DevBuf<uint8_t> d_tacc, d_swaph;
   DevBuf<ChainArgs> d_args;
   auto fail = [&]() { return 1; };
   if (!d_init.upload(init, ncp, "upload init")) return fail();
@@ -527,6 +566,23 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   if (want_tprop && !d_tprop.alloc(n_draws, "allocate trace")) return fail();
   if (want_tacc && !d_tacc.alloc((size_t)n_iter * n_chains, "allocate trace")) return fail();
   if (want_tinit && !d_tinit.alloc(n_chains, "allocate trace")) return fail();
+  const bool want_swaph = ext && ext->swap_hist && n_batches > 0;
+  const bool want_acch = ext && ext->acc_hist && n_batches > 0;
+  const bool want_fsd = ext && ext->final_sd;
+  const bool want_fscale = ext && ext->final_scale;
+  if (ext && ext->init_scale &&
+      !d_iscale.upload(ext->init_scale, n_chains, "upload init_scale"))
+    return fail();
+  if (want_swaph && !d_swaph.alloc((size_t)n_batches * n_chains, "allocate swap history"))
+    return fail();
+  if (want_swaph && hipMemset(d_swaph.p, 0xFF, (size_t)n_batches * n_chains) != hipSuccess) {
+    set_err("clear swap history: device error");
+    return fail();
+  }
+  if (want_acch && !d_acch.alloc((size_t)n_batches * n_chains, "allocate accept history"))
+    return fail();
+  if (want_fsd && !d_fsd.alloc(ncp, "allocate final_sd")) return fail();
+  if (want_fscale && !d_fscale.alloc(n_chains, "allocate final_scale")) return fail();
 
   ChainArgs a;
   memset(&a, 0, sizeof a);
@@ -565,6 +621,17 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   a.trace_prop = want_tprop ? d_tprop.p : nullptr;
   a.trace_accept = want_tacc ? d_tacc.p : nullptr;
   a.init_log_post = want_tinit ? d_tinit.p : nullptr;
+  a.iter_offset = ext ? ext->iter_offset : 0;
+  a.swap_every = ext ? ext->swap_every : 0;
+  a.n_batches = n_batches;
+  a.swap_group = ext ? ext->swap_group : 0;
+  a.adapt_batches = ext ? ext->adapt_batches : 0;
+  a.batch0 = ext ? ext->batch0 : 0;
+  a.init_scale = (ext && ext->init_scale) ? d_iscale.p : nullptr;
+  a.swap_hist = want_swaph ? d_swaph.p : nullptr;
+  a.acc_hist = want_acch ? d_acch.p : nullptr;
+  a.final_sd = want_fsd ? d_fsd.p : nullptr;
+  a.final_scale = want_fscale ? d_fscale.p : nullptr;
   a.cpb = t.cpb;
   a.wpc = t.wpc;
   a.tile_len = t.tile_len;
@@ -590,7 +657,10 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   h.tile_len = t.tile_len;
   h.stream_mode = t.stream_mode;
   h.lds_tile_offset = t.lds_tile_offset;
-  h.state_words = 3 * np + 3;
+  h.state_words = 6 * np + 6;
+  h.swap_every = a.swap_every;
+  h.n_batches = a.n_batches;
+  h.swap_group = a.swap_group;
 
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -616,8 +686,9 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   hipEventElapsedTime(&ms, e0, e1);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  if (ext && ext->struct_size >= offsetof(gpum_run_ext, kernel_ms) + sizeof(double))
-    ext->kernel_ms = ms;
+  if (caller_ext &&
+      caller_ext->struct_size >= offsetof(gpum_run_ext, kernel_ms) + sizeof(double))
+    caller_ext->kernel_ms = ms;
 
   if (loglik_out) {
     e = hipMemcpy(loglik_out, d_lp.p, n_chains * sizeof(double), hipMemcpyDeviceToHost);
@@ -655,6 +726,18 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
                    hipMemcpyDeviceToHost) == hipSuccess;
   if (ok && want_tinit)
     ok = hipMemcpy(ext->init_log_post, d_tinit.p, n_chains * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_swaph)
+    ok = hipMemcpy(ext->swap_hist, d_swaph.p, (size_t)n_batches * n_chains,
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_acch)
+    ok = hipMemcpy(ext->acc_hist, d_acch.p, (size_t)n_batches * n_chains * sizeof(float),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_fsd)
+    ok = hipMemcpy(ext->final_sd, d_fsd.p, ncp * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_fscale)
+    ok = hipMemcpy(ext->final_scale, d_fscale.p, n_chains * sizeof(double),
                    hipMemcpyDeviceToHost) == hipSuccess;
   if (ok && want_tacc)
     ok = hipMemcpy(ext->trace_accept, d_tacc.p, (size_t)n_iter * n_chains,

@@ -51,9 +51,12 @@ def normals(seed: int, n: int, stream: int = 0) -> np.ndarray:
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
 
 
-def uniforms(seed: int, batch: int, n: int, stream: int = 0) -> np.ndarray:
-    """n uniforms for swap step `batch`: word 0 of block (batch, k)."""
+def uniforms(seed: int, batch: int, n: int, stream: int = 0,
+             group: int = 0) -> np.ndarray:
+    """n uniforms for swap step `batch` of ladder `group`: word 0 of Philox
+    block (batch, k, group, TAG_SWAP), k = position in the sweep."""
     k = np.arange(n, dtype=np.uint32)
-    ctr = np.stack([np.full_like(k, batch & 0xFFFFFFFF), k, np.zeros_like(k),
+    ctr = np.stack([np.full_like(k, batch & 0xFFFFFFFF), k,
+                    np.full_like(k, group & 0xFFFFFFFF),
                     np.full_like(k, TAG_SWAP)], axis=1)
     return to_unit(philox(ctr, seed, stream)[:, 0])

@@ -81,6 +81,25 @@ typedef struct gpum_run_ext {
   int32_t fuse;
   /* Out: device time of the chain kernel in milliseconds. */
   double kernel_ms;
+  /* Batches inside one launch -- the parallel-tempering orchestration of
+   * R/adaptive.R:272-413 fused into the kernel.  swap_every = 0: none.
+   * The run is cut into max(1, n_iter / swap_every) batches (the last
+   * absorbs the remainder); after each batch, chains adapt (during the first
+   * adapt_batches batches: Robbins-Monro on the batch acceptance and
+   * sd = scale * sqrt(diag cov), R/adaptive.R:342-356) and each ladder of
+   * swap_group consecutive chains attempts the alternating adjacent swap
+   * sweep (R/adaptive.R:362-391).  swap_group must divide n_chains and be at
+   * most 16; swap_group <= 1 adapts without swapping. */
+  uint32_t iter_offset;   /* Philox iteration counter of the first iteration */
+  uint32_t swap_every;
+  uint32_t swap_group;
+  uint32_t adapt_batches;
+  uint32_t batch0;        /* global index offset of the first batch */
+  const double* init_scale;  /* in: n_chains starting RM scales, NULL = 1 */
+  uint8_t* swap_hist;     /* out: n_batches * n_chains, 1/0 per attempted pair */
+  float* acc_hist;        /* out: n_batches * n_chains batch acceptance */
+  double* final_sd;       /* out: n_chains * n_params adapted proposal sd */
+  double* final_scale;    /* out: n_chains RM scales */
 } gpum_run_ext;
 
 /* Static description of a lowered model. */

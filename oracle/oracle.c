@@ -324,7 +324,8 @@ static double uniform_at(const orc_run_args* a, uint32_t chain, uint32_t t,
     uint32_t seedmix = orc_seedmix(a->seed, chain);
     return orc_uniform_t32(seedmix, t * spi + slot + 1u);
   }
-  return orc_uniform_philox(a->seed, a->stream_id, 0u, t, chain, slot);
+  return orc_uniform_philox(a->seed, a->stream_id, 0u, a->iter_offset + t,
+                            chain, slot);
 }
 
 static double log_post(const orc_run_args* a, const double* params) {

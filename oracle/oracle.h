@@ -43,6 +43,7 @@ typedef struct {
   uint32_t stream_id;       /* Philox key word 1 (model index) */
   uint32_t n_threads;
   orc_loglik_fn loglik_fn;  /* compiled data term, or NULL to interpret */
+  uint32_t iter_offset;     /* Philox iteration counter of iteration 0 */
 } orc_run_args;
 
 typedef struct {

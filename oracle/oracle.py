@@ -43,6 +43,7 @@ class RunArgs(C.Structure):
         ("de_noise", C.c_double), ("proposal_l", C.POINTER(C.c_double)),
         ("stream", C.c_uint32), ("stream_id", C.c_uint32),
         ("n_threads", C.c_uint32), ("loglik_fn", C.c_void_p),
+        ("iter_offset", C.c_uint32),
     ]
 
 
@@ -199,7 +200,7 @@ def run(code, consts, prior_code, prior_consts, n_params, data, n_cols, n_obs,
         init, proposal_sd, temperatures, n_iter, seed, proposal_mode=0,
         gamma=0.0, de_noise=0.0, proposal_l=None, stream=STREAM_PHILOX,
         stream_id=0, n_threads=None, trace=False, loglik_fn=None,
-        de_sync=False) -> OracleResult:
+        de_sync=False, iter_offset=0) -> OracleResult:
     """cpu_native::run (cpu_native.rs:510-825) or run_de_sync (:400-506)."""
     L = lib()
     keep = dict(code=_u32(code), consts=_f64(consts), pc=_u32(prior_code),
@@ -232,6 +233,7 @@ def run(code, consts, prior_code, prior_consts, n_params, data, n_cols, n_obs,
     a.stream_id = stream_id
     a.n_threads = n_threads or os.cpu_count() or 1
     a.loglik_fn = int(loglik_fn) if loglik_fn else None
+    a.iter_offset = iter_offset
     draws = np.zeros(max(n_iter * n_chains * n_params, 1), np.float32)
     acc = np.zeros(n_chains, np.float32)
     lp = np.zeros(n_chains, np.float64)

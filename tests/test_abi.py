@@ -36,6 +36,7 @@ def test_library_exports_all_declared_symbols():
 def test_struct_layouts_match_header():
     # field order and sizes of the ctypes mirrors against the header text
     text = open(os.path.join(ROOT, "include", "gpum_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     body = text[text.index("typedef struct gpum_run_ext"):]
     body = body[:body.index("} gpum_run_ext;")]
     fields = re.findall(r"^\s+[\w\s\*]+?\b(\w+);", body, re.M)
