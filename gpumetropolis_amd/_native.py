@@ -46,7 +46,27 @@ class RunExt(C.Structure):
                 ("swap_hist", C.POINTER(C.c_uint8)),
                 ("acc_hist", C.POINTER(C.c_float)),
                 ("final_sd", C.POINTER(C.c_double)),
-                ("final_scale", C.POINTER(C.c_double))]
+                ("final_scale", C.POINTER(C.c_double)),
+                ("trace_ll", C.POINTER(C.c_double))]
+
+
+class DaExt(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("stream_id", C.c_uint32),
+                ("iter_offset", C.c_uint32), ("n_slices", C.c_uint32),
+                ("final_state", C.POINTER(C.c_double)),
+                ("stage1_rate", C.POINTER(C.c_double)),
+                ("trace_s", C.POINTER(C.c_double)),
+                ("trace_plp", C.POINTER(C.c_double)),
+                ("trace_stage", C.POINTER(C.c_uint8)),
+                ("kernel_ms", C.c_double), ("exact_evals", C.c_double)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("steps", C.c_uint32),
+                ("batch", C.c_uint32), ("seed", C.c_uint32),
+                ("lr", C.c_double), ("lr_final", C.c_double),
+                ("beta1", C.c_double), ("beta2", C.c_double),
+                ("eps", C.c_double)]
 
 
 class ModelInfo(C.Structure):
@@ -105,6 +125,35 @@ def lib():
     L.gpum_hip_model_loglik.restype = C.c_int
     L.gpum_hip_model_free.argtypes = [C.c_void_p]
     L.gpum_hip_model_free.restype = None
+    _f32p = C.POINTER(C.c_float)
+    L.gpum_hip_surrogate_n_weights.restype = C.c_uint32
+    L.gpum_hip_surrogate_create.argtypes = [C.c_uint32, C.c_uint32]
+    L.gpum_hip_surrogate_create.restype = C.c_void_p
+    L.gpum_hip_surrogate_free.argtypes = [C.c_void_p]
+    L.gpum_hip_surrogate_free.restype = None
+    L.gpum_hip_surrogate_set_scaling.argtypes = [C.c_void_p, _f64p, _f64p,
+                                                 C.c_double, C.c_double]
+    L.gpum_hip_surrogate_set_scaling.restype = C.c_int
+    L.gpum_hip_surrogate_get_weights.argtypes = [C.c_void_p, _f32p]
+    L.gpum_hip_surrogate_get_weights.restype = C.c_int
+    L.gpum_hip_surrogate_set_weights.argtypes = [C.c_void_p, _f32p]
+    L.gpum_hip_surrogate_set_weights.restype = C.c_int
+    L.gpum_hip_surrogate_gradient.argtypes = [C.c_void_p, _f64p, _f64p,
+                                              C.c_uint32, _f32p, _f64p]
+    L.gpum_hip_surrogate_gradient.restype = C.c_int
+    L.gpum_hip_surrogate_train.argtypes = [C.c_void_p, _f64p, _f64p,
+                                           C.c_uint32, C.POINTER(TrainOpts),
+                                           _f64p, _f64p]
+    L.gpum_hip_surrogate_train.restype = C.c_int
+    L.gpum_hip_surrogate_predict.argtypes = [C.c_void_p, _f64p, C.c_uint32,
+                                             _f64p]
+    L.gpum_hip_surrogate_predict.restype = C.c_int
+    L.gpum_hip_model_run_da.argtypes = [C.c_void_p, C.c_void_p, _f64p,
+                                        C.c_uint32, _f64p, _f64p, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, _f64p,
+                                        C.POINTER(DaExt),
+                                        C.POINTER(ChainResult)]
+    L.gpum_hip_model_run_da.restype = C.c_int
     _lib = L
     return L
 
@@ -152,6 +201,12 @@ class RunOutput:
     acc_hist: Optional[np.ndarray] = None     # (n_batches, n_chains)
     final_sd: Optional[np.ndarray] = None
     final_scale: Optional[np.ndarray] = None
+    trace_ll: Optional[np.ndarray] = None     # (n_iter, n_chains) data term
+    # stepwise / delayed-acceptance runs
+    stage1_rate: Optional[np.ndarray] = None  # (n_chains,)
+    exact_evals: float = 0.0
+    trace_s: Optional[np.ndarray] = None      # (n_iter, n_chains)
+    trace_stage: Optional[np.ndarray] = None  # (n_iter, n_chains) uint8
 
 
 def _collect(res: ChainResult) -> RunOutput:
@@ -195,6 +250,8 @@ def _ext(n_iter, n_chains, n_params, trace, final_state, stream_id, tiling,
         keep["plp"] = np.zeros(n_iter * n_chains, np.float64)
         keep["prop"] = np.zeros(n_iter * n_chains * n_params, np.float64)
         keep["acc"] = np.zeros(n_iter * n_chains, np.uint8)
+        keep["ll"] = np.zeros(n_iter * n_chains, np.float64)
+        ext.trace_ll = _p(keep["ll"], C.c_double)
         ext.trace_plp = _p(keep["plp"], C.c_double)
         ext.trace_prop = _p(keep["prop"], C.c_double)
         ext.trace_accept = _p(keep["acc"], C.c_uint8)
@@ -216,6 +273,7 @@ def _finish(out: RunOutput, ext, keep, n_iter, n_chains, n_params):
         out.trace_plp = keep["plp"].reshape(n_iter, n_chains)
         out.trace_prop = keep["prop"].reshape(n_iter, n_chains, n_params)
         out.trace_accept = keep["acc"].reshape(n_iter, n_chains).astype(bool)
+        out.trace_ll = keep["ll"].reshape(n_iter, n_chains)
         out.init_log_post = keep["init"]
     if "final" in keep:
         out.final_state = keep["final"].reshape(n_chains, n_params)
@@ -318,6 +376,55 @@ class DeviceModel:
         _check(rc)
         return _finish(_collect(res), ext, keep, n_iter, n_chains, np_)
 
+    def run_da(self, surrogate, init, temperatures, n_iter, seed,
+               proposal_sd=None, proposal_l=None, trace=False,
+               final_state=False, stream_id=0, iter_offset=0,
+               n_slices=0) -> RunOutput:
+        """Stepwise random-walk Metropolis; with `surrogate` (a Surrogate)
+        two-stage delayed acceptance, without it plain Metropolis with the
+        uniforms of `run`.  Mode 2 when proposal_l is given, else mode 0."""
+        np_ = self.n_params
+        init, temps = _f64(init), _f64(temperatures)
+        n_chains = init.size // np_
+        mode = 2 if proposal_l is not None else 0
+        sd = None if proposal_sd is None else _f64(proposal_sd)
+        pl = None if proposal_l is None else _f64(proposal_l)
+        ext = DaExt()
+        ext.struct_size = C.sizeof(DaExt)
+        ext.stream_id = stream_id
+        ext.iter_offset = iter_offset
+        ext.n_slices = n_slices
+        keep = {"rate": np.zeros(n_chains, np.float64)}
+        ext.stage1_rate = _p(keep["rate"], C.c_double)
+        if final_state:
+            keep["final"] = np.zeros(n_chains * np_, np.float64)
+            ext.final_state = _p(keep["final"], C.c_double)
+        if trace:
+            keep["s"] = np.zeros(n_iter * n_chains, np.float64)
+            keep["plp"] = np.zeros(n_iter * n_chains, np.float64)
+            keep["stage"] = np.zeros(n_iter * n_chains, np.uint8)
+            ext.trace_s = _p(keep["s"], C.c_double)
+            ext.trace_plp = _p(keep["plp"], C.c_double)
+            ext.trace_stage = _p(keep["stage"], C.c_uint8)
+        res = ChainResult()
+        sur = surrogate._h if surrogate is not None else None
+        rc = self._L.gpum_hip_model_run_da(
+            self._h, sur, _p(init, C.c_double), n_chains, _p(sd, C.c_double),
+            _p(temps, C.c_double), n_iter, seed & 0xFFFFFFFF, mode,
+            _p(pl, C.c_double), C.byref(ext), C.byref(res))
+        _check(rc)
+        out = _collect(res)
+        out.kernel_ms = ext.kernel_ms
+        out.exact_evals = ext.exact_evals
+        out.stage1_rate = keep["rate"]
+        if final_state:
+            out.final_state = keep["final"].reshape(n_chains, np_)
+        if trace:
+            out.trace_s = keep["s"].reshape(n_iter, n_chains)
+            out.trace_plp = keep["plp"].reshape(n_iter, n_chains)
+            out.trace_stage = keep["stage"].reshape(n_iter, n_chains)
+        return out
+
     def loglik(self, points) -> np.ndarray:
         points = _f64(points)
         n = points.size // self.n_params
@@ -329,6 +436,80 @@ class DeviceModel:
     def close(self):
         if getattr(self, "_h", None):
             self._L.gpum_hip_model_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Surrogate:
+    """The device-resident surrogate network (include/gpum_hip.h)."""
+
+    def __init__(self, n_in: int, seed: int = 0):
+        L = lib()
+        self._L = L
+        self.n_in = int(n_in)
+        h = L.gpum_hip_surrogate_create(self.n_in, seed & 0xFFFFFFFF)
+        if not h:
+            raise BackendError(last_error())
+        self._h = h
+        self.n_weights = int(L.gpum_hip_surrogate_n_weights())
+
+    def set_scaling(self, mean, whiten, target_mean, target_sd):
+        mean, whiten = _f64(mean), _f64(whiten)
+        _check(self._L.gpum_hip_surrogate_set_scaling(
+            self._h, _p(mean, C.c_double), _p(whiten, C.c_double),
+            float(target_mean), float(target_sd)))
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(self.n_weights, np.float32)
+        _check(self._L.gpum_hip_surrogate_get_weights(self._h, _p(w, C.c_float)))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(np.asarray(w, np.float32).ravel())
+        if w.size != self.n_weights:
+            raise ValueError(f"expected {self.n_weights} weights, got {w.size}")
+        _check(self._L.gpum_hip_surrogate_set_weights(self._h, _p(w, C.c_float)))
+
+    def gradient(self, theta, loglik):
+        theta, loglik = _f64(theta), _f64(loglik)
+        n = loglik.size
+        g = np.zeros(self.n_weights, np.float32)
+        loss = C.c_double(0.0)
+        _check(self._L.gpum_hip_surrogate_gradient(
+            self._h, _p(theta, C.c_double), _p(loglik, C.c_double), n,
+            _p(g, C.c_float), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, theta, loglik, steps=2000, batch=1024, seed=0, lr=2e-3,
+              lr_final=0.0):
+        theta, loglik = _f64(theta), _f64(loglik)
+        o = TrainOpts()
+        o.struct_size = C.sizeof(TrainOpts)
+        o.steps, o.batch, o.seed = steps, batch, seed & 0xFFFFFFFF
+        o.lr, o.lr_final = lr, lr_final
+        trace = np.zeros(max(steps, 1), np.float64)
+        ms = C.c_double(0.0)
+        _check(self._L.gpum_hip_surrogate_train(
+            self._h, _p(theta, C.c_double), _p(loglik, C.c_double),
+            loglik.size, C.byref(o), _p(trace, C.c_double), C.byref(ms)))
+        return trace[:steps], ms.value
+
+    def predict(self, theta) -> np.ndarray:
+        theta = _f64(theta)
+        n = theta.size // self.n_in
+        out = np.zeros(max(n, 1), np.float64)
+        _check(self._L.gpum_hip_surrogate_predict(
+            self._h, _p(theta, C.c_double), n, _p(out, C.c_double)))
+        return out[:n]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.gpum_hip_surrogate_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -8,6 +8,12 @@ at a time exactly where the reference calls ``rust_gpu_metropolis``:
                              with the frozen per-chain Cholesky factor
                              (R/api.R:529-572, R/adaptive.R:101-254)
   method="rwm", adapt=False  one run, warmup rows trimmed (R/api.R:574-598)
+  method="rwm", surrogate=..  the same warmup, recording every proposal and
+                             its exact log-likelihood; a surrogate network
+                             trained on those pairs then screens proposals
+                             for two-stage delayed acceptance (no reference
+                             counterpart), or plain sampling if its held-out
+                             error exceeds the bound
   method="pt"                .pt_orchestrate (R/adaptive.R:272-413)
   method="de"                .de_orchestrate path A (R/adaptive.R:427-524)
 
@@ -85,7 +91,7 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                    temperatures=None, swap_every=None, gamma=None,
                    de_noise: float = 1e-3, de_every=None, de_sync=False,
                    seed: int = 1, backend: str = "hip",
-                   tiling=None, ladder_size=None) -> Fit:
+                   tiling=None, ladder_size=None, surrogate=None) -> Fit:
     if backend not in ("hip", "auto"):
         raise ValueError(f"backend '{backend}' is not available here; this "
                          "package is the hip backend")
@@ -136,7 +142,12 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                       warmup, de_every, adapt, seed, n_chains)
         elif adapt and warmup > 0:
             fit = _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup,
-                                seed, warmup_auto, n_chains)
+                                seed, warmup_auto, n_chains,
+                                surrogate=surrogate, dm=dm)
+        elif surrogate:
+            raise ValueError("`surrogate` needs the adaptive warmup "
+                             "(adapt = TRUE and warmup > 0) for its training "
+                             "pairs.")
         else:
             r = call(init_mat, sd_mat, n_iter, seed)
             d = r.draws[warmup:].astype(np.float64)
@@ -157,13 +168,13 @@ class _Caller:
         self.kernel_ms = 0.0
 
     def __call__(self, init_m, sd_m, iters, s, temps=None, mode=0, g=0.0,
-                 noise=0.0, L=None, iter_offset=0):
+                 noise=0.0, L=None, iter_offset=0, trace=False):
         t = np.ones(self.n_chains) if temps is None else temps
         batches = {"iter_offset": iter_offset} if iter_offset else None
         r = self.dm.run(init_m, sd_m, t, iters, int(s) & 0xFFFFFFFF,
                         proposal_mode=mode, gamma=g, de_noise=noise,
                         proposal_l=L, final_state=True, tiling=self.tiling,
-                        batches=batches)
+                        batches=batches, trace=trace)
         self.kernel_ms += r.kernel_ms
         return r
 
@@ -178,13 +189,17 @@ class WarmupResult:
     batches_run: int
     batch_sizes: list
     accept_history: np.ndarray
+    pairs_theta: Optional[np.ndarray] = None   # (n, d) proposals
+    pairs_loglik: Optional[np.ndarray] = None  # (n,) their data terms
 
 
 def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
-                    auto_stop=False) -> WarmupResult:
+                    auto_stop=False, collect=False) -> WarmupResult:
     """.am_orchestrate_warmup (R/adaptive.R:101-254): per-chain and pooled
     Welford, Robbins-Monro scale per chain, mid-warmup reset, per-chain
-    L = scale * chol(pooled covariance), optional early freeze."""
+    L = scale * chol(pooled covariance), optional early freeze.  With
+    `collect`, the batches after the mid-warmup reset also return every
+    proposal with its exact log-likelihood (surrogate training pairs)."""
     np_ = n_params
     nb = len(sizes)
     states = ad.WelfordState.init(n_chains, np_, sd_mat ** 2)
@@ -202,10 +217,16 @@ def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
     streak = 0
     batches_run = 0
     used = 0
+    pth, pll = [], []
     for b in range(1, nb + 1):
         use_L = np_ > 1 and cur_L is not None
+        rec = collect and b > reset_batch
         r = call(cur_init, cur_sd, sizes[b - 1], seed + b - 1,
-                 mode=2 if use_L else 0, L=cur_L if use_L else None)
+                 mode=2 if use_L else 0, L=cur_L if use_L else None,
+                 trace=rec)
+        if rec:
+            pth.append(r.trace_prop.reshape(-1, np_))
+            pll.append(r.trace_ll.reshape(-1))
         X = r.draws.astype(np.float64)  # the f32 draws, as R receives them
         states.update_batch(X)
         pool.update_batch(X)
@@ -240,27 +261,51 @@ def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
         prev_scales = scales.copy()
     return WarmupResult(cur_init, cur_sd, cur_L if np_ > 1 else None, scales,
                         used, batches_run, sizes[:batches_run],
-                        acc_hist[:, :batches_run])
+                        acc_hist[:, :batches_run],
+                        np.concatenate(pth) if pth else None,
+                        np.concatenate(pll) if pll else None)
 
 
 def _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup, seed,
-                  auto_stop, n_chains) -> Fit:
-    """R/adaptive.R:101-254 then R/api.R:537-571."""
+                  auto_stop, n_chains, surrogate=None, dm=None) -> Fit:
+    """R/adaptive.R:101-254 then R/api.R:537-571; with `surrogate` the
+    sampling batch is two-stage delayed acceptance."""
     np_ = model.n_params
     w = adaptive_warmup(call, np_, n_chains, init_mat, sd_mat,
-                        ad.warmup_batches(warmup), seed, auto_stop)
+                        ad.warmup_batches(warmup), seed, auto_stop,
+                        collect=bool(surrogate))
     sampling = max(1, n_iter - w.used)
     use_L = w.L is not None
-    r = call(w.init, w.sd, sampling, seed + w.batches_run,
-             mode=2 if use_L else 0, L=w.L if use_L else None)
+    adaptation = {"final_proposal_sd": w.sd, "final_scales": w.scales,
+                  "n_batches": w.batches_run, "batch_sizes": w.batch_sizes,
+                  "accept_history": w.accept_history, "final_L": w.L}
+    net = None
+    if surrogate:
+        from .surrogate import fit_surrogate
+        opts = dict(surrogate) if isinstance(surrogate, dict) else {}
+        sf = fit_surrogate(w.pairs_theta, w.pairs_loglik, seed=seed, **opts)
+        info = sf.summary()
+        info["used"] = sf.usable
+        if sf.usable:
+            net = sf.net
+        adaptation["surrogate"] = info
+    if net is not None:
+        r = dm.run_da(net, w.init, np.ones(n_chains), sampling,
+                      int(seed + w.batches_run) & 0xFFFFFFFF,
+                      proposal_sd=None if use_L else w.sd,
+                      proposal_l=w.L if use_L else None, final_state=True)
+        call.kernel_ms += r.kernel_ms
+        adaptation["surrogate"].update(
+            stage1_rate=float(np.mean(r.stage1_rate)),
+            exact_evals=float(r.exact_evals))
+        net.close()
+    else:
+        r = call(w.init, w.sd, sampling, seed + w.batches_run,
+                 mode=2 if use_L else 0, L=w.L if use_L else None)
     d = r.draws.astype(np.float64)
     return Fit(d, r.accept_rate.astype(np.float64), "rwm", d.shape[0],
                w.used + d.shape[0], w.used, n_chains, np_, model.params, seed,
-               adaptation={"final_proposal_sd": w.sd, "final_scales": w.scales,
-                           "n_batches": w.batches_run,
-                           "batch_sizes": w.batch_sizes,
-                           "accept_history": w.accept_history,
-                           "final_L": w.L})
+               adaptation=adaptation)
 
 
 def _pt(call, model, init_mat, sd_mat, temperatures, n_iter, warmup,

@@ -25,56 +25,9 @@
 #include "device_vm.h"
 #include "math64.h"
 #include "philox.h"
+#include "step_common.h"
 
 namespace gpum {
-
-// FP64 stack evaluation of a reference program over the parameters (no
-// data), with the hoist extensions writing the uniform table.
-__device__ __noinline__ double eval_params64(const uint32_t* __restrict__ code,
-                                uint32_t n_words,
-                                const double* __restrict__ consts,
-                                const double* params, float* U) {
-  if (n_words < 2) return 0.0;
-  double st[kMaxStack + 1];
-  int sp = 0;
-  for (uint32_t pc = 0; pc < n_words / 2; ++pc) {
-    uint32_t op = code[2 * pc], arg = code[2 * pc + 1];
-    switch (op) {
-      case OP_PUSH_CONST: st[sp++] = consts[arg]; break;
-      case OP_PUSH_PARAM: st[sp++] = params[arg]; break;
-      case OP_ADD: st[sp - 2] += st[sp - 1]; --sp; break;
-      case OP_SUB: st[sp - 2] -= st[sp - 1]; --sp; break;
-      case OP_MUL: st[sp - 2] *= st[sp - 1]; --sp; break;
-      case OP_DIV: st[sp - 2] /= st[sp - 1]; --sp; break;
-      case OP_NEG: st[sp - 1] = -st[sp - 1]; break;
-      case OP_EXP: st[sp - 1] = exp(st[sp - 1]); break;
-      case OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
-      case OP_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
-      case OP_LGAMMA: st[sp - 1] = lgamma64(st[sp - 1]); break;
-      case OP_POW: st[sp - 2] = pow(st[sp - 2], st[sp - 1]); --sp; break;
-      case OP_STORE_U: U[arg] = (float)st[--sp]; break;
-      case OP_STORE_U_RCP: U[arg] = (float)(1.0 / st[--sp]); break;
-      case OP_STORE_U_NEG: U[arg] = (float)(-st[--sp]); break;
-      default: break;
-    }
-  }
-  return sp > 0 ? st[0] : 0.0;
-}
-
-// The uniforms of one chain-iteration, materialised four at a time.
-struct StepUniforms {
-  uint32_t seed, stream, t, chain;
-  U4 cache;
-  int cached = -1;
-  __device__ double get(uint32_t slot) {
-    int blk = (int)(slot >> 2);
-    if (blk != cached) {
-      cache = philox4x32_10(U4{t, chain, (uint32_t)blk, TAG_STEP}, seed, stream);
-      cached = blk;
-    }
-    return u32_to_unit(u4_word(cache, slot & 3u));
-  }
-};
 
 // Per-chain record in LDS (doubles):
 //   state[np] prop[np] z[np] sd[np] wmean[np] wm2[np]
@@ -142,6 +95,7 @@ __device__ __noinline__ void chain_finish_iter(const ChainArgs* __restrict__ A,
   double u = us.get(spi - 1);
   bool acc_ok = log(u) < (plp - r.cur()) / A->temperatures[chain];
   if (A->trace_plp) A->trace_plp[(size_t)t * C + chain] = plp;
+  if (A->trace_ll) A->trace_ll[(size_t)t * C + chain] = ll;
   if (A->trace_prop)
     for (int j = 0; j < np; ++j)
       A->trace_prop[((size_t)t * C + chain) * np + j] = r.prop[j];
@@ -168,12 +122,6 @@ __device__ __noinline__ void chain_finish_iter(const ChainArgs* __restrict__ A,
       }
     }
   }
-}
-
-// Target acceptance by dimension (R/adaptive.R:69-72).
-__device__ __forceinline__ double target_accept(int d) {
-  const double low[8] = {0.44, 0.35, 0.32, 0.30, 0.28, 0.27, 0.26, 0.25};
-  return (d >= 1 && d <= 8) ? low[d - 1] : 0.234;
 }
 
 // End of batch b for one chain: Robbins-Monro on the batch acceptance and
@@ -338,6 +286,33 @@ __device__ __noinline__ void chain_finish(const ChainArgs* __restrict__ A,
 __shared__ float4 g_tile_a[kTileF4];
 __shared__ float4 g_tile_b[kTileF4];
 
+// Asynchronous tile copy: global_load_lds moves 16 bytes per lane from
+// HBM straight into LDS (no VGPR staging); lane l of wave w writes float4
+// i0 + l where i0 = k*blockDim + 64*w is wave-uniform.  A tile holds a
+// multiple of 64 float4s per column, so a wave's 64 slots are all inside
+// the tile or all outside.  Streaming launches use 1024 threads, so a tile
+// takes at most kTileIssues copies per wave; the loop is unrolled so that
+// every copy in flight keeps its own address registers.
+__device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
+                                               uint32_t ld4, uint32_t n4,
+                                               uint32_t tile4,
+                                               uint32_t wave_base, int lane,
+                                               uint32_t ti, float4* dst) {
+#pragma unroll
+  for (uint32_t k = 0; k < kTileIssues; ++k) {
+    // Slots past the tile repeat its last 64 float4s (identical bytes to
+    // identical addresses), which keeps the sequence free of branches.
+    // n4 is a multiple of 64, so the column is uniform across the wave and
+    // the address is a scalar base plus the lane's 16-byte offset.
+    const uint32_t i0 = min(wave_base + k * 1024u, tile4 - 64u);
+    const uint32_t col = i0 / n4, off0 = i0 - col * n4;
+    const float4* base = src + ((size_t)col * ld4 + (size_t)ti * n4 + off0);
+    __builtin_amdgcn_global_load_lds(
+        (const void*)(base + lane),
+        (__attribute__((address_space(3))) void*)(dst + i0), 16, 0, 0);
+  }
+}
+
 // The chain's share of one tile: chunks part, part + wpc, ... of 64*OPT
 // observations; FP32 sum per chunk, FP64 across chunks.
 template <int NS, int OPT>
@@ -408,28 +383,8 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   const uint32_t wave_base = (uint32_t)(wave * 64);
   float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
 
-  // Asynchronous tile copy: global_load_lds moves 16 bytes per lane from
-  // HBM straight into LDS (no VGPR staging); lane l of wave w writes float4
-  // i0 + l where i0 = k*blockDim + 64*w is wave-uniform.  A tile holds a
-  // multiple of 64 float4s per column, so a wave's 64 slots are all inside
-  // the tile or all outside.
-  // Streaming launches use 1024 threads, so a tile takes at most
-  // kTileIssues copies per wave; the loop is unrolled so that every copy in
-  // flight keeps its own address registers.
   auto issue_tile = [&](uint32_t ti, float4* dst) {
-#pragma unroll
-    for (uint32_t k = 0; k < kTileIssues; ++k) {
-      // Slots past the tile repeat its last 64 float4s (identical bytes to
-      // identical addresses), which keeps the sequence free of branches.
-      // n4 is a multiple of 64, so the column is uniform across the wave and
-      // the address is a scalar base plus the lane's 16-byte offset.
-      const uint32_t i0 = min(wave_base + k * 1024u, tile4 - 64u);
-      const uint32_t col = i0 / n4, off0 = i0 - col * n4;
-      const float4* base = src + ((size_t)col * ld4 + (size_t)ti * n4 + off0);
-      __builtin_amdgcn_global_load_lds(
-          (const void*)(base + lane),
-          (__attribute__((address_space(3))) void*)(dst + i0), 16, 0, 0);
-    }
+    issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti, dst);
   };
 
   if constexpr (!STREAM) {
@@ -523,10 +478,98 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   if (lead) chain_finish(Ap, chain, st);
 }
 
+// Data term of a list of parameter points (no chain state): workgroup
+// (g, s) sums observation slice s for slots g*cpb .. g*cpb+cpb-1, one wave
+// per slot, and writes one FP64 partial per (slice, slot).  The caller adds
+// the slices in order, so every slot's sum is independent of how many
+// slots are active.  Slots at or past *count exit before any work.
+template <int NS, int OPT, bool STREAM>
+__global__ void __launch_bounds__(1024)
+points_kernel(PointsArgs P, ChainHot H) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t count = P.count ? *P.count : P.n_points;
+  const uint32_t cpb = H.cpb;
+  const uint32_t k0 = blockIdx.x * cpb;
+  if (k0 >= count) return;  // uniform over the workgroup
+  const int NU = (int)H.n_uniform_pad;
+  float* Uall = reinterpret_cast<float*>(smem);
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t k = k0 + (uint32_t)wave;
+  const bool active = k < count;
+  float* U = Uall + wave * NU;
+  for (int i = tid; i < (int)cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
+  __syncthreads();
+  if (active && lane == 0) {
+    const uint32_t pi = P.index ? P.index[k] : k;
+    eval_params64(P.hoist, P.n_hoist, P.consts,
+                  P.points + (size_t)pi * P.n_params, U);
+  }
+  const uint32_t tile_len = H.tile_len;
+  const uint32_t n4 = tile_len / 4;
+  const uint32_t tile4 = H.n_cols * n4;
+  const uint32_t ld4 = H.ld / 4;
+  const uint32_t n_tiles = H.n_pad / tile_len;
+  const uint32_t ti0 = min(blockIdx.y * P.tiles_per_slice, n_tiles);
+  const uint32_t ti1 = min(ti0 + P.tiles_per_slice, n_tiles);
+  const float4* src = reinterpret_cast<const float4*>(H.data);
+  const uint32_t wave_base = (uint32_t)(wave * 64);
+  double acc = 0.0;
+  if (H.n_vm > 0) {
+    if constexpr (!STREAM) {
+      float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
+      for (uint32_t i = tid; i < tile4; i += blockDim.x) {
+        uint32_t col = i / n4, off = i - col * n4;
+        res[i] = src[(size_t)col * ld4 + off];
+      }
+      __syncthreads();
+      if (active && ti0 < ti1)
+        acc = tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(res), 0, 0,
+                                 1, U, lane);
+    } else if (H.stream_mode == 1) {
+      if (ti0 < ti1)
+        issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti0, g_tile_a);
+      __syncthreads();
+      for (uint32_t ti = ti0; ti < ti1; ti += 2) {
+        const bool two = ti + 1 < ti1;
+        if (two) issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 1, g_tile_b);
+        if (active)
+          acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
+                                    ti * tile_len, 0, 1, U, lane);
+        __syncthreads();
+        if (ti + 2 < ti1)
+          issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 2, g_tile_a);
+        if (two && active)
+          acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_b),
+                                    (ti + 1) * tile_len, 0, 1, U, lane);
+        __syncthreads();
+      }
+    } else {
+      for (uint32_t ti = ti0; ti < ti1; ++ti) {
+        __syncthreads();
+        for (uint32_t i = tid; i < tile4; i += blockDim.x) {
+          uint32_t col = i / n4, off = i - col * n4;
+          g_tile_a[i] = src[(size_t)col * ld4 + (size_t)ti * n4 + off];
+        }
+        __syncthreads();
+        if (active)
+          acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
+                                    ti * tile_len, 0, 1, U, lane);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (active && lane == 0) P.partial[(size_t)blockIdx.y * P.n_points + k] = acc;
+}
+
 #define GPUM_INST(NS, OPT)                                                   \
   template __global__ void chain_kernel<NS, OPT, false>(const ChainArgs*,   \
                                                         ChainHot);          \
-  template __global__ void chain_kernel<NS, OPT, true>(const ChainArgs*, ChainHot);
+  template __global__ void chain_kernel<NS, OPT, true>(const ChainArgs*, ChainHot); \
+  template __global__ void points_kernel<NS, OPT, false>(PointsArgs, ChainHot); \
+  template __global__ void points_kernel<NS, OPT, true>(PointsArgs, ChainHot);
 GPUM_INST(0, 16)
 GPUM_INST(1, 16)
 GPUM_INST(3, 16)
@@ -549,6 +592,32 @@ hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
     else                                                                     \
       hipLaunchKernelGGL((chain_kernel<NS_, OPT_, false>), grid, block,      \
                          lds_bytes, s, dev_args, h);                         \
+    return hipGetLastError();                                                \
+  }
+  GPUM_CASE(0, 16)
+  GPUM_CASE(1, 16)
+  GPUM_CASE(3, 16)
+  GPUM_CASE(7, 8)
+  GPUM_CASE(15, 4)
+  GPUM_CASE(31, 4)
+#undef GPUM_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_points_kernel(const PointsArgs& p, const ChainHot& h,
+                                int ns, int opt, size_t lds_bytes,
+                                hipStream_t s) {
+  dim3 grid((p.n_points + h.cpb - 1) / h.cpb, p.n_slices);
+  dim3 block(h.cpb * 64);
+  const bool stream = h.stream_mode != 0;
+#define GPUM_CASE(NS_, OPT_)                                                 \
+  if (ns == NS_ && opt == OPT_) {                                            \
+    if (stream)                                                              \
+      hipLaunchKernelGGL((points_kernel<NS_, OPT_, true>), grid, block,      \
+                         lds_bytes, s, p, h);                                \
+    else                                                                     \
+      hipLaunchKernelGGL((points_kernel<NS_, OPT_, false>), grid, block,     \
+                         lds_bytes, s, p, h);                                \
     return hipGetLastError();                                                \
   }
   GPUM_CASE(0, 16)

@@ -1,0 +1,683 @@
+// da.cpp -- host side of the surrogate network and of the stepwise sampler
+// (plain or two-stage delayed-acceptance Metropolis).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/gpum_hip.h"
+#include "da.h"
+#include "mlp.h"
+#include "philox.h"
+#include "runtime.h"
+
+using namespace gpum;
+using namespace gpum::rt;
+
+struct gpum_hip_surrogate {
+  uint32_t n_in = 0;
+  DevBuf<float> P, G, Ma, Va;
+  DevBuf<uint16_t> W1b, W2b;
+  std::vector<double> mean, whiten;
+  double t_mean = 0.0, t_sd = 1.0;
+  DevBuf<double> d_mean, d_whiten;
+  uint32_t adam_t = 0;
+  // workspace for `cap` rows (a multiple of 64)
+  uint32_t cap = 0;
+  DevBuf<uint16_t> Xb, H1b, D2b, D1b;
+  DevBuf<float> A1, A2, H2, s, tb, part, cs_b1, loss;
+};
+
+namespace {
+
+bool sur_workspace(gpum_hip_surrogate* S, uint32_t rows) {
+  rows = round_up(std::max(rows, 64u), 64u);
+  if (rows <= S->cap) return true;
+  const size_t r = rows, h = kMlpHid;
+  if (!S->Xb.alloc(r * kMlpIn, "allocate surrogate workspace") ||
+      !S->H1b.alloc(r * h, "allocate surrogate workspace") ||
+      !S->D2b.alloc(r * h, "allocate surrogate workspace") ||
+      !S->D1b.alloc(r * h, "allocate surrogate workspace") ||
+      !S->A1.alloc(r * h, "allocate surrogate workspace") ||
+      !S->A2.alloc(r * h, "allocate surrogate workspace") ||
+      !S->H2.alloc(r * h, "allocate surrogate workspace") ||
+      !S->s.alloc(r, "allocate surrogate workspace") ||
+      !S->tb.alloc(r, "allocate surrogate workspace") ||
+      !S->part.alloc((r / 64) * kPartStride, "allocate surrogate workspace") ||
+      !S->cs_b1.alloc((r / 16) * h, "allocate surrogate workspace") ||
+      !S->loss.alloc(1, "allocate surrogate workspace"))
+    return false;
+  // Rows past a call's count are read as zero inputs.
+  HIP_TRY("clear surrogate workspace",
+          hipMemsetAsync(S->Xb.p, 0, r * kMlpIn * sizeof(uint16_t), g_dev.stream));
+  S->cap = rows;
+  return true;
+}
+
+// Forward pass over `rows` rows of S->Xb (a multiple of 16); with `train`
+// the pre-activations are kept for the backward pass.
+bool sur_forward(gpum_hip_surrogate* S, uint32_t rows, bool train,
+                 hipStream_t st) {
+  GemmArgs g;
+  memset(&g, 0, sizeof g);
+  g.A = S->Xb.p, g.sam = kMlpIn, g.sak = 1;
+  g.B = S->W1b.p, g.sbk = kMlpHid, g.sbn = 1;
+  g.M = (int)rows, g.N = kMlpHid, g.K = kMlpIn;
+  g.bias = S->P.p + kOffB1;
+  g.out = train ? S->A1.p : nullptr;
+  g.act_b = S->H1b.p;
+  g.ld = kMlpHid;
+  HIP_TRY("surrogate layer 1", launch_gemm_bf16(g, st));
+  memset(&g, 0, sizeof g);
+  g.A = S->H1b.p, g.sam = kMlpHid, g.sak = 1;
+  g.B = S->W2b.p, g.sbk = kMlpHid, g.sbn = 1;
+  g.M = (int)rows, g.N = kMlpHid, g.K = kMlpHid;
+  g.bias = S->P.p + kOffB2;
+  g.out = train ? S->A2.p : nullptr;
+  g.act = S->H2.p;
+  g.ld = kMlpHid;
+  HIP_TRY("surrogate layer 2", launch_gemm_bf16(g, st));
+  HIP_TRY("surrogate head", launch_mlp_head(S->H2.p, S->P.p, S->s.p, (int)rows, st));
+  return true;
+}
+
+// Backward pass of the loss over the first `valid` of `rows` rows (a
+// multiple of 64) into S->G; the summed squared error lands in S->loss.
+bool sur_backward(gpum_hip_surrogate* S, uint32_t rows, uint32_t valid,
+                  hipStream_t st) {
+  HIP_TRY("surrogate head backward",
+          launch_mlp_head_bwd(S->s.p, S->tb.p, S->H2.p, S->A2.p, S->P.p,
+                              S->D2b.p, S->part.p, (int)rows, (int)valid, st));
+  GemmArgs g;
+  // dW2 = H1^T D2
+  memset(&g, 0, sizeof g);
+  g.A = S->H1b.p, g.sam = 1, g.sak = kMlpHid;
+  g.B = S->D2b.p, g.sbk = kMlpHid, g.sbn = 1;
+  g.M = kMlpHid, g.N = kMlpHid, g.K = (int)rows;
+  g.out = S->G.p + kOffW2;
+  g.ld = kMlpHid;
+  HIP_TRY("surrogate dW2", launch_gemm_bf16(g, st));
+  // D1 = (D2 W2^T) * silu'(A1), column sums for b1
+  memset(&g, 0, sizeof g);
+  g.A = S->D2b.p, g.sam = kMlpHid, g.sak = 1;
+  g.B = S->W2b.p, g.sbk = 1, g.sbn = kMlpHid;
+  g.M = (int)rows, g.N = kMlpHid, g.K = kMlpHid;
+  g.dact = S->A1.p;
+  g.out_b = S->D1b.p;
+  g.colsum = S->cs_b1.p;
+  g.ld = kMlpHid;
+  HIP_TRY("surrogate delta1", launch_gemm_bf16(g, st));
+  // dW1 = X^T D1
+  memset(&g, 0, sizeof g);
+  g.A = S->Xb.p, g.sam = 1, g.sak = kMlpIn;
+  g.B = S->D1b.p, g.sbk = kMlpHid, g.sbn = 1;
+  g.M = kMlpIn, g.N = kMlpHid, g.K = (int)rows;
+  g.out = S->G.p + kOffW1;
+  g.ld = kMlpHid;
+  HIP_TRY("surrogate dW1", launch_gemm_bf16(g, st));
+  HIP_TRY("surrogate gradient",
+          launch_mlp_grad_finish(S->part.p, S->cs_b1.p, (int)rows, S->G.p,
+                                 S->loss.p, st));
+  return true;
+}
+
+// Network inputs of n host points into S->Xb.
+bool sur_features(gpum_hip_surrogate* S, const double* theta, uint32_t n,
+                  DevBuf<double>* tmp, hipStream_t st) {
+  if (!tmp->upload(theta, (size_t)n * S->n_in, "upload points")) return false;
+  HIP_TRY("surrogate features",
+          launch_da_features(tmp->p, n, S->n_in, S->d_mean.p, S->d_whiten.p,
+                             S->Xb.p, st));
+  return true;
+}
+
+bool sur_check(gpum_hip_surrogate* S) {
+  if (!S) {
+    set_err("null surrogate");
+    return false;
+  }
+  if (!device_init()) return false;
+  hipSetDevice(g_dev.id);
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t gpum_hip_surrogate_n_weights(void) { return (uint32_t)kMlpParams; }
+
+gpum_hip_surrogate* gpum_hip_surrogate_create(uint32_t n_in, uint32_t seed) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!device_init()) return nullptr;
+  hipSetDevice(g_dev.id);
+  if (n_in == 0 || n_in > (uint32_t)kMlpIn) {
+    set_err("surrogate inputs must be in [1, %d] (got %u)", kMlpIn, n_in);
+    return nullptr;
+  }
+  gpum_hip_surrogate* S = new gpum_hip_surrogate();
+  S->n_in = n_in;
+  // LeCun-uniform weights (variance 1 / fan_in), zero biases, from Philox
+  // blocks (i, 0, 1, TAG_MLP).
+  std::vector<float> w(kMlpParams, 0.0f);
+  auto fill = [&](int off, int count, int fan_in) {
+    const double a = sqrt(3.0 / (double)fan_in);
+    for (int i = 0; i < count; ++i) {
+      U4 h = philox4x32_10(U4{(uint32_t)(off + i), 0u, 1u, TAG_MLP}, seed, 0u);
+      w[off + i] = (float)(a * (2.0 * u32_to_unit(h.x) - 1.0));
+    }
+  };
+  fill(kOffW1, kMlpIn * kMlpHid, (int)n_in);
+  // Inputs past n_in are always zero; their rows of W1 stay zero.
+  for (int k = (int)n_in; k < kMlpIn; ++k)
+    for (int j = 0; j < kMlpHid; ++j) w[kOffW1 + k * kMlpHid + j] = 0.0f;
+  fill(kOffW2, kMlpHid * kMlpHid, kMlpHid);
+  fill(kOffW3, kMlpHid, kMlpHid);
+  S->mean.assign(n_in, 0.0);
+  S->whiten.assign((size_t)n_in * n_in, 0.0);
+  for (uint32_t k = 0; k < n_in; ++k) S->whiten[k * n_in + k] = 1.0;
+  std::vector<float> zero(kMlpParams, 0.0f);
+  bool ok = S->P.upload(w.data(), w.size(), "upload surrogate weights") &&
+            S->G.upload(zero.data(), zero.size(), "allocate surrogate") &&
+            S->Ma.upload(zero.data(), zero.size(), "allocate surrogate") &&
+            S->Va.upload(zero.data(), zero.size(), "allocate surrogate") &&
+            S->W1b.alloc((size_t)kMlpIn * kMlpHid, "allocate surrogate") &&
+            S->W2b.alloc((size_t)kMlpHid * kMlpHid, "allocate surrogate") &&
+            S->d_mean.upload(S->mean.data(), n_in, "allocate surrogate") &&
+            S->d_whiten.upload(S->whiten.data(), S->whiten.size(), "allocate surrogate");
+  if (ok) {
+    hipError_t e = launch_mlp_mirror(S->P.p, S->W1b.p, S->W2b.p, g_dev.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_dev.stream);
+    if (e != hipSuccess) {
+      set_hip_err("surrogate init", e);
+      ok = false;
+    }
+  }
+  if (!ok) {
+    delete S;
+    return nullptr;
+  }
+  return S;
+}
+
+void gpum_hip_surrogate_free(gpum_hip_surrogate* S) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  delete S;
+}
+
+int gpum_hip_surrogate_set_scaling(gpum_hip_surrogate* S, const double* mean,
+                                   const double* whiten, double target_mean,
+                                   double target_sd) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!sur_check(S)) return 1;
+  if (!mean || !whiten) {
+    set_err("mean and whiten must not be NULL");
+    return 1;
+  }
+  if (!(target_sd > 0.0) || !isfinite(target_sd) || !isfinite(target_mean)) {
+    set_err("target_sd must be positive and finite");
+    return 1;
+  }
+  const uint32_t n = S->n_in;
+  S->mean.assign(mean, mean + n);
+  S->whiten.assign(whiten, whiten + (size_t)n * n);
+  S->t_mean = target_mean;
+  S->t_sd = target_sd;
+  if (!S->d_mean.upload(S->mean.data(), n, "upload scaling") ||
+      !S->d_whiten.upload(S->whiten.data(), S->whiten.size(), "upload scaling"))
+    return 1;
+  return 0;
+}
+
+int gpum_hip_surrogate_get_weights(gpum_hip_surrogate* S, float* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!sur_check(S)) return 1;
+  if (hipMemcpy(out, S->P.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    set_err("read surrogate weights: device error");
+    return 1;
+  }
+  return 0;
+}
+
+int gpum_hip_surrogate_set_weights(gpum_hip_surrogate* S, const float* w) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!sur_check(S)) return 1;
+  hipError_t e = hipMemcpy(S->P.p, w, kMlpParams * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_mlp_mirror(S->P.p, S->W1b.p, S->W2b.p, g_dev.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_dev.stream);
+  if (e != hipSuccess) {
+    set_hip_err("write surrogate weights", e);
+    return 1;
+  }
+  S->adam_t = 0;
+  e = hipMemsetAsync(S->Ma.p, 0, kMlpParams * sizeof(float), g_dev.stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(S->Va.p, 0, kMlpParams * sizeof(float), g_dev.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_dev.stream);
+  if (e != hipSuccess) {
+    set_hip_err("reset optimiser", e);
+    return 1;
+  }
+  return 0;
+}
+
+int gpum_hip_surrogate_gradient(gpum_hip_surrogate* S, const double* theta,
+                                const double* loglik, uint32_t n,
+                                float* grad_out, double* loss_out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!sur_check(S)) return 1;
+  if (n == 0 || n > 65536 || !theta || !loglik || !grad_out) {
+    set_err("gradient needs 1..65536 points and non-NULL buffers");
+    return 1;
+  }
+  const uint32_t rows = round_up(n, 64);
+  if (!sur_workspace(S, rows)) return 1;
+  hipStream_t st = g_dev.stream;
+  DevBuf<double> tmp;
+  if (!sur_features(S, theta, n, &tmp, st)) return 1;
+  std::vector<float> t(rows, 0.0f);
+  for (uint32_t i = 0; i < n; ++i) t[i] = (float)((loglik[i] - S->t_mean) / S->t_sd);
+  if (hipMemcpyAsync(S->tb.p, t.data(), rows * sizeof(float), hipMemcpyHostToDevice,
+                     st) != hipSuccess) {
+    set_err("upload targets: device error");
+    return 1;
+  }
+  if (!sur_forward(S, rows, true, st) || !sur_backward(S, rows, n, st)) return 1;
+  float se = 0.0f;
+  hipError_t e = hipMemcpyAsync(grad_out, S->G.p, kMlpParams * sizeof(float),
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&se, S->loss.p, sizeof(float), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    set_hip_err("surrogate gradient", e);
+    return 1;
+  }
+  if (loss_out) *loss_out = (double)se / (double)n;
+  return 0;
+}
+
+int gpum_hip_surrogate_train(gpum_hip_surrogate* S, const double* theta,
+                             const double* loglik, uint32_t n,
+                             const gpum_train_opts* opts, double* loss_trace,
+                             double* kernel_ms) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!sur_check(S)) return 1;
+  gpum_train_opts o;
+  memset(&o, 0, sizeof o);
+  if (opts) memcpy(&o, opts, std::min<size_t>(opts->struct_size, sizeof o));
+  if (o.steps == 0) o.steps = 2000;
+  if (o.batch == 0) o.batch = 1024;
+  if (!(o.lr > 0.0)) o.lr = 2e-3;
+  if (!(o.lr_final >= 0.0)) o.lr_final = 0.0;
+  if (o.beta1 == 0.0) o.beta1 = 0.9;
+  if (o.beta2 == 0.0) o.beta2 = 0.999;
+  if (o.eps == 0.0) o.eps = 1e-8;
+  if (n == 0 || !theta || !loglik) {
+    set_err("training needs at least one (theta, loglik) pair");
+    return 1;
+  }
+  if (o.batch % 64 || o.batch > 65536) {
+    set_err("batch must be a multiple of 64 and at most 65536");
+    return 1;
+  }
+  if (!sur_workspace(S, o.batch)) return 1;
+  hipStream_t st = g_dev.stream;
+  // The training set as network inputs (bf16 rows) and standardised targets.
+  DevBuf<double> tmp;
+  DevBuf<uint16_t> Xtr;
+  DevBuf<float> Ttr, losses;
+  if (!tmp.upload(theta, (size_t)n * S->n_in, "upload training set") ||
+      !Xtr.alloc((size_t)n * kMlpIn, "allocate training set") ||
+      !losses.alloc(o.steps, "allocate loss trace"))
+    return 1;
+  std::vector<float> t(n);
+  for (uint32_t i = 0; i < n; ++i) t[i] = (float)((loglik[i] - S->t_mean) / S->t_sd);
+  if (!Ttr.upload(t.data(), n, "upload training set")) return 1;
+  hipError_t fe = launch_da_features(tmp.p, n, S->n_in, S->d_mean.p,
+                                     S->d_whiten.p, Xtr.p, st);
+  if (fe != hipSuccess) {
+    set_hip_err("surrogate features", fe);
+    return 1;
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, st);
+  const uint32_t B = o.batch;
+  const uint32_t valid = std::min(B, n);
+  bool ok = true;
+  for (uint32_t k = 0; k < o.steps && ok; ++k) {
+    ok = launch_mlp_gather(Xtr.p, Ttr.p, n, o.seed, k, S->Xb.p, S->tb.p, (int)B,
+                           (int)valid, st) == hipSuccess;
+    ok = ok && sur_forward(S, B, true, st) && sur_backward(S, B, valid, st);
+    if (!ok) break;
+    // cosine decay from lr to lr_final over the steps
+    const double frac = o.steps > 1 ? (double)k / (double)(o.steps - 1) : 1.0;
+    const double lr = o.lr_final + 0.5 * (o.lr - o.lr_final) * (1.0 + cos(M_PI * frac));
+    S->adam_t += 1;
+    const double bc1 = 1.0 - pow(o.beta1, (double)S->adam_t);
+    const double bc2 = 1.0 - pow(o.beta2, (double)S->adam_t);
+    const float lr_t = (float)(lr * sqrt(bc2) / bc1);
+    ok = hipMemcpyAsync(losses.p + k, S->loss.p, sizeof(float),
+                        hipMemcpyDeviceToDevice, st) == hipSuccess &&
+         launch_adam(S->P.p, S->G.p, S->Ma.p, S->Va.p, S->W1b.p, S->W2b.p, lr_t,
+                     (float)o.beta1, (float)o.beta2, (float)o.eps, st) == hipSuccess;
+  }
+  hipEventRecord(e1, st);
+  hipError_t e = hipStreamSynchronize(st);
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (!ok || e != hipSuccess) {
+    if (ok) set_hip_err("surrogate training", e);
+    return 1;
+  }
+  if (kernel_ms) *kernel_ms = ms;
+  if (loss_trace) {
+    std::vector<float> l(o.steps);
+    if (hipMemcpy(l.data(), losses.p, o.steps * sizeof(float), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      set_err("read loss trace: device error");
+      return 1;
+    }
+    for (uint32_t k = 0; k < o.steps; ++k) loss_trace[k] = (double)l[k] / (double)valid;
+  }
+  return 0;
+}
+
+int gpum_hip_surrogate_predict(gpum_hip_surrogate* S, const double* theta,
+                               uint32_t n, double* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!sur_check(S)) return 1;
+  if (n == 0) return 0;
+  if (!theta || !out) {
+    set_err("theta and out must not be NULL");
+    return 1;
+  }
+  const uint32_t chunk = 16384;
+  if (!sur_workspace(S, std::min(round_up(n, 64), chunk))) return 1;
+  hipStream_t st = g_dev.stream;
+  DevBuf<double> tmp;
+  std::vector<float> s(chunk);
+  for (uint32_t i0 = 0; i0 < n; i0 += chunk) {
+    const uint32_t k = std::min(chunk, n - i0);
+    if (!sur_features(S, theta + (size_t)i0 * S->n_in, k, &tmp, st)) return 1;
+    if (!sur_forward(S, round_up(k, 16), false, st)) return 1;
+    hipError_t e = hipMemcpyAsync(s.data(), S->s.p, k * sizeof(float),
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      set_hip_err("surrogate predict", e);
+      return 1;
+    }
+    for (uint32_t i = 0; i < k; ++i) out[i0 + i] = S->t_mean + S->t_sd * (double)s[i];
+  }
+  return 0;
+}
+
+int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
+                          const double* init, uint32_t n_chains,
+                          const double* proposal_sd,
+                          const double* temperatures, uint32_t n_iter,
+                          uint32_t seed, uint32_t proposal_mode,
+                          const double* proposal_l, gpum_da_ext* ext_in,
+                          gpum_chain_result* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!m || !out) {
+    set_err("null model or result");
+    return 1;
+  }
+  if (!device_init()) return 1;
+  hipSetDevice(g_dev.id);
+  gpum_da_ext X;
+  memset(&X, 0, sizeof X);
+  if (ext_in) memcpy(&X, ext_in, std::min<size_t>(ext_in->struct_size, sizeof X));
+  const uint32_t np = m->n_params;
+  const uint32_t C = n_chains;
+  if (C == 0 || !init || !temperatures) {
+    set_err("n_chains must be positive; init and temperatures non-NULL");
+    return 1;
+  }
+  if (proposal_mode != 0 && proposal_mode != 2) {
+    set_err("the stepwise sampler runs proposal_mode 0 or 2 (got %u)", proposal_mode);
+    return 1;
+  }
+  if ((proposal_mode == 0 && !proposal_sd) || (proposal_mode == 2 && !proposal_l)) {
+    set_err("proposal_mode %u needs %s", proposal_mode,
+            proposal_mode == 0 ? "proposal_sd" : "proposal_l");
+    return 1;
+  }
+  for (uint32_t c = 0; c < C; ++c) {
+    if (!(temperatures[c] > 0.0) || !isfinite(temperatures[c])) {
+      set_err("temperatures must be positive and finite");
+      return 1;
+    }
+  }
+  if (S && S->n_in != np) {
+    set_err("surrogate has %u inputs but the model %u parameters", S->n_in, np);
+    return 1;
+  }
+  Tiling t;
+  if (!pick_tiling(m, C, nullptr, &t, true)) return 1;
+  if (!ensure_data(m, t.n_pad)) return 1;
+  const uint32_t n_tiles = t.n_pad / t.tile_len;
+  uint32_t S_sl = X.n_slices;
+  if (S_sl == 0) {
+    // Enough workgroups for the expected number of active slot groups to
+    // cover every CU twice.
+    const double pass = S ? 0.3 : 1.0;
+    const uint32_t groups =
+        std::max(1u, (uint32_t)ceil(pass * (double)C / (double)t.cpb));
+    S_sl = (2u * (uint32_t)g_dev.n_cu + groups - 1) / groups;
+  }
+  S_sl = std::max(1u, std::min(S_sl, n_tiles));
+  const uint32_t tps = (n_tiles + S_sl - 1) / S_sl;
+  S_sl = (n_tiles + tps - 1) / tps;
+
+  const size_t ncp = (size_t)C * np;
+  const size_t n_draws = (size_t)n_iter * ncp;
+  DevBuf<double> theta, prop, cur, cur_s, pprior, prop_s, acc, passes, partial,
+      temps, sd, L;
+  DevBuf<int32_t> pos;
+  DevBuf<uint32_t> list, count;
+  DevBuf<float> draws;
+  DevBuf<double> tr_s, tr_plp;
+  DevBuf<uint8_t> tr_stage;
+  DevBuf<DaArgs> d_args;
+  bool ok = theta.upload(init, ncp, "upload init") &&
+            prop.alloc(ncp, "allocate state") && cur.alloc(C, "allocate state") &&
+            cur_s.alloc(C, "allocate state") && pprior.alloc(C, "allocate state") &&
+            prop_s.alloc(C, "allocate state") && acc.alloc(C, "allocate state") &&
+            passes.alloc(C, "allocate state") &&
+            partial.alloc((size_t)S_sl * C, "allocate partials") &&
+            temps.upload(temperatures, C, "upload temperatures") &&
+            pos.alloc(C, "allocate state") && list.alloc(C, "allocate state") &&
+            count.alloc(1, "allocate state") &&
+            draws.alloc(n_draws, "allocate draws");
+  if (ok && proposal_mode == 0) ok = sd.upload(proposal_sd, ncp, "upload proposal_sd");
+  if (ok && proposal_mode == 2)
+    ok = L.upload(proposal_l, ncp * np, "upload proposal_l");
+  if (ok && X.trace_s) ok = tr_s.alloc((size_t)n_iter * C, "allocate trace");
+  if (ok && X.trace_plp) ok = tr_plp.alloc((size_t)n_iter * C, "allocate trace");
+  if (ok && X.trace_stage) ok = tr_stage.alloc((size_t)n_iter * C, "allocate trace");
+  if (ok && S) ok = sur_workspace(S, round_up(C, 64));
+  if (!ok) return 1;
+  if (X.trace_s && hipMemsetAsync(tr_s.p, 0xFF, (size_t)n_iter * C * sizeof(double),
+                                  g_dev.stream) != hipSuccess) {
+    set_err("clear trace: device error");
+    return 1;
+  }
+
+  DaArgs a;
+  memset(&a, 0, sizeof a);
+  a.n_chains = C;
+  a.n_params = np;
+  a.n_iter = n_iter;
+  a.seed = seed;
+  a.stream_id = X.stream_id;
+  a.iter_offset = X.iter_offset;
+  a.mode = proposal_mode;
+  a.temperatures = temps.p;
+  a.proposal_sd = sd.p;
+  a.proposal_l = L.p;
+  a.prior = m->prior.p;
+  a.n_prior = (uint32_t)m->prior_code.size();
+  a.prior_consts = m->prior_consts.p;
+  a.theta = theta.p;
+  a.prop = prop.p;
+  a.cur = cur.p;
+  a.cur_s = cur_s.p;
+  a.prop_prior = pprior.p;
+  a.prop_s = prop_s.p;
+  a.accepts = acc.p;
+  a.passes = passes.p;
+  a.pos = pos.p;
+  a.list = list.p;
+  a.count = count.p;
+  a.partial = partial.p;
+  a.n_slices = S_sl;
+  a.use_surrogate = S ? 1u : 0u;
+  if (S) {
+    a.n_in = S->n_in;
+    a.norm_mean = S->d_mean.p;
+    a.norm_w = S->d_whiten.p;
+    a.t_mean = S->t_mean;
+    a.t_sd = S->t_sd;
+    a.Xb = S->Xb.p;
+    a.s_raw = S->s.p;
+  }
+  a.draws = draws.p;
+  a.trace_s = X.trace_s ? tr_s.p : nullptr;
+  a.trace_plp = X.trace_plp ? tr_plp.p : nullptr;
+  a.trace_stage = X.trace_stage ? tr_stage.p : nullptr;
+  if (!d_args.upload(&a, 1, "upload launch arguments")) return 1;
+
+  ChainHot h;
+  memset(&h, 0, sizeof h);
+  h.vm = reinterpret_cast<const VmIns4*>(m->vm.p);
+  h.u_init = m->u_init.p;
+  h.data = m->data.p;
+  h.n_vm = m->L.empty ? 0 : (uint32_t)m->L.vm.size();
+  h.n_uniform_pad = m->n_uniform_pad;
+  h.n_cols = m->n_cols_total;
+  h.n_obs = m->n_obs;
+  h.n_pad = t.n_pad;
+  h.ld = m->data_ld;
+  h.n_chains = C;
+  h.cpb = t.cpb;
+  h.wpc = 1;
+  h.tile_len = t.tile_len;
+  h.stream_mode = t.stream_mode;
+  h.lds_tile_offset = t.lds_tile_offset;
+  PointsArgs pa;
+  memset(&pa, 0, sizeof pa);
+  pa.n_points = C;
+  pa.n_params = np;
+  pa.hoist = m->hoist.p;
+  pa.n_hoist = (uint32_t)m->L.hoist.size();
+  pa.consts = m->consts.p;
+  pa.n_slices = S_sl;
+  pa.tiles_per_slice = tps;
+  pa.partial = partial.p;
+
+  hipStream_t st = g_dev.stream;
+  const uint32_t rows16 = round_up(C, 16);
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  // Initial states: exact log-posterior and surrogate value.
+  chk(launch_da_init(d_args.p, C, st));
+  if (S && e == hipSuccess && !sur_forward(S, rows16, false, st)) return 1;
+  pa.points = theta.p;
+  pa.index = nullptr;
+  pa.count = nullptr;
+  chk(launch_points_kernel(pa, h, t.v.ns, t.v.opt, t.lds_bytes, st));
+  chk(launch_da_init_finish(d_args.p, C, st));
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, st);
+  pa.points = prop.p;
+  pa.index = list.p;
+  pa.count = count.p;
+  for (uint32_t it = 0; it < n_iter && e == hipSuccess; ++it) {
+    chk(launch_da_propose(d_args.p, C, it, st));
+    if (S && e == hipSuccess && !sur_forward(S, rows16, false, st)) {
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      return 1;
+    }
+    chk(launch_da_stage1(d_args.p, it, st));
+    chk(launch_points_kernel(pa, h, t.v.ns, t.v.opt, t.lds_bytes, st));
+    chk(launch_da_stage2(d_args.p, C, it, st));
+  }
+  hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  float ms = 0.0f;
+  if (e == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (e != hipSuccess) {
+    set_hip_err("stepwise sampler", e);
+    return 1;
+  }
+  std::vector<double> h_acc(C), h_pass(C);
+  out->n_iter = n_iter;
+  out->n_chains = C;
+  out->n_params = np;
+  out->draws = (float*)malloc(std::max<size_t>(1, n_draws) * sizeof(float));
+  out->accept_rate = (float*)malloc(C * sizeof(float));
+  out->last_log_post = (double*)malloc(C * sizeof(double));
+  if (!out->draws || !out->accept_rate || !out->last_log_post) {
+    gpum_hip_result_free(out);
+    set_err("host out of memory for results");
+    return 1;
+  }
+  ok = hipMemcpy(out->draws, draws.p, n_draws * sizeof(float), hipMemcpyDeviceToHost) ==
+           hipSuccess &&
+       hipMemcpy(out->last_log_post, cur.p, C * sizeof(double), hipMemcpyDeviceToHost) ==
+           hipSuccess &&
+       hipMemcpy(h_acc.data(), acc.p, C * sizeof(double), hipMemcpyDeviceToHost) ==
+           hipSuccess &&
+       hipMemcpy(h_pass.data(), passes.p, C * sizeof(double), hipMemcpyDeviceToHost) ==
+           hipSuccess;
+  if (ok && X.final_state)
+    ok = hipMemcpy(X.final_state, theta.p, ncp * sizeof(double), hipMemcpyDeviceToHost) ==
+         hipSuccess;
+  if (ok && X.trace_s)
+    ok = hipMemcpy(X.trace_s, tr_s.p, (size_t)n_iter * C * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && X.trace_plp)
+    ok = hipMemcpy(X.trace_plp, tr_plp.p, (size_t)n_iter * C * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && X.trace_stage)
+    ok = hipMemcpy(X.trace_stage, tr_stage.p, (size_t)n_iter * C,
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) {
+    gpum_hip_result_free(out);
+    set_err("read results: device error");
+    return 1;
+  }
+  double evals = 0.0;
+  for (uint32_t c = 0; c < C; ++c) {
+    out->accept_rate[c] = n_iter == 0 ? 0.0f : (float)h_acc[c] / (float)n_iter;
+    if (X.stage1_rate) X.stage1_rate[c] = n_iter == 0 ? 0.0 : h_pass[c] / n_iter;
+    evals += h_pass[c];
+  }
+  if (ext_in) {
+    gpum_da_ext* o = ext_in;
+    if (o->struct_size >= offsetof(gpum_da_ext, kernel_ms) + sizeof(double))
+      o->kernel_ms = ms;
+    if (o->struct_size >= offsetof(gpum_da_ext, exact_evals) + sizeof(double))
+      o->exact_evals = evals;
+  }
+  return 0;
+}
+
+}  // extern "C"

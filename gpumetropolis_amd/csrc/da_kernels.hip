@@ -1,0 +1,229 @@
+// da_kernels.hip -- the stepwise sampler's per-iteration kernels.
+//
+// One iteration is a short sequence of launches on one stream:
+//   propose   one thread per chain: Philox normals and the proposal
+//             (cpu_native.rs:726-784, modes 0 and 2), its prior, and with a
+//             surrogate the network's input row;
+//   [network forward over all proposals]
+//   stage1    one workgroup: the surrogate's Metropolis test
+//             log u1 < (s(prop) - s(theta)) / T, and an ordered compaction of
+//             the surviving proposals into the exact stage's slot list;
+//   points    the exact data term of the listed proposals (chain kernel
+//             file), split over observation slices;
+//   stage2    one thread per chain: the exact test, corrected by the
+//             surrogate ratio so the target is unchanged (Christen and Fox,
+//             "Markov chain Monte Carlo using an approximation", JCGS 2005):
+//             log u < ((lp' - lp) - (s' - s)) / T; record the draw.
+// Without a surrogate every proposal passes stage 1 and stage 2 is the
+// reference's Metropolis test (cpu_native.rs:785-799) with the same
+// uniforms as the fused chain kernel.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "bf16.h"
+#include "da.h"
+#include "mlp.h"
+#include "philox.h"
+#include "step_common.h"
+
+namespace gpum {
+
+__device__ __forceinline__ void write_features(const double* __restrict__ p,
+                                               uint32_t n_in,
+                                               const double* __restrict__ mean,
+                                               const double* __restrict__ w,
+                                               uint16_t* __restrict__ row) {
+  for (uint32_t k = 0; k < (uint32_t)kMlpIn; ++k) {
+    double x = 0.0;
+    if (k < n_in)
+      for (uint32_t j = 0; j < n_in; ++j) x += w[k * n_in + j] * (p[j] - mean[j]);
+    row[k] = f2bf((float)x);
+  }
+}
+
+__global__ void da_features_kernel(const double* __restrict__ points, uint32_t n,
+                                   uint32_t n_in, const double* __restrict__ mean,
+                                   const double* __restrict__ w,
+                                   uint16_t* __restrict__ Xb) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  write_features(points + (size_t)i * n_in, n_in, mean, w, Xb + (size_t)i * kMlpIn);
+}
+
+__global__ void da_init_kernel(const DaArgs* __restrict__ A) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A->n_chains) return;
+  const uint32_t np = A->n_params;
+  const double* th = A->theta + (size_t)c * np;
+  A->prop_prior[c] = eval_params64(A->prior, A->n_prior, A->prior_consts, th, nullptr);
+  A->accepts[c] = 0.0;
+  A->passes[c] = 0.0;
+  if (A->use_surrogate)
+    write_features(th, A->n_in, A->norm_mean, A->norm_w, A->Xb + (size_t)c * kMlpIn);
+}
+
+// cur = data term of the initial state (slot c) + its prior; the reference
+// evaluates the start before its loop (cpu_native.rs:565).
+__global__ void da_init_finish_kernel(const DaArgs* __restrict__ A) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t C = A->n_chains;
+  if (c >= C) return;
+  double ll = 0.0;
+  for (uint32_t s = 0; s < A->n_slices; ++s) ll += A->partial[(size_t)s * C + c];
+  const double pr = A->prop_prior[c];
+  A->cur[c] = ll + pr;
+  A->cur_s[c] = A->use_surrogate
+                    ? pr + (A->t_mean + A->t_sd * (double)A->s_raw[c])
+                    : 0.0;
+}
+
+__global__ void da_propose_kernel(const DaArgs* __restrict__ A, uint32_t t) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A->n_chains) return;
+  const uint32_t np = A->n_params;
+  const double two_pi = 6.283185307179586;
+  StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
+  const double* th = A->theta + (size_t)c * np;
+  double* pr = A->prop + (size_t)c * np;
+  if (A->mode == 2) {
+    double z[kMaxParams];
+    for (uint32_t j = 0; j < np; ++j) {
+      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
+      z[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+    }
+    const double* L = A->proposal_l + (size_t)c * np * np;
+    for (uint32_t k = 0; k < np; ++k) {
+      double a = th[k];
+      for (uint32_t j = 0; j <= k; ++j) a += L[k * np + j] * z[j];
+      pr[k] = a;
+    }
+  } else {
+    const double* sd = A->proposal_sd + (size_t)c * np;
+    for (uint32_t j = 0; j < np; ++j) {
+      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
+      double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+      pr[j] = th[j] + sd[j] * z;
+    }
+  }
+  A->prop_prior[c] = eval_params64(A->prior, A->n_prior, A->prior_consts, pr, nullptr);
+  if (A->use_surrogate)
+    write_features(pr, A->n_in, A->norm_mean, A->norm_w, A->Xb + (size_t)c * kMlpIn);
+}
+
+// One workgroup of 1024 threads walks the chains in order, so the slot of
+// every surviving proposal is its rank among the survivors.
+__global__ void __launch_bounds__(1024) da_stage1_kernel(const DaArgs* __restrict__ A,
+                                                         uint32_t t) {
+  __shared__ uint32_t wave_tot[16];
+  const uint32_t C = A->n_chains;
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  uint32_t base = 0;
+  for (uint32_t c0 = 0; c0 < C; c0 += 1024) {
+    const uint32_t c = c0 + (uint32_t)tid;
+    bool pass = false;
+    if (c < C) {
+      if (A->use_surrogate) {
+        const double sp = A->prop_prior[c] + (A->t_mean + A->t_sd * (double)A->s_raw[c]);
+        A->prop_s[c] = sp;
+        const U4 w = philox4x32_10(U4{A->iter_offset + t, c, 0u, TAG_DA}, A->seed,
+                                   A->stream_id);
+        const double u1 = u32_to_unit(w.x);
+        pass = isfinite(sp) && log(u1) < (sp - A->cur_s[c]) / A->temperatures[c];
+        if (A->trace_s) A->trace_s[(size_t)t * C + c] = sp;
+      } else {
+        pass = true;
+      }
+    }
+    const unsigned long long mask = __ballot(pass);
+    const uint32_t rank =
+        (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t off = base, total = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wave) off += wave_tot[w];
+      total += wave_tot[w];
+    }
+    if (c < C) {
+      if (pass) {
+        A->list[off + rank] = c;
+        A->pos[c] = (int32_t)(off + rank);
+        A->passes[c] += 1.0;
+      } else {
+        A->pos[c] = -1;
+      }
+    }
+    base += total;
+    __syncthreads();
+  }
+  if (tid == 0) *A->count = base;
+}
+
+__global__ void da_stage2_kernel(const DaArgs* __restrict__ A, uint32_t t) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t C = A->n_chains;
+  if (c >= C) return;
+  const uint32_t np = A->n_params;
+  const int32_t p = A->pos[c];
+  bool acc_ok = false;
+  double lpp = __builtin_nan("");
+  if (p >= 0) {
+    double ll = 0.0;
+    for (uint32_t s = 0; s < A->n_slices; ++s) ll += A->partial[(size_t)s * C + p];
+    lpp = ll + A->prop_prior[c];
+    double la = lpp - A->cur[c];
+    if (A->use_surrogate) la -= A->prop_s[c] - A->cur_s[c];
+    StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
+    const double u = us.get(2 * np);
+    acc_ok = log(u) < la / A->temperatures[c];
+  }
+  double* th = A->theta + (size_t)c * np;
+  if (acc_ok) {
+    const double* pr = A->prop + (size_t)c * np;
+    for (uint32_t j = 0; j < np; ++j) th[j] = pr[j];
+    A->cur[c] = lpp;
+    if (A->use_surrogate) A->cur_s[c] = A->prop_s[c];
+    A->accepts[c] += 1.0;
+  }
+  if (A->trace_plp) A->trace_plp[(size_t)t * C + c] = lpp;
+  if (A->trace_stage) A->trace_stage[(size_t)t * C + c] = p < 0 ? 0 : (acc_ok ? 2 : 1);
+  const size_t T = A->n_iter;
+  for (uint32_t j = 0; j < np; ++j) A->draws[t + T * (c + (size_t)C * j)] = (float)th[j];
+}
+
+static dim3 chain_grid(uint32_t n) { return dim3((n + 255) / 256); }
+
+hipError_t launch_da_init(const DaArgs* A, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(da_init_kernel, chain_grid(n), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+hipError_t launch_da_init_finish(const DaArgs* A, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(da_init_finish_kernel, chain_grid(n), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+hipError_t launch_da_propose(const DaArgs* A, uint32_t n, uint32_t t,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(da_propose_kernel, chain_grid(n), dim3(256), 0, s, A, t);
+  return hipGetLastError();
+}
+hipError_t launch_da_stage1(const DaArgs* A, uint32_t t, hipStream_t s) {
+  hipLaunchKernelGGL(da_stage1_kernel, dim3(1), dim3(1024), 0, s, A, t);
+  return hipGetLastError();
+}
+hipError_t launch_da_stage2(const DaArgs* A, uint32_t n, uint32_t t,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(da_stage2_kernel, chain_grid(n), dim3(256), 0, s, A, t);
+  return hipGetLastError();
+}
+hipError_t launch_da_features(const double* points, uint32_t n, uint32_t n_in,
+                              const double* mean, const double* w,
+                              uint16_t* Xb, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(da_features_kernel, chain_grid(n), dim3(256), 0, s, points,
+                     n, n_in, mean, w, Xb);
+  return hipGetLastError();
+}
+
+}  // namespace gpum

@@ -21,12 +21,17 @@
 #include "../../include/gpum_hip.h"
 #include "chain_kernel.h"
 #include "lower.h"
+#include "runtime.h"
 
 using namespace gpum;
+using namespace gpum::rt;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace gpum {
+namespace rt {
 
 void set_err(const char* fmt, ...) {
   char buf[1024];
@@ -47,26 +52,7 @@ void set_hip_err(const char* what, hipError_t e) {
     set_err("%s: kernel/device error (%s)", what, hipGetErrorString(e));
 }
 
-#define HIP_TRY(what, expr)            \
-  do {                                 \
-    hipError_t e_ = (expr);            \
-    if (e_ != hipSuccess) {            \
-      set_hip_err(what, e_);           \
-      return false;                    \
-    }                                  \
-  } while (0)
-
-struct Device {
-  bool ready = false;
-  bool ok = false;
-  int id = 0;
-  int n_cu = 0;
-  size_t lds_max = 0;
-  hipStream_t stream = nullptr;
-  std::string why;
-};
-
-std::mutex g_mu;  // serialises device work of the library
+std::mutex g_mu;
 Device g_dev;
 
 bool device_init() {
@@ -112,44 +98,6 @@ bool device_init() {
   return true;
 }
 
-template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  bool alloc(size_t count, const char* what) {
-    release();
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      set_hip_err(what, e);
-      return false;
-    }
-    n = count;
-    return true;
-  }
-  bool upload(const T* src, size_t count, const char* what) {
-    if (!alloc(count, what)) return false;
-    if (count == 0 || src == nullptr) return true;
-    hipError_t e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      set_hip_err(what, e);
-      return false;
-    }
-    return true;
-  }
-};
-
-struct Variant {
-  int ns, opt;
-};
-
 Variant pick_variant(int vm_depth) {
   int need = vm_depth > 1 ? vm_depth - 1 : 0;
   if (need == 0) return {0, 16};
@@ -162,26 +110,8 @@ Variant pick_variant(int vm_depth) {
 
 uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
 
-}  // namespace
-
-struct gpum_hip_model {
-  Lowered L;
-  uint32_t n_params = 0;
-  uint32_t n_cols_total = 0;
-  uint32_t n_obs = 0;
-  // host copies needed to lay the data out per tiling
-  std::vector<float> cols;  // column-major, n_obs per column, unpadded
-  // device program
-  DevBuf<uint32_t> vm, hoist, prior;
-  DevBuf<double> consts, prior_consts;
-  DevBuf<float> u_init;
-  uint32_t n_uniform_pad = 1;
-  std::vector<uint32_t> prior_code;
-  // device data, laid out for `data_ld`
-  DevBuf<float> data;
-  uint32_t data_ld = 0;
-  uint32_t data_chunk = 0;  // layout granularity the data was padded to
-};
+}  // namespace rt
+}  // namespace gpum
 
 namespace {
 
@@ -276,15 +206,13 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
   return true;
 }
 
-struct Tiling {
-  Variant v;
-  uint32_t cpb, wpc, chunk, tile_len, n_pad, resident, stream_mode,
-      lds_tile_offset;
-  size_t lds_bytes;
-};
+}  // namespace
+
+namespace gpum {
+namespace rt {
 
 bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
-                 const gpum_run_ext* ext, Tiling* t) {
+                 const gpum_run_ext* ext, Tiling* t, bool points) {
   t->v = pick_variant(m->L.vm_depth);
   const uint32_t opt = (uint32_t)t->v.opt;
   t->chunk = 64u * opt;
@@ -323,6 +251,12 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
     cpb = G * std::max(1u, (16u / wpc) / G);
     while (cpb > G && cpb > n_chains) cpb -= G;
   }
+  if (points) {
+    // One wave per slot: the points kernel splits observations over
+    // workgroups instead of waves.
+    wpc = 1;
+    cpb = std::min(16u, std::max(1u, n_chains));
+  }
   if (cpb * wpc > 16 || cpb == 0 || wpc == 0) {
     set_err("invalid tiling: chains_per_block * waves_per_chain must be <= 16");
     return false;
@@ -331,6 +265,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   t->wpc = wpc;
   const uint32_t ncol = std::max(1u, m->n_cols_total);
   auto head_bytes = [&](uint32_t c, uint32_t w) {
+    if (points) return ((size_t)c * m->n_uniform_pad * 4 + 15) / 16 * 16;
     const size_t state_bytes = (size_t)c * (6 * m->n_params + 6) * 8;
     const size_t part_bytes = (size_t)c * w * 8;
     const size_t u_bytes = (size_t)c * m->n_uniform_pad * 4;
@@ -397,6 +332,11 @@ bool ensure_data(gpum_hip_model* m, uint32_t n_pad) {
   m->data_ld = n_pad;
   return true;
 }
+
+}  // namespace rt
+}  // namespace gpum
+
+namespace {
 
 bool validate_run(const gpum_hip_model* m, const double* init,
                   uint32_t n_chains, const double* proposal_sd,
@@ -542,7 +482,7 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   const size_t ncp = (size_t)n_chains * np;
   const size_t n_draws = (size_t)n_iter * ncp;
   DevBuf<double> d_init, d_sd, d_temp, d_l, d_lp, d_final, d_tplp, d_tprop, d_tinit;
-  DevBuf<double> d_iscale, d_fsd, d_fscale;
+  DevBuf<double> d_iscale, d_fsd, d_fscale, d_tll;
   DevBuf<float> d_draws, d_acc, d_acch;
   DevBuf<uint8_t> d_tacc, d_swaph;
   DevBuf<ChainArgs> d_args;
@@ -561,11 +501,13 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   const bool want_tprop = ext && ext->trace_prop;
   const bool want_tacc = ext && ext->trace_accept;
   const bool want_tinit = ext && ext->init_log_post;
+  const bool want_tll = ext && ext->trace_ll;
   if (want_final && !d_final.alloc(ncp, "allocate final_state")) return fail();
   if (want_tplp && !d_tplp.alloc((size_t)n_iter * n_chains, "allocate trace")) return fail();
   if (want_tprop && !d_tprop.alloc(n_draws, "allocate trace")) return fail();
   if (want_tacc && !d_tacc.alloc((size_t)n_iter * n_chains, "allocate trace")) return fail();
   if (want_tinit && !d_tinit.alloc(n_chains, "allocate trace")) return fail();
+  if (want_tll && !d_tll.alloc((size_t)n_iter * n_chains, "allocate trace")) return fail();
   const bool want_swaph = ext && ext->swap_hist && n_batches > 0;
   const bool want_acch = ext && ext->acc_hist && n_batches > 0;
   const bool want_fsd = ext && ext->final_sd;
@@ -620,6 +562,7 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   a.trace_plp = want_tplp ? d_tplp.p : nullptr;
   a.trace_prop = want_tprop ? d_tprop.p : nullptr;
   a.trace_accept = want_tacc ? d_tacc.p : nullptr;
+  a.trace_ll = want_tll ? d_tll.p : nullptr;
   a.init_log_post = want_tinit ? d_tinit.p : nullptr;
   a.iter_offset = ext ? ext->iter_offset : 0;
   a.swap_every = ext ? ext->swap_every : 0;
@@ -720,6 +663,9 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
                    hipMemcpyDeviceToHost) == hipSuccess;
   if (ok && want_tplp)
     ok = hipMemcpy(ext->trace_plp, d_tplp.p, (size_t)n_iter * n_chains * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && want_tll)
+    ok = hipMemcpy(ext->trace_ll, d_tll.p, (size_t)n_iter * n_chains * sizeof(double),
                    hipMemcpyDeviceToHost) == hipSuccess;
   if (ok && want_tprop)
     ok = hipMemcpy(ext->trace_prop, d_tprop.p, n_draws * sizeof(double),

@@ -22,6 +22,7 @@ enum PhiloxTag : uint32_t {
   TAG_INIT = 2,   // initial values drawn by the host API
   TAG_DA = 3,     // delayed-acceptance stage uniforms
   TAG_DATA = 4,   // synthetic data generation
+  TAG_MLP = 5,    // surrogate initialisation and minibatch sampling
 };
 
 struct U4 {

@@ -1,0 +1,125 @@
+// runtime.h -- internals of the host runtime shared by its source files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpum_hip.h"
+#include "chain_kernel.h"
+#include "lower.h"
+
+namespace gpum {
+namespace rt {
+
+void set_err(const char* fmt, ...);
+// A HIP failure as a message carrying the phrases the R guard matches.
+void set_hip_err(const char* what, hipError_t e);
+
+#define HIP_TRY(what, expr)            \
+  do {                                 \
+    hipError_t e_ = (expr);            \
+    if (e_ != hipSuccess) {            \
+      ::gpum::rt::set_hip_err(what, e_); \
+      return false;                    \
+    }                                  \
+  } while (0)
+
+struct Device {
+  bool ready = false;
+  bool ok = false;
+  int id = 0;
+  int n_cu = 0;
+  size_t lds_max = 0;
+  hipStream_t stream = nullptr;
+  std::string why;
+};
+
+extern std::mutex g_mu;  // serialises device work of the library
+extern Device g_dev;
+bool device_init();
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  bool alloc(size_t count, const char* what) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_hip_err(what, e);
+      return false;
+    }
+    n = count;
+    return true;
+  }
+  bool upload(const T* src, size_t count, const char* what) {
+    if (!alloc(count, what)) return false;
+    if (count == 0 || src == nullptr) return true;
+    hipError_t e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      set_hip_err(what, e);
+      return false;
+    }
+    return true;
+  }
+};
+
+struct Variant {
+  int ns, opt;
+};
+
+Variant pick_variant(int vm_depth);
+uint32_t round_up(uint32_t x, uint32_t m);
+
+struct Tiling {
+  Variant v;
+  uint32_t cpb, wpc, chunk, tile_len, n_pad, resident, stream_mode,
+      lds_tile_offset;
+  size_t lds_bytes;
+};
+
+}  // namespace rt
+}  // namespace gpum
+
+struct gpum_hip_model {
+  gpum::Lowered L;
+  uint32_t n_params = 0;
+  uint32_t n_cols_total = 0;
+  uint32_t n_obs = 0;
+  // host copies needed to lay the data out per tiling
+  std::vector<float> cols;  // column-major, n_obs per column, unpadded
+  // device program
+  gpum::rt::DevBuf<uint32_t> vm, hoist, prior;
+  gpum::rt::DevBuf<double> consts, prior_consts;
+  gpum::rt::DevBuf<float> u_init;
+  uint32_t n_uniform_pad = 1;
+  std::vector<uint32_t> prior_code;
+  // device data, laid out for `data_ld`
+  gpum::rt::DevBuf<float> data;
+  uint32_t data_ld = 0;
+  uint32_t data_chunk = 0;  // layout granularity the data was padded to
+};
+
+namespace gpum {
+namespace rt {
+
+// Tiling of a launch over n_chains chains (or, with `points`, of the points
+// kernel over n_chains slots: one wave per slot, 16 slots per workgroup).
+bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
+                 const gpum_run_ext* ext, Tiling* t, bool points = false);
+// Lay the model's columns out on the device with column stride n_pad.
+bool ensure_data(gpum_hip_model* m, uint32_t n_pad);
+
+}  // namespace rt
+}  // namespace gpum

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import numpy as np
 
-TAG_STEP, TAG_SWAP, TAG_INIT, TAG_DA, TAG_DATA = 0, 1, 2, 3, 4
+TAG_STEP, TAG_SWAP, TAG_INIT, TAG_DA, TAG_DATA, TAG_MLP = 0, 1, 2, 3, 4, 5
 
 _M0 = np.uint64(0xD2511F53)
 _M1 = np.uint64(0xCD9E8D57)

@@ -100,6 +100,7 @@ typedef struct gpum_run_ext {
   float* acc_hist;        /* out: n_batches * n_chains batch acceptance */
   double* final_sd;       /* out: n_chains * n_params adapted proposal sd */
   double* final_scale;    /* out: n_chains RM scales */
+  double* trace_ll;       /* out: n_iter * n_chains data term of each proposal */
 } gpum_run_ext;
 
 /* Static description of a lowered model. */
@@ -186,6 +187,92 @@ int gpum_hip_model_loglik(gpum_hip_model* m, const double* points,
                           uint32_t n_points, double* out);
 
 void gpum_hip_model_free(gpum_hip_model* m);
+
+/* ---- Surrogate delayed acceptance -------------------------------------
+ * No counterpart in the reference: these entry points add a learned first
+ * stage to the random-walk sampler.  The surrogate is an MLP
+ * n_in(<=16, zero-padded) - 256 - 256 - 1 with SiLU activations, trained on
+ * the GPU (bf16 matrix-core products, FP32 accumulation, Adam on FP32
+ * weights) to predict a model's summed log-likelihood from a parameter
+ * vector.  Inputs are whitened, x = W (theta - mean); outputs are
+ * loglik = target_mean + target_sd * network(x). */
+typedef struct gpum_hip_surrogate gpum_hip_surrogate;
+
+/* Number of FP32 weights of the network (the flat vector's length):
+ * W1 16x256 | b1 256 | W2 256x256 | b2 256 | w3 256 | b3 1, row-major
+ * (input index first). */
+uint32_t gpum_hip_surrogate_n_weights(void);
+
+/* A network for n_in <= 16 inputs, weights drawn from `seed`. */
+gpum_hip_surrogate* gpum_hip_surrogate_create(uint32_t n_in, uint32_t seed);
+void gpum_hip_surrogate_free(gpum_hip_surrogate* s);
+
+/* mean: n_in; whiten: n_in x n_in row-major. */
+int gpum_hip_surrogate_set_scaling(gpum_hip_surrogate* s, const double* mean,
+                                   const double* whiten, double target_mean,
+                                   double target_sd);
+int gpum_hip_surrogate_get_weights(gpum_hip_surrogate* s, float* out);
+int gpum_hip_surrogate_set_weights(gpum_hip_surrogate* s, const float* w);
+
+/* Gradient of the mean squared error of the standardised prediction,
+ * mean_i (network(x_i) - (loglik_i - target_mean) / target_sd)^2, over n
+ * points (n <= 65536), with respect to the flat weights.  grad_out:
+ * gpum_hip_surrogate_n_weights() floats; loss_out: the loss. */
+int gpum_hip_surrogate_gradient(gpum_hip_surrogate* s, const double* theta,
+                                const double* loglik, uint32_t n,
+                                float* grad_out, double* loss_out);
+
+typedef struct gpum_train_opts {
+  uint32_t struct_size;
+  uint32_t steps;     /* Adam steps */
+  uint32_t batch;     /* rows per step (multiple of 64, <= 65536) */
+  uint32_t seed;      /* minibatch sampling stream */
+  double lr;          /* initial learning rate */
+  double lr_final;    /* cosine decay target */
+  double beta1, beta2, eps;
+} gpum_train_opts;
+
+/* Train on n (theta, loglik) pairs; loss_trace (steps doubles, may be NULL)
+ * receives each step's minibatch loss; kernel_ms (may be NULL) the device
+ * time of the whole training. */
+int gpum_hip_surrogate_train(gpum_hip_surrogate* s, const double* theta,
+                             const double* loglik, uint32_t n,
+                             const gpum_train_opts* opts, double* loss_trace,
+                             double* kernel_ms);
+
+/* Predicted log-likelihood at n points. */
+int gpum_hip_surrogate_predict(gpum_hip_surrogate* s, const double* theta,
+                               uint32_t n, double* out);
+
+/* Extension of a stepwise run.  Zero-initialise and set struct_size. */
+typedef struct gpum_da_ext {
+  uint32_t struct_size;
+  uint32_t stream_id;
+  uint32_t iter_offset;
+  uint32_t n_slices;        /* observation slices of the exact stage (0 auto) */
+  double* final_state;      /* out: n_chains * n_params */
+  double* stage1_rate;      /* out: n_chains fraction reaching the exact stage */
+  double* trace_s;          /* out: n_iter * n_chains surrogate log-posterior */
+  double* trace_plp;        /* out: n_iter * n_chains exact (NaN: not run) */
+  uint8_t* trace_stage;     /* out: 0 stage-1 reject, 1 stage-2 reject, 2 accept */
+  double kernel_ms;         /* out: device time of the iterations */
+  double exact_evals;       /* out: proposals evaluated exactly */
+} gpum_da_ext;
+
+/* Random-walk Metropolis (proposal_mode 0 or 2) run one iteration per
+ * launch sequence.  With a surrogate (s non-NULL, n_in == n_params), each
+ * proposal first faces the surrogate's Metropolis test and only survivors
+ * are evaluated exactly; the exact test divides out the surrogate ratio,
+ * so the target is unchanged (two-stage delayed acceptance).  With
+ * s == NULL every proposal is evaluated exactly, with the uniforms of
+ * gpum_hip_model_run.  Result layout as gpum_hip_run_chains. */
+int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* s,
+                          const double* init, uint32_t n_chains,
+                          const double* proposal_sd,
+                          const double* temperatures, uint32_t n_iter,
+                          uint32_t seed, uint32_t proposal_mode,
+                          const double* proposal_l, gpum_da_ext* ext,
+                          gpum_chain_result* out);
 
 #ifdef __cplusplus
 }
