@@ -47,6 +47,15 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target duration of the CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-da", action="store_true",
+                   help="skip the surrogate delayed-acceptance comparison")
+    p.add_argument("--da-obs", type=int, default=4_000_000)
+    p.add_argument("--da-chains", type=int, default=1024)
+    p.add_argument("--da-warmup", type=int, default=600)
+    p.add_argument("--da-iters", type=int, default=1000,
+                   help="iterations per chunk of the R-hat-gated runs")
+    p.add_argument("--da-max-iters", type=int, default=16000)
+    p.add_argument("--da-train-steps", type=int, default=4000)
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device-to-device copy of MIB MiB "
                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -95,6 +104,148 @@ def cpu_baseline(w, target_s):
                       f"(compiled f64 loglik, {cores} threads, {dt:.1f} s)"}
 
 
+MLP_FLOP_PER_ROW = 2 * (16 * 256 + 256 * 256)  # matrix-core flops, forward
+MFMA_BF16_PEAK = 2.5e15
+
+
+def da_section(a):
+    """Plain random-walk Metropolis vs surrogate delayed acceptance on the
+    16-dim logistic regression (4e6 FP32 rows, 1024 chains).  Both sample
+    the same number of iterations from the same warmed-up states with the
+    same proposal; plain MH runs on the faster of the two plain engines
+    (the stepwise sampler splits the observations over every CU; the fused
+    chain kernel's 1024 chains fill a quarter of them)."""
+    import torch
+
+    from gpumetropolis_amd import _native, adaptive as ad, workloads
+    from gpumetropolis_amd.api import _StepCaller, adaptive_warmup
+    from gpumetropolis_amd.diagnostics import rhat, summarize
+    from gpumetropolis_amd.surrogate import fit_surrogate, tune_scale
+
+    t0 = time.perf_counter()
+    w = workloads.logistic(n=a.da_obs, n_chains=a.da_chains)
+    m = w.model
+    d, C, N = m.n_params, a.da_chains, w.n_obs
+    dm = _native.DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
+                             m.prior.consts, d, w.data, w.n_cols, N)
+    info = dm.info()
+    print(f"[bench-da] data + model {time.perf_counter() - t0:.1f} s: {info}",
+          file=sys.stderr, flush=True)
+    rng = np.random.default_rng(a.seed)
+    # chains start within a few posterior sds (~1.3e-3) of the truth
+    init = w.meta["true"] + rng.normal(0.0, 0.003, (C, d))
+    call = _StepCaller(dm, C)
+    t0 = time.perf_counter()
+    wu = adaptive_warmup(call, d, C, init, np.full((C, d), 5e-4),
+                         ad.warmup_batches(a.da_warmup), a.seed, collect=True)
+    warm_s = time.perf_counter() - t0
+    print(f"[bench-da] warmup {a.da_warmup} it {warm_s:.1f} s, accept "
+          f"{wu.accept_history[:, -1].mean():.3f}, pairs {wu.pairs_loglik.size}",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    fit = fit_surrogate(wu.pairs_theta, wu.pairs_loglik, seed=a.seed,
+                        steps=a.da_train_steps)
+    fit_s = time.perf_counter() - t0
+    print(f"[bench-da] surrogate {fit_s:.1f} s: {fit.summary()}",
+          file=sys.stderr, flush=True)
+    seed2 = (a.seed + wu.batches_run) & 0xFFFFFFFF
+    res = {}
+    # The fused chain kernel on the same states, briefly, for the record.
+    r = dm.run(wu.init, wu.sd, np.ones(C), 20, seed2, proposal_mode=2,
+               proposal_l=wu.L)
+    fused_ms = r.kernel_ms / 20
+    factor, pilots = 1.0, []
+    if fit.usable:
+        t0 = time.perf_counter()
+        factor, pilots = tune_scale(dm, fit.net, wu.init, fit.whiten, seed2,
+                                    proposal_l=wu.L)
+        pilot_s = time.perf_counter() - t0
+        print(f"[bench-da] scale factor {factor} ({pilot_s:.1f} s): {pilots}",
+              file=sys.stderr, flush=True)
+    for name, net, f in (("plain", None, 1.0),
+                         ("da", fit.net if fit.usable else None, factor)):
+        if name == "da" and net is None:
+            break
+        # Chunks of da_iters iterations, continued from the exact states,
+        # until every parameter's split R-hat is below 1.01 (or the cap).
+        draws, state, wall, kms, exact, t = [], wu.init, 0.0, 0.0, 0.0, 0
+        s1, acc = [], []
+        while True:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = dm.run_da(net, state, np.ones(C), a.da_iters, seed2,
+                          proposal_l=wu.L * f, final_state=True, iter_offset=t)
+            torch.cuda.synchronize()
+            wall += time.perf_counter() - t0
+            kms += r.kernel_ms
+            exact += r.exact_evals
+            s1.append(float(np.mean(r.stage1_rate)))
+            acc.append(float(np.mean(r.accept_rate)))
+            draws.append(r.draws)
+            state = r.final_state
+            t += a.da_iters
+            x = np.concatenate(draws)
+            rh = max(rhat(x[:, :, j]) for j in range(d))
+            print(f"[bench-da] {name}: {t} it, {wall:.1f} s, R-hat {rh:.4f}",
+                  file=sys.stderr, flush=True)
+            if rh < 1.01 or t >= a.da_max_iters:
+                break
+        summ = summarize(x.astype(np.float64))
+        moved = float(np.mean(np.any(np.diff(x, axis=0) != 0, axis=(0, 2))))
+        res[name] = {
+            "iters": t, "scale_factor": f,
+            "wall_s": wall, "ms_per_iter": wall / t * 1e3,
+            "kernel_ms_per_iter": kms / t,
+            "ess_min": summ["min_ess"], "ess_per_s": summ["min_ess"] / wall,
+            "rhat_max": summ["max_rhat"],
+            "accept": float(np.mean(acc)),
+            "stage1_rate": float(np.mean(s1)),
+            "exact_evals": exact,
+            "obs_evals_per_s": exact * N / wall,
+            "chains_that_moved": moved,
+            "params": summ["params"]}
+        print(f"[bench-da] {name}: {res[name]['ms_per_iter']:.2f} ms/it, ESS "
+              f"{summ['min_ess']:.0f}, R-hat {summ['max_rhat']:.4f}",
+              file=sys.stderr, flush=True)
+        del draws, x
+    out = {"workload": f"logistic regression, {d} params, {N} FP32 rows x "
+                       f"{w.n_cols} cols, {C} chains, {wu.used} warmup "
+                       f"iterations, then chunks of {a.da_iters} until split "
+                       f"R-hat < 1.01",
+           "plain_engine": "stepwise",
+           "plain_fused_ms_per_iter": fused_ms,
+           "da_scale_pilots": pilots,
+           "warmup_s": warm_s,
+           "surrogate": dict(fit.summary(),
+                             train_steps=a.da_train_steps,
+                             mfma_tflops=(a.da_train_steps * 1024 * 3 *
+                                          MLP_FLOP_PER_ROW /
+                                          (fit.train_ms / 1e3) / 1e12),
+                             mfma_frac=(a.da_train_steps * 1024 * 3 *
+                                        MLP_FLOP_PER_ROW /
+                                        (fit.train_ms / 1e3) / MFMA_BF16_PEAK)),
+           "fit_s": fit_s}
+    for name in res:
+        params = res[name].pop("params")
+        res[name]["posterior_mean"] = [p["mean"] for p in params]
+        res[name]["posterior_mcse"] = [p["mcse"] for p in params]
+    out.update(res)
+    if "da" in res:
+        p, q = res["plain"], res["da"]
+        out["ess_per_s_ratio"] = q["ess_per_s"] / p["ess_per_s"]
+        out["ess_per_s_ratio_incl_training"] = (
+            q["ess_min"] / (q["wall_s"] + fit_s)) / p["ess_per_s"]
+        z = [abs(x - y) / math.hypot(u, v) for x, y, u, v in zip(
+            p["posterior_mean"], q["posterior_mean"], p["posterior_mcse"],
+            q["posterior_mcse"])]
+        out["max_mean_diff_in_mcse"] = max(z)
+        out["rhat_ok"] = bool(p["rhat_max"] < 1.01 and q["rhat_max"] < 1.01)
+    if fit.net is not None:
+        fit.net.close()
+    dm.close()
+    return out
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -126,10 +277,10 @@ def main():
     state = {"kernel_ms": []}
 
     def call(init_m, sd_m, iters, s, temps=None, mode=0, g=0.0, noise=0.0,
-             L=None):
+             L=None, trace=False):
         r = dm.run(init_m, sd_m, np.ones(C) if temps is None else temps,
                    iters, int(s) & 0xFFFFFFFF, proposal_mode=mode, gamma=g,
-                   de_noise=noise, proposal_l=L, final_state=True)
+                   de_noise=noise, proposal_l=L, final_state=True, trace=trace)
         state["kernel_ms"].append(r.kernel_ms)
         return r
 
@@ -219,10 +370,12 @@ def main():
             "stream_GBps": bytes_per_launch / avg_launch_s / 1e9,
         },
     }
+    dm.close()
+    if not a.no_da:
+        out["surrogate_da"] = da_section(a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     print(json.dumps(out), flush=True)
-    dm.close()
 
 
 if __name__ == "__main__":

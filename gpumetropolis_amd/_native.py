@@ -52,13 +52,16 @@ class RunExt(C.Structure):
 
 class DaExt(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("stream_id", C.c_uint32),
-                ("iter_offset", C.c_uint32), ("n_slices", C.c_uint32),
+                ("iter_offset", C.c_uint32), ("work_items", C.c_uint32),
+                ("work_groups", C.c_uint32),
                 ("final_state", C.POINTER(C.c_double)),
                 ("stage1_rate", C.POINTER(C.c_double)),
                 ("trace_s", C.POINTER(C.c_double)),
                 ("trace_plp", C.POINTER(C.c_double)),
                 ("trace_stage", C.POINTER(C.c_uint8)),
-                ("kernel_ms", C.c_double), ("exact_evals", C.c_double)]
+                ("kernel_ms", C.c_double), ("exact_evals", C.c_double),
+                ("trace_prop", C.POINTER(C.c_double)),
+                ("trace_ll", C.POINTER(C.c_double))]
 
 
 class TrainOpts(C.Structure):
@@ -379,7 +382,7 @@ class DeviceModel:
     def run_da(self, surrogate, init, temperatures, n_iter, seed,
                proposal_sd=None, proposal_l=None, trace=False,
                final_state=False, stream_id=0, iter_offset=0,
-               n_slices=0) -> RunOutput:
+               work_items=0, work_groups=0) -> RunOutput:
         """Stepwise random-walk Metropolis; with `surrogate` (a Surrogate)
         two-stage delayed acceptance, without it plain Metropolis with the
         uniforms of `run`.  Mode 2 when proposal_l is given, else mode 0."""
@@ -393,7 +396,8 @@ class DeviceModel:
         ext.struct_size = C.sizeof(DaExt)
         ext.stream_id = stream_id
         ext.iter_offset = iter_offset
-        ext.n_slices = n_slices
+        ext.work_items = work_items
+        ext.work_groups = work_groups
         keep = {"rate": np.zeros(n_chains, np.float64)}
         ext.stage1_rate = _p(keep["rate"], C.c_double)
         if final_state:
@@ -406,6 +410,10 @@ class DeviceModel:
             ext.trace_s = _p(keep["s"], C.c_double)
             ext.trace_plp = _p(keep["plp"], C.c_double)
             ext.trace_stage = _p(keep["stage"], C.c_uint8)
+            keep["prop"] = np.zeros(n_iter * n_chains * np_, np.float64)
+            keep["ll"] = np.zeros(n_iter * n_chains, np.float64)
+            ext.trace_prop = _p(keep["prop"], C.c_double)
+            ext.trace_ll = _p(keep["ll"], C.c_double)
         res = ChainResult()
         sur = surrogate._h if surrogate is not None else None
         rc = self._L.gpum_hip_model_run_da(
@@ -423,6 +431,8 @@ class DeviceModel:
             out.trace_s = keep["s"].reshape(n_iter, n_chains)
             out.trace_plp = keep["plp"].reshape(n_iter, n_chains)
             out.trace_stage = keep["stage"].reshape(n_iter, n_chains)
+            out.trace_prop = keep["prop"].reshape(n_iter, n_chains, np_)
+            out.trace_ll = keep["ll"].reshape(n_iter, n_chains)
         return out
 
     def loglik(self, points) -> np.ndarray:

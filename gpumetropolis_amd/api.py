@@ -91,7 +91,8 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                    temperatures=None, swap_every=None, gamma=None,
                    de_noise: float = 1e-3, de_every=None, de_sync=False,
                    seed: int = 1, backend: str = "hip",
-                   tiling=None, ladder_size=None, surrogate=None) -> Fit:
+                   tiling=None, ladder_size=None, surrogate=None,
+                   engine: str = "auto") -> Fit:
     if backend not in ("hip", "auto"):
         raise ValueError(f"backend '{backend}' is not available here; this "
                          "package is the hip backend")
@@ -125,7 +126,14 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
 
     dm = DeviceModel(model.loglik.code, model.loglik.consts, model.prior.code,
                      model.prior.consts, np_, X, model.n_cols, n_obs)
-    call = _Caller(dm, n_chains, tiling)
+    if engine not in ("auto", "fused", "stepwise"):
+        raise ValueError(f"unknown engine '{engine}'")
+    if engine == "auto":
+        engine = pick_engine(dm.info(), n_chains, n_obs) if method == "rwm" else "fused"
+    if engine == "stepwise" and method != "rwm":
+        raise ValueError('`engine = "stepwise"` runs method "rwm" only.')
+    call = (_StepCaller(dm, n_chains) if engine == "stepwise"
+            else _Caller(dm, n_chains, tiling))
 
     try:
         if method == "pt":
@@ -156,7 +164,33 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
     finally:
         dm.close()
     fit.kernel_ms = call.kernel_ms
+    fit.adaptation["engine"] = engine
     return fit
+
+
+# Geometry of the fused chain kernel (csrc/chain_kernel.h): one static LDS
+# tile buffer of 4608 float4s; 16 waves per workgroup; 256 CUs.
+_TILE_BYTES = 4608 * 16
+_FULL_WAVES = 16 * 256
+
+
+def pick_engine(info: dict, n_chains: int, n_obs: int) -> str:
+    """Fused chain kernel or stepwise sampler for a random-walk run.
+
+    The fused kernel keeps every chain's whole iteration loop in one
+    workgroup, so it fills the GPU only with enough chains (times the waves
+    each chain's rows allow in one LDS tile).  With fewer, and enough work
+    per iteration to pay for a handful of launches, the stepwise sampler
+    splits the observations over all CUs instead."""
+    need = max(0, int(info["vm_depth"]) - 1)
+    opt = 16 if need <= 3 else (8 if need <= 7 else 4)
+    row_bytes = 64 * opt * max(1, int(info["n_cols_total"])) * 4
+    wpc = 1
+    while wpc < 16 and 2 * wpc * row_bytes <= _TILE_BYTES:
+        wpc *= 2
+    if n_chains * wpc < _FULL_WAVES and n_chains * n_obs >= (1 << 27):
+        return "stepwise"
+    return "fused"
 
 
 class _Caller:
@@ -175,6 +209,29 @@ class _Caller:
                         proposal_mode=mode, gamma=g, de_noise=noise,
                         proposal_l=L, final_state=True, tiling=self.tiling,
                         batches=batches, trace=trace)
+        self.kernel_ms += r.kernel_ms
+        return r
+
+
+class _StepCaller:
+    """The `_Caller` interface on the stepwise sampler (proposal modes 0 and
+    2): one launch sequence per iteration, observations split over every
+    CU."""
+
+    def __init__(self, dm, n_chains):
+        self.dm, self.n_chains = dm, n_chains
+        self.kernel_ms = 0.0
+
+    def __call__(self, init_m, sd_m, iters, s, temps=None, mode=0, g=0.0,
+                 noise=0.0, L=None, iter_offset=0, trace=False):
+        if mode not in (0, 2):
+            raise ValueError("the stepwise engine runs proposal modes 0 and 2")
+        t = np.ones(self.n_chains) if temps is None else temps
+        r = self.dm.run_da(None, init_m, t, iters, int(s) & 0xFFFFFFFF,
+                           proposal_sd=sd_m if mode == 0 else None,
+                           proposal_l=L if mode == 2 else None,
+                           final_state=True, iter_offset=iter_offset,
+                           trace=trace)
         self.kernel_ms += r.kernel_ms
         return r
 
@@ -281,8 +338,9 @@ def _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup, seed,
                   "accept_history": w.accept_history, "final_L": w.L}
     net = None
     if surrogate:
-        from .surrogate import fit_surrogate
+        from .surrogate import fit_surrogate, tune_scale
         opts = dict(surrogate) if isinstance(surrogate, dict) else {}
+        scale = opts.pop("scale", "auto")
         sf = fit_surrogate(w.pairs_theta, w.pairs_loglik, seed=seed, **opts)
         info = sf.summary()
         info["used"] = sf.usable
@@ -290,10 +348,20 @@ def _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup, seed,
             net = sf.net
         adaptation["surrogate"] = info
     if net is not None:
-        r = dm.run_da(net, w.init, np.ones(n_chains), sampling,
-                      int(seed + w.batches_run) & 0xFFFFFFFF,
-                      proposal_sd=None if use_L else w.sd,
-                      proposal_l=w.L if use_L else None, final_state=True)
+        s_seed = int(seed + w.batches_run) & 0xFFFFFFFF
+        L_da = w.L if use_L else None
+        sd_da = None if use_L else w.sd
+        if scale == "auto":
+            factor, pilots = tune_scale(dm, net, w.init, sf.whiten, s_seed,
+                                        proposal_l=L_da, proposal_sd=sd_da)
+            adaptation["surrogate"]["pilots"] = pilots
+        else:
+            factor = float(scale)
+        adaptation["surrogate"]["scale_factor"] = factor
+        r = dm.run_da(net, w.init, np.ones(n_chains), sampling, s_seed,
+                      proposal_sd=None if sd_da is None else sd_da * factor,
+                      proposal_l=None if L_da is None else L_da * factor,
+                      final_state=True)
         call.kernel_ms += r.kernel_ms
         adaptation["surrogate"].update(
             stage1_rate=float(np.mean(r.stage1_rate)),

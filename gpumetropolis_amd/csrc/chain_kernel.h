@@ -106,27 +106,30 @@ constexpr size_t kStreamStaticLds = 2 * (size_t)kTileF4 * 16;
 constexpr uint32_t kTileIssues = (kTileF4 + 1023) / 1024;
 
 // The data term of a list of parameter points, summed per observation
-// slice: grid (slot groups of cpb, n_slices), one wave per slot.
+// slice by a persistent grid of n_workgroups that takes (slot group, slice)
+// items from a queue; one wave per slot.
 struct PointsArgs {
   const double* points;    // FP64, np per point
   const uint32_t* index;   // slot k evaluates point index[k] (null: point k)
   const uint32_t* count;   // device count of active slots (null: n_points)
-  uint32_t n_points;       // slot capacity
+  uint32_t n_points;       // slot capacity (partial row stride)
   uint32_t n_params;
   const uint32_t* hoist;   // FP64 hoist program writing the uniform table
   uint32_t n_hoist;
   const double* consts;
-  uint32_t n_slices;
-  uint32_t tiles_per_slice;
-  double* partial;         // n_slices x n_points
+  uint32_t target_items;   // work items to aim for (slices * slot groups)
+  uint32_t* queue;         // [next item, finished workgroups], zero between launches
+  uint32_t* n_slices_out;  // slices of this launch
+  double* partial;         // slices x n_points; slices <= min(tiles, target_items)
 };
 
 hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
                                int ns, int opt, size_t lds_bytes,
                                hipStream_t s);
-hipError_t launch_points_kernel(const PointsArgs& p, const ChainHot& h,
-                                int ns, int opt, size_t lds_bytes,
-                                hipStream_t s);
+// `p` is a device copy of the arguments.
+hipError_t launch_points_kernel(const PointsArgs* p, uint32_t n_workgroups,
+                                const ChainHot& h, int ns, int opt,
+                                size_t lds_bytes, hipStream_t s);
 hipError_t set_chain_kernel_lds_limit(size_t bytes);
 
 }  // namespace gpum

@@ -478,98 +478,152 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   if (lead) chain_finish(Ap, chain, st);
 }
 
-// Data term of a list of parameter points (no chain state): workgroup
-// (g, s) sums observation slice s for slots g*cpb .. g*cpb+cpb-1, one wave
-// per slot, and writes one FP64 partial per (slice, slot).  The caller adds
-// the slices in order, so every slot's sum is independent of how many
-// slots are active.  Slots at or past *count exit before any work.
+// Data term of a list of parameter points (no chain state).  A persistent
+// grid takes work items from a queue: item (g, s) sums observation slice s
+// for slots g*cpb .. g*cpb+cpb-1, one wave per slot, and writes one FP64
+// partial per (slice, slot); the caller adds the slices in order.  The
+// number of slices is chosen on the device from the active slot count so
+// that about `target_items` items exist whatever the count, and published
+// in *n_slices_out.  The last workgroup to finish resets the queue.
+// The uniform table of slot k's point (lead lane of the slot's wave).
+__device__ __noinline__ void points_hoist(const PointsArgs* __restrict__ P,
+                                          uint32_t k, float* U) {
+  const uint32_t pi = P->index ? P->index[k] : k;
+  eval_params64(P->hoist, P->n_hoist, P->consts,
+                P->points + (size_t)pi * P->n_params, U);
+}
+
+// Item count and slicing of one launch (identical in every workgroup).
+struct PointsPlan {
+  uint32_t count, groups, n_sl, tps, total;
+};
+
+__device__ __noinline__ PointsPlan points_plan(const PointsArgs* __restrict__ P,
+                                               uint32_t cpb, uint32_t n_tiles) {
+  PointsPlan q;
+  q.count = P->count ? *P->count : P->n_points;
+  q.groups = (q.count + cpb - 1) / cpb;
+  uint32_t n_sl = q.groups ? (P->target_items + q.groups - 1) / q.groups : 1u;
+  n_sl = max(1u, min(n_sl, n_tiles));
+  q.tps = (n_tiles + n_sl - 1) / n_sl;
+  q.n_sl = (n_tiles + q.tps - 1) / q.tps;
+  q.total = q.groups * q.n_sl;
+  return q;
+}
+
 template <int NS, int OPT, bool STREAM>
 __global__ void __launch_bounds__(1024)
-points_kernel(PointsArgs P, ChainHot H) {
+points_kernel(const PointsArgs* __restrict__ P, ChainHot H) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t count = P.count ? *P.count : P.n_points;
-  const uint32_t cpb = H.cpb;
-  const uint32_t k0 = blockIdx.x * cpb;
-  if (k0 >= count) return;  // uniform over the workgroup
-  const int NU = (int)H.n_uniform_pad;
-  float* Uall = reinterpret_cast<float*>(smem);
+  __shared__ uint32_t item_sh;
+  __shared__ PointsPlan plan_sh;
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t k = k0 + (uint32_t)wave;
-  const bool active = k < count;
-  float* U = Uall + wave * NU;
-  for (int i = tid; i < (int)cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
-  __syncthreads();
-  if (active && lane == 0) {
-    const uint32_t pi = P.index ? P.index[k] : k;
-    eval_params64(P.hoist, P.n_hoist, P.consts,
-                  P.points + (size_t)pi * P.n_params, U);
-  }
   const uint32_t tile_len = H.tile_len;
-  const uint32_t n4 = tile_len / 4;
-  const uint32_t tile4 = H.n_cols * n4;
-  const uint32_t ld4 = H.ld / 4;
   const uint32_t n_tiles = H.n_pad / tile_len;
-  const uint32_t ti0 = min(blockIdx.y * P.tiles_per_slice, n_tiles);
-  const uint32_t ti1 = min(ti0 + P.tiles_per_slice, n_tiles);
-  const float4* src = reinterpret_cast<const float4*>(H.data);
-  const uint32_t wave_base = (uint32_t)(wave * 64);
-  double acc = 0.0;
-  if (H.n_vm > 0) {
-    if constexpr (!STREAM) {
-      float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
-      for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-        uint32_t col = i / n4, off = i - col * n4;
-        res[i] = src[(size_t)col * ld4 + off];
-      }
-      __syncthreads();
-      if (active && ti0 < ti1)
-        acc = tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(res), 0, 0,
-                                 1, U, lane);
-    } else if (H.stream_mode == 1) {
-      if (ti0 < ti1)
-        issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti0, g_tile_a);
-      __syncthreads();
-      for (uint32_t ti = ti0; ti < ti1; ti += 2) {
-        const bool two = ti + 1 < ti1;
-        if (two) issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 1, g_tile_b);
-        if (active)
-          acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
-                                    ti * tile_len, 0, 1, U, lane);
-        __syncthreads();
-        if (ti + 2 < ti1)
-          issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 2, g_tile_a);
-        if (two && active)
-          acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_b),
-                                    (ti + 1) * tile_len, 0, 1, U, lane);
-        __syncthreads();
-      }
-    } else {
-      for (uint32_t ti = ti0; ti < ti1; ++ti) {
-        __syncthreads();
-        for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-          uint32_t col = i / n4, off = i - col * n4;
-          g_tile_a[i] = src[(size_t)col * ld4 + (size_t)ti * n4 + off];
+  if (tid == 0) {
+    plan_sh = points_plan(P, H.cpb, n_tiles);
+    if (blockIdx.x == 0) *P->n_slices_out = plan_sh.n_sl;
+  }
+  const int NU = (int)H.n_uniform_pad;
+  float* Uall = reinterpret_cast<float*>(smem);
+  float* U = Uall + wave * NU;
+  bool loaded = false;  // resident layout: data loaded with the first item
+  for (;;) {
+    __syncthreads();  // previous item's reads of U and the tiles are done
+    if (tid == 0) item_sh = atomicAdd(&P->queue[0], 1u);
+    __syncthreads();
+    const uint32_t item = item_sh;
+    if (item >= plan_sh.total) break;  // uniform: every wave leaves here
+    // Slice-major order: the items in flight together read few slices, so
+    // the slot groups of one slice share its tiles in the caches.
+    const uint32_t groups = plan_sh.groups, tps = plan_sh.tps;
+    const uint32_t sl = item / groups, g = item - sl * groups;
+    const uint32_t k = g * H.cpb + (uint32_t)wave;
+    const bool active = k < plan_sh.count;
+    for (int i = tid; i < (int)H.cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
+    __syncthreads();
+    if (active && lane == 0) points_hoist(P, k, U);
+    const uint32_t ti0 = min(sl * tps, n_tiles);
+    const uint32_t ti1 = min(ti0 + tps, n_tiles);
+    const uint32_t n4 = tile_len / 4;
+    const uint32_t tile4 = H.n_cols * n4;
+    const uint32_t ld4 = H.ld / 4;
+    const float4* src = reinterpret_cast<const float4*>(H.data);
+    const uint32_t wave_base = (uint32_t)(wave * 64);
+    double acc = 0.0;
+    if (H.n_vm > 0) {
+      if constexpr (!STREAM) {
+        // One tile holds the whole data set; load it once per workgroup.
+        if (!loaded) {
+          float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
+          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
+            uint32_t col = i / n4, off = i - col * n4;
+            res[i] = src[(size_t)col * ld4 + off];
+          }
+          loaded = true;
         }
         __syncthreads();
-        if (active)
-          acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
-                                    ti * tile_len, 0, 1, U, lane);
+        if (active && ti0 < ti1)
+          acc = tile_pass<NS, OPT>(
+              H, reinterpret_cast<const float*>(smem + H.lds_tile_offset), 0,
+              0, 1, U, lane);
+      } else if (H.stream_mode == 1) {
+        if (ti0 < ti1)
+          issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti0, g_tile_a);
+        __syncthreads();
+        for (uint32_t ti = ti0; ti < ti1; ti += 2) {
+          const bool two = ti + 1 < ti1;
+          if (two)
+            issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 1, g_tile_b);
+          if (active)
+            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
+                                      ti * tile_len, 0, 1, U, lane);
+          __syncthreads();
+          if (ti + 2 < ti1)
+            issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 2, g_tile_a);
+          if (two && active)
+            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_b),
+                                      (ti + 1) * tile_len, 0, 1, U, lane);
+          __syncthreads();
+        }
+      } else {
+        for (uint32_t ti = ti0; ti < ti1; ++ti) {
+          __syncthreads();
+          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
+            uint32_t col = i / n4, off = i - col * n4;
+            g_tile_a[i] = src[(size_t)col * ld4 + (size_t)ti * n4 + off];
+          }
+          __syncthreads();
+          if (active)
+            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
+                                      ti * tile_len, 0, 1, U, lane);
+        }
       }
     }
-  }
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if (active && lane == 0) P.partial[(size_t)blockIdx.y * P.n_points + k] = acc;
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (active && lane == 0) P->partial[(size_t)sl * P->n_points + k] = acc;
+  }
+  // The last workgroup out leaves the queue at zero for the next launch.
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(&P->queue[1], 1u) == gridDim.x - 1) {
+      P->queue[0] = 0u;
+      P->queue[1] = 0u;
+      __threadfence();
+    }
+  }
 }
 
 #define GPUM_INST(NS, OPT)                                                   \
   template __global__ void chain_kernel<NS, OPT, false>(const ChainArgs*,   \
                                                         ChainHot);          \
   template __global__ void chain_kernel<NS, OPT, true>(const ChainArgs*, ChainHot); \
-  template __global__ void points_kernel<NS, OPT, false>(PointsArgs, ChainHot); \
-  template __global__ void points_kernel<NS, OPT, true>(PointsArgs, ChainHot);
+  template __global__ void points_kernel<NS, OPT, false>(const PointsArgs*,  \
+                                                         ChainHot);         \
+  template __global__ void points_kernel<NS, OPT, true>(const PointsArgs*, ChainHot);
 GPUM_INST(0, 16)
 GPUM_INST(1, 16)
 GPUM_INST(3, 16)
@@ -604,10 +658,10 @@ hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_points_kernel(const PointsArgs& p, const ChainHot& h,
-                                int ns, int opt, size_t lds_bytes,
-                                hipStream_t s) {
-  dim3 grid((p.n_points + h.cpb - 1) / h.cpb, p.n_slices);
+hipError_t launch_points_kernel(const PointsArgs* p, uint32_t n_workgroups,
+                                const ChainHot& h, int ns, int opt,
+                                size_t lds_bytes, hipStream_t s) {
+  dim3 grid(n_workgroups);
   dim3 block(h.cpb * 64);
   const bool stream = h.stream_mode != 0;
 #define GPUM_CASE(NS_, OPT_)                                                 \

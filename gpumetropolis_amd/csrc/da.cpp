@@ -466,27 +466,25 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   if (!pick_tiling(m, C, nullptr, &t, true)) return 1;
   if (!ensure_data(m, t.n_pad)) return 1;
   const uint32_t n_tiles = t.n_pad / t.tile_len;
-  uint32_t S_sl = X.n_slices;
-  if (S_sl == 0) {
-    // Enough workgroups for the expected number of active slot groups to
-    // cover every CU twice.
-    const double pass = S ? 0.3 : 1.0;
-    const uint32_t groups =
-        std::max(1u, (uint32_t)ceil(pass * (double)C / (double)t.cpb));
-    S_sl = (2u * (uint32_t)g_dev.n_cu + groups - 1) / groups;
-  }
-  S_sl = std::max(1u, std::min(S_sl, n_tiles));
-  const uint32_t tps = (n_tiles + S_sl - 1) / S_sl;
-  S_sl = (n_tiles + tps - 1) / tps;
+  // Exact-stage work items per launch: every CU twice by default.
+  const uint32_t items = X.work_items ? X.work_items : 2u * (uint32_t)g_dev.n_cu;
+  const uint32_t max_slices = std::max(1u, std::min(n_tiles, items));
+  const uint32_t groups_cap = (C + t.cpb - 1) / t.cpb;
+  const uint32_t n_wg =
+      X.work_groups
+          ? X.work_groups
+          : std::max(1u, (uint32_t)std::min<uint64_t>(
+                             (uint64_t)groups_cap * n_tiles,
+                             2u * (uint32_t)g_dev.n_cu));
 
   const size_t ncp = (size_t)C * np;
   const size_t n_draws = (size_t)n_iter * ncp;
   DevBuf<double> theta, prop, cur, cur_s, pprior, prop_s, acc, passes, partial,
       temps, sd, L;
   DevBuf<int32_t> pos;
-  DevBuf<uint32_t> list, count;
+  DevBuf<uint32_t> list, count, queue, n_sl;
   DevBuf<float> draws;
-  DevBuf<double> tr_s, tr_plp;
+  DevBuf<double> tr_s, tr_plp, tr_prop, tr_ll;
   DevBuf<uint8_t> tr_stage;
   DevBuf<DaArgs> d_args;
   bool ok = theta.upload(init, ncp, "upload init") &&
@@ -494,7 +492,8 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
             cur_s.alloc(C, "allocate state") && pprior.alloc(C, "allocate state") &&
             prop_s.alloc(C, "allocate state") && acc.alloc(C, "allocate state") &&
             passes.alloc(C, "allocate state") &&
-            partial.alloc((size_t)S_sl * C, "allocate partials") &&
+            partial.alloc((size_t)max_slices * C, "allocate partials") &&
+            queue.alloc(2, "allocate state") && n_sl.alloc(1, "allocate state") &&
             temps.upload(temperatures, C, "upload temperatures") &&
             pos.alloc(C, "allocate state") && list.alloc(C, "allocate state") &&
             count.alloc(1, "allocate state") &&
@@ -505,8 +504,14 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   if (ok && X.trace_s) ok = tr_s.alloc((size_t)n_iter * C, "allocate trace");
   if (ok && X.trace_plp) ok = tr_plp.alloc((size_t)n_iter * C, "allocate trace");
   if (ok && X.trace_stage) ok = tr_stage.alloc((size_t)n_iter * C, "allocate trace");
+  if (ok && X.trace_prop) ok = tr_prop.alloc(n_draws, "allocate trace");
+  if (ok && X.trace_ll) ok = tr_ll.alloc((size_t)n_iter * C, "allocate trace");
   if (ok && S) ok = sur_workspace(S, round_up(C, 64));
   if (!ok) return 1;
+  if (hipMemsetAsync(queue.p, 0, 2 * sizeof(uint32_t), g_dev.stream) != hipSuccess) {
+    set_err("clear work queue: device error");
+    return 1;
+  }
   if (X.trace_s && hipMemsetAsync(tr_s.p, 0xFF, (size_t)n_iter * C * sizeof(double),
                                   g_dev.stream) != hipSuccess) {
     set_err("clear trace: device error");
@@ -540,7 +545,7 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   a.list = list.p;
   a.count = count.p;
   a.partial = partial.p;
-  a.n_slices = S_sl;
+  a.n_slices = n_sl.p;
   a.use_surrogate = S ? 1u : 0u;
   if (S) {
     a.n_in = S->n_in;
@@ -555,6 +560,8 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   a.trace_s = X.trace_s ? tr_s.p : nullptr;
   a.trace_plp = X.trace_plp ? tr_plp.p : nullptr;
   a.trace_stage = X.trace_stage ? tr_stage.p : nullptr;
+  a.trace_prop = X.trace_prop ? tr_prop.p : nullptr;
+  a.trace_ll = X.trace_ll ? tr_ll.p : nullptr;
   if (!d_args.upload(&a, 1, "upload launch arguments")) return 1;
 
   ChainHot h;
@@ -581,8 +588,9 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   pa.hoist = m->hoist.p;
   pa.n_hoist = (uint32_t)m->L.hoist.size();
   pa.consts = m->consts.p;
-  pa.n_slices = S_sl;
-  pa.tiles_per_slice = tps;
+  pa.target_items = items;
+  pa.queue = queue.p;
+  pa.n_slices_out = n_sl.p;
   pa.partial = partial.p;
 
   hipStream_t st = g_dev.stream;
@@ -594,18 +602,23 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   // Initial states: exact log-posterior and surrogate value.
   chk(launch_da_init(d_args.p, C, st));
   if (S && e == hipSuccess && !sur_forward(S, rows16, false, st)) return 1;
-  pa.points = theta.p;
-  pa.index = nullptr;
-  pa.count = nullptr;
-  chk(launch_points_kernel(pa, h, t.v.ns, t.v.opt, t.lds_bytes, st));
+  // Device copies of the points-kernel arguments: the initial states
+  // (every slot), then the compacted proposals of each iteration.
+  PointsArgs pinit = pa;
+  pinit.points = theta.p;
+  pa.points = prop.p;
+  pa.index = list.p;
+  pa.count = count.p;
+  DevBuf<PointsArgs> d_pinit, d_pa;
+  if (!d_pinit.upload(&pinit, 1, "upload launch arguments") ||
+      !d_pa.upload(&pa, 1, "upload launch arguments"))
+    return 1;
+  chk(launch_points_kernel(d_pinit.p, n_wg, h, t.v.ns, t.v.opt, t.lds_bytes, st));
   chk(launch_da_init_finish(d_args.p, C, st));
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   hipEventRecord(e0, st);
-  pa.points = prop.p;
-  pa.index = list.p;
-  pa.count = count.p;
   for (uint32_t it = 0; it < n_iter && e == hipSuccess; ++it) {
     chk(launch_da_propose(d_args.p, C, it, st));
     if (S && e == hipSuccess && !sur_forward(S, rows16, false, st)) {
@@ -614,7 +627,7 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
       return 1;
     }
     chk(launch_da_stage1(d_args.p, it, st));
-    chk(launch_points_kernel(pa, h, t.v.ns, t.v.opt, t.lds_bytes, st));
+    chk(launch_points_kernel(d_pa.p, n_wg, h, t.v.ns, t.v.opt, t.lds_bytes, st));
     chk(launch_da_stage2(d_args.p, C, it, st));
   }
   hipEventRecord(e1, st);
@@ -658,6 +671,12 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
                    hipMemcpyDeviceToHost) == hipSuccess;
   if (ok && X.trace_stage)
     ok = hipMemcpy(X.trace_stage, tr_stage.p, (size_t)n_iter * C,
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && X.trace_prop)
+    ok = hipMemcpy(X.trace_prop, tr_prop.p, n_draws * sizeof(double),
+                   hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok && X.trace_ll)
+    ok = hipMemcpy(X.trace_ll, tr_ll.p, (size_t)n_iter * C * sizeof(double),
                    hipMemcpyDeviceToHost) == hipSuccess;
   if (!ok) {
     gpum_hip_result_free(out);

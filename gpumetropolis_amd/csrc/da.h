@@ -34,8 +34,8 @@ struct DaArgs {
   int32_t* pos;        // slot of the chain's proposal, or -1
   uint32_t* list;      // slot -> chain
   uint32_t* count;     // active slots
-  const double* partial;  // n_slices x n_chains
-  uint32_t n_slices;
+  const double* partial;     // slices x n_chains
+  const uint32_t* n_slices;  // slices of the last exact-stage launch
   // surrogate first stage
   uint32_t use_surrogate;
   uint32_t n_in;
@@ -49,6 +49,8 @@ struct DaArgs {
   double* trace_s;       // n_iter x C surrogate log-posterior of prop
   double* trace_plp;     // n_iter x C exact log-posterior (NaN: not run)
   uint8_t* trace_stage;  // n_iter x C: 0 stage-1 reject, 1 stage-2 reject, 2 accept
+  double* trace_prop;    // n_iter x C x np proposals
+  double* trace_ll;      // n_iter x C data term of the proposal (NaN: not run)
 };
 
 hipError_t launch_da_init(const DaArgs* A, uint32_t n_chains, hipStream_t s);

@@ -70,7 +70,8 @@ __global__ void da_init_finish_kernel(const DaArgs* __restrict__ A) {
   const uint32_t C = A->n_chains;
   if (c >= C) return;
   double ll = 0.0;
-  for (uint32_t s = 0; s < A->n_slices; ++s) ll += A->partial[(size_t)s * C + c];
+  const uint32_t ns = *A->n_slices;
+  for (uint32_t s = 0; s < ns; ++s) ll += A->partial[(size_t)s * C + c];
   const double pr = A->prop_prior[c];
   A->cur[c] = ll + pr;
   A->cur_s[c] = A->use_surrogate
@@ -169,9 +170,11 @@ __global__ void da_stage2_kernel(const DaArgs* __restrict__ A, uint32_t t) {
   const int32_t p = A->pos[c];
   bool acc_ok = false;
   double lpp = __builtin_nan("");
+  double ll = __builtin_nan("");
   if (p >= 0) {
-    double ll = 0.0;
-    for (uint32_t s = 0; s < A->n_slices; ++s) ll += A->partial[(size_t)s * C + p];
+    ll = 0.0;
+    const uint32_t ns = *A->n_slices;
+    for (uint32_t s = 0; s < ns; ++s) ll += A->partial[(size_t)s * C + p];
     lpp = ll + A->prop_prior[c];
     double la = lpp - A->cur[c];
     if (A->use_surrogate) la -= A->prop_s[c] - A->cur_s[c];
@@ -188,6 +191,10 @@ __global__ void da_stage2_kernel(const DaArgs* __restrict__ A, uint32_t t) {
     A->accepts[c] += 1.0;
   }
   if (A->trace_plp) A->trace_plp[(size_t)t * C + c] = lpp;
+  if (A->trace_ll) A->trace_ll[(size_t)t * C + c] = ll;
+  if (A->trace_prop)
+    for (uint32_t j = 0; j < np; ++j)
+      A->trace_prop[((size_t)t * C + c) * np + j] = A->prop[(size_t)c * np + j];
   if (A->trace_stage) A->trace_stage[(size_t)t * C + c] = p < 0 ? 0 : (acc_ok ? 2 : 1);
   const size_t T = A->n_iter;
   for (uint32_t j = 0; j < np; ++j) A->draws[t + T * (c + (size_t)C * j)] = (float)th[j];

@@ -265,7 +265,8 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   t->wpc = wpc;
   const uint32_t ncol = std::max(1u, m->n_cols_total);
   auto head_bytes = [&](uint32_t c, uint32_t w) {
-    if (points) return ((size_t)c * m->n_uniform_pad * 4 + 15) / 16 * 16;
+    // points: uniform tables, and 16 bytes for the kernel's static item slot
+    if (points) return ((size_t)c * m->n_uniform_pad * 4 + 15) / 16 * 16 + 16;
     const size_t state_bytes = (size_t)c * (6 * m->n_params + 6) * 8;
     const size_t part_bytes = (size_t)c * w * 8;
     const size_t u_bytes = (size_t)c * m->n_uniform_pad * 4;

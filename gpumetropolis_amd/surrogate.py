@@ -34,6 +34,8 @@ DEFAULT_BOUND = 1.0
 @dataclass
 class SurrogateFit:
     net: Optional[_native.Surrogate]
+    mean: np.ndarray           # input whitening: x = whiten (theta - mean)
+    whiten: np.ndarray
     usable: bool
     holdout_err: float         # sd of (prediction - loglik) on held-out pairs
     holdout_bias: float
@@ -72,7 +74,9 @@ def fit_surrogate(theta, loglik, bound: float = DEFAULT_BOUND,
     theta, loglik = theta[ok], loglik[ok]
     n = loglik.size
     if n < 64:
-        return SurrogateFit(None, False, np.inf, np.nan, np.inf, bound, 0, 0, 0.0)
+        d = theta.shape[1] if theta.ndim == 2 else 1
+        return SurrogateFit(None, np.zeros(d), np.eye(d), False, np.inf,
+                            np.nan, np.inf, bound, 0, 0, 0.0)
     rng = np.random.default_rng(seed)
     perm = rng.permutation(n)[:max_pairs]
     n_hold = max(1, int(round(holdout * perm.size)))
@@ -90,7 +94,46 @@ def fit_surrogate(theta, loglik, bound: float = DEFAULT_BOUND,
     k = min(train.size, 20000)
     err_t = net.predict(th_tr[:k]) - ll_tr[:k]
     herr = float(np.std(err_h))
-    fit = SurrogateFit(net, bool(herr <= bound), herr, float(np.mean(err_h)),
+    fit = SurrogateFit(net, mean, W, bool(herr <= bound), herr,
+                       float(np.mean(err_h)),
                        float(np.std(err_t)), bound, int(train.size),
                        int(n_hold), ms, trace)
     return fit
+
+
+# Proposal scale factors tried by the pilot (multiples of the warmup's
+# Metropolis-tuned proposal).  Delayed acceptance moves the optimum up: a
+# proposal the surrogate rejects costs no exact evaluation, so longer steps
+# buy more distance per exact evaluation until the chains grow sticky
+# (Sherlock, Thiery and Golightly, "Efficiency of delayed-acceptance random
+# walk Metropolis algorithms", Ann. Statist. 49(5), 2021).
+PILOT_FACTORS = (1.0, 1.3, 1.6, 2.0)
+# First stream iteration of the pilots: far from the sampling run's.
+PILOT_ITER_OFFSET = 1 << 30
+
+
+def tune_scale(dm, net, init, whiten, seed, proposal_l=None, proposal_sd=None,
+               factors=PILOT_FACTORS, iters=60, keep=0.95):
+    """Scale factor of the delayed-acceptance proposal: short pilot runs from
+    `init` at each factor, scored by the expected squared jump distance
+    (whitened) per millisecond of device time; the smallest factor within
+    `keep` of the best wins (the larger ones only add stickiness)."""
+    n_chains, d = np.asarray(init).reshape(-1, net.n_in).shape
+    W = np.asarray(whiten, dtype=np.float64).reshape(d, d)
+    rows = []
+    for k, f in enumerate(factors):
+        r = dm.run_da(net, init, np.ones(n_chains), iters, seed,
+                      proposal_l=None if proposal_l is None else np.asarray(proposal_l) * f,
+                      proposal_sd=None if proposal_sd is None else np.asarray(proposal_sd) * f,
+                      iter_offset=PILOT_ITER_OFFSET + k * iters)
+        x = r.draws.astype(np.float64)
+        z = np.diff(x, axis=0) @ W.T
+        esjd = float(np.mean(np.sum(z * z, axis=2)))
+        ms = r.kernel_ms / iters
+        rows.append({"factor": f, "esjd": esjd, "ms_per_iter": ms,
+                     "esjd_per_ms": esjd / ms,
+                     "stage1_rate": float(np.mean(r.stage1_rate)),
+                     "accept": float(np.mean(r.accept_rate))})
+    best = max(r["esjd_per_ms"] for r in rows)
+    pick = next(r for r in rows if r["esjd_per_ms"] >= keep * best)
+    return pick["factor"], rows

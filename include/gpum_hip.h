@@ -249,7 +249,8 @@ typedef struct gpum_da_ext {
   uint32_t struct_size;
   uint32_t stream_id;
   uint32_t iter_offset;
-  uint32_t n_slices;        /* observation slices of the exact stage (0 auto) */
+  uint32_t work_items;      /* exact-stage work items per launch (0 auto) */
+  uint32_t work_groups;     /* exact-stage persistent grid size (0 auto) */
   double* final_state;      /* out: n_chains * n_params */
   double* stage1_rate;      /* out: n_chains fraction reaching the exact stage */
   double* trace_s;          /* out: n_iter * n_chains surrogate log-posterior */
@@ -257,6 +258,8 @@ typedef struct gpum_da_ext {
   uint8_t* trace_stage;     /* out: 0 stage-1 reject, 1 stage-2 reject, 2 accept */
   double kernel_ms;         /* out: device time of the iterations */
   double exact_evals;       /* out: proposals evaluated exactly */
+  double* trace_prop;       /* out: n_iter * n_chains * n_params proposals */
+  double* trace_ll;         /* out: n_iter * n_chains data term (NaN: not run) */
 } gpum_da_ext;
 
 /* Random-walk Metropolis (proposal_mode 0 or 2) run one iteration per

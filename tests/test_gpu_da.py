@@ -97,7 +97,7 @@ def test_mlp_training_fits_a_quadratic(hip):
     fit.net.close()
 
 
-def _replay_plain(hip, wl, mode, C=8, T=80, seed=21, n_slices=0):
+def _replay_plain(hip, wl, mode, C=8, T=80, seed=21, work_items=0):
     from oracle import da
     m = wl.model
     np_ = m.n_params
@@ -115,7 +115,7 @@ def _replay_plain(hip, wl, mode, C=8, T=80, seed=21, n_slices=0):
     r = dm.run_da(None, init, np.ones(C), T, seed,
                   proposal_sd=sd if mode == 0 else None,
                   proposal_l=None if L is None else L.ravel(), trace=True,
-                  n_slices=n_slices)
+                  work_items=work_items)
     rep = da.replay_da(r, m, wl.data, wl.n_cols, wl.n_obs, init, np.ones(C),
                        seed, proposal_sd=sd if mode == 0 else None,
                        proposal_l=L, surrogate=False)
@@ -123,20 +123,26 @@ def _replay_plain(hip, wl, mode, C=8, T=80, seed=21, n_slices=0):
     return r, rep
 
 
-@pytest.mark.parametrize("name,mode,n_slices", [("gamma", 0, 0),
-                                                ("gamma", 2, 3),
-                                                ("logistic", 2, 0)])
-def test_stepwise_plain_replays_oracle(hip, name, mode, n_slices):
-    print(f"[stepwise] {name} mode {mode} slices {n_slices}", flush=True)
+@pytest.mark.parametrize("name,mode,work_items", [("gamma", 0, 0),
+                                                  ("gamma", 2, 3),
+                                                  ("logistic", 2, 0)])
+def test_stepwise_plain_replays_oracle(hip, name, mode, work_items):
+    print(f"[stepwise] {name} mode {mode} items {work_items}", flush=True)
     if name == "gamma":
         wl = workloads.gamma(n=30011)
     else:
         wl = workloads.logistic(n=20000, d=4, seed=3)
-    r, rep = _replay_plain(hip, wl, mode, n_slices=n_slices)
+    r, rep = _replay_plain(hip, wl, mode, work_items=work_items)
     print(rep)
     assert rep.max_plp_rel <= 1e-6
     assert rep.uncertified_flips == 0
     assert rep.exact_checked >= 0.9 * r.trace_stage.size
+    # the traced proposals and data terms are what the replay evaluated
+    from oracle import oracle as orc
+    m = wl.model
+    ll = orc.loglik_batch(m.loglik.code, m.loglik.consts, m.n_params, wl.data,
+                          wl.n_cols, wl.n_obs, r.trace_prop[:3].reshape(-1, m.n_params))
+    assert np.allclose(r.trace_ll[:3].ravel(), ll, rtol=1e-6, atol=0)
     assert np.all(r.stage1_rate == 1.0)
     assert 0.05 < r.accept_rate.mean() < 0.95
 
