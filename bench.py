@@ -56,6 +56,10 @@ def parse():
                    help="iterations per chunk of the R-hat-gated runs")
     p.add_argument("--da-max-iters", type=int, default=16000)
     p.add_argument("--da-train-steps", type=int, default=4000)
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the many-model (512 regressions) section")
+    p.add_argument("--batch-models", type=int, default=512)
+    p.add_argument("--batch-iters", type=int, default=4000)
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device-to-device copy of MIB MiB "
                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -246,6 +250,52 @@ def da_section(a):
     return out
 
 
+def batch_section(a):
+    """512 independent linear regressions of log-uniform size in [1e3, 1e6]
+    rows (x1..x8, FP32), DE-MC with 64 chains each, every batch of every
+    model in one launch and all batches one HIP graph; tiles autotuned."""
+    from gpumetropolis_amd import workloads
+    from gpumetropolis_amd.batch import gpu_metropolis_batch
+    from gpumetropolis_amd.diagnostics import summarize
+
+    t0 = time.perf_counter()
+    wls = workloads.regressions(n_models=a.batch_models)
+    gen_s = time.perf_counter() - t0
+    n_tot = sum(w.n_obs for w in wls)
+    rng = np.random.default_rng(a.seed)
+    post = [workloads.regression_posterior(w.data) for w in wls[:8]]
+    # chains start around the truth (posterior sds are 1e-3 .. 3e-2)
+    inits = [w.meta["true"] + 0.01 * rng.standard_normal((64, 10)) for w in wls]
+    t0 = time.perf_counter()
+    bf = gpu_metropolis_batch([w.model for w in wls], [w.data for w in wls],
+                              n_chains=64, n_iter=a.batch_iters,
+                              init=inits, proposal_sd=0.003, seed=a.seed,
+                              tiling="auto")
+    wall = time.perf_counter() - t0
+    evals = n_tot * 64 * a.batch_iters
+    # closed-form check on the first models
+    worst = 0.0
+    for fit, p in zip(bf.fits[:8], post):
+        s = summarize(fit.draws)
+        for j, row in enumerate(s["params"]):
+            worst = max(worst, abs(row["mean"] - p["mean"][j]) / row["mcse"])
+    info = dict(bf.fits[0].adaptation)
+    return {"workload": f"{len(wls)} linear regressions, {n_tot} FP32 rows x "
+                        f"9 cols, DE-MC 64 chains each, {a.batch_iters} "
+                        f"iterations ({bf.batch_sizes[0]} per batch)",
+            "obs_evals_per_s": evals / (bf.kernel_ms / 1e3),
+            "graph_ms": bf.kernel_ms, "wall_s_incl_setup": wall,
+            "data_gen_s": gen_s,
+            "tiling": list(bf.tiling), "tile_key": bf.tile_key,
+            "tile_timings_ms": {f"{k[0]}x{k[1]}": v
+                                for k, v in bf.tile_timings.items()},
+            "workgroups_per_launch": bf.n_workgroups,
+            "launches": 2 * info["n_batches"],
+            "closed_form_max_mean_err_mcse_first8": worst,
+            "mean_accept": float(np.mean([f.accept_rate.mean()
+                                          for f in bf.fits]))}
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -373,6 +423,8 @@ def main():
     dm.close()
     if not a.no_da:
         out["surrogate_da"] = da_section(a)
+    if not a.no_batch:
+        out["batched_models"] = batch_section(a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     print(json.dumps(out), flush=True)

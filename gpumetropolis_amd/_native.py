@@ -64,6 +64,28 @@ class DaExt(C.Structure):
                 ("trace_ll", C.POINTER(C.c_double))]
 
 
+class BatchModel(C.Structure):
+    _fields_ = [("model", C.c_void_p), ("n_chains", C.c_uint32),
+                ("init", C.POINTER(C.c_double)),
+                ("proposal_sd", C.POINTER(C.c_double)),
+                ("seed", C.c_uint32), ("stream_id", C.c_uint32),
+                ("gamma", C.c_double), ("de_noise", C.c_double),
+                ("draws", C.POINTER(C.c_float)),
+                ("accept_hist", C.POINTER(C.c_float)),
+                ("final_sd", C.POINTER(C.c_double)),
+                ("last_log_post", C.POINTER(C.c_double)),
+                ("final_state", C.POINTER(C.c_double))]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("proposal_mode", C.c_uint32),
+                ("n_iter", C.c_uint32), ("batch_len", C.c_uint32),
+                ("warmup_batches", C.c_uint32), ("adapt", C.c_int32),
+                ("chains_per_block", C.c_int32),
+                ("waves_per_chain", C.c_int32), ("use_graph", C.c_int32),
+                ("kernel_ms", C.c_double), ("n_workgroups", C.c_uint32)]
+
+
 class TrainOpts(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("steps", C.c_uint32),
                 ("batch", C.c_uint32), ("seed", C.c_uint32),
@@ -157,6 +179,9 @@ def lib():
                                         C.POINTER(DaExt),
                                         C.POINTER(ChainResult)]
     L.gpum_hip_model_run_da.restype = C.c_int
+    L.gpum_hip_batch_run.argtypes = [C.POINTER(BatchModel), C.c_uint32,
+                                     C.POINTER(BatchOpts)]
+    L.gpum_hip_batch_run.restype = C.c_int
     _lib = L
     return L
 
@@ -345,6 +370,8 @@ class DeviceModel:
         consts, prior_consts = _f64(consts), _f64(prior_consts)
         data = _f64(data)
         self.n_params = int(n_params)
+        self.n_obs = int(n_obs)
+        self.n_cols = int(n_cols)
         h = L.gpum_hip_model_create(
             _p(code, C.c_uint32), code.size, _p(consts, C.c_double),
             consts.size, _p(prior_code, C.c_uint32), prior_code.size,
@@ -527,3 +554,65 @@ class Surrogate:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class BatchOutput:
+    draws: np.ndarray          # (n_iter, n_chains, n_params), every iteration
+    accept_hist: np.ndarray    # (n_batches, n_chains)
+    final_sd: np.ndarray
+    last_log_post: np.ndarray
+    final_state: np.ndarray
+
+
+def batch_run(models, n_chains, inits, sds, seeds, stream_ids, gammas,
+              de_noises, n_iter, batch_len, warmup_batches, adapt,
+              proposal_mode=1, tiling=None, use_graph=True):
+    """gpum_hip_batch_run over DeviceModels; returns (outputs, info)."""
+    L = lib()
+    k = len(models)
+    arr = (BatchModel * k)()
+    keep = []
+    nb = max(1, n_iter // batch_len)
+    outs = []
+    for i, dm in enumerate(models):
+        C_, np_ = int(n_chains[i]), dm.n_params
+        init, sd = _f64(inits[i]), _f64(sds[i])
+        if init.size != C_ * np_ or sd.size != C_ * np_:
+            raise ValueError(f"model {i}: init and sd must be n_chains x n_params")
+        o = BatchOutput(np.zeros(n_iter * C_ * np_, np.float32),
+                        np.zeros(nb * C_, np.float32), np.zeros(C_ * np_),
+                        np.zeros(C_), np.zeros(C_ * np_))
+        keep += [init, sd]
+        b = arr[i]
+        b.model = dm._h
+        b.n_chains = C_
+        b.init, b.proposal_sd = _p(init, C.c_double), _p(sd, C.c_double)
+        b.seed = int(seeds[i]) & 0xFFFFFFFF
+        b.stream_id = int(stream_ids[i])
+        b.gamma, b.de_noise = float(gammas[i]), float(de_noises[i])
+        b.draws = _p(o.draws, C.c_float)
+        b.accept_hist = _p(o.accept_hist, C.c_float)
+        b.final_sd = _p(o.final_sd, C.c_double)
+        b.last_log_post = _p(o.last_log_post, C.c_double)
+        b.final_state = _p(o.final_state, C.c_double)
+        outs.append(o)
+    opts = BatchOpts()
+    opts.struct_size = C.sizeof(BatchOpts)
+    opts.proposal_mode = proposal_mode
+    opts.n_iter = n_iter
+    opts.batch_len = batch_len
+    opts.warmup_batches = warmup_batches
+    opts.adapt = 1 if adapt else 0
+    if tiling:
+        opts.chains_per_block, opts.waves_per_chain = tiling
+    opts.use_graph = 1 if use_graph else 0
+    _check(L.gpum_hip_batch_run(arr, k, C.byref(opts)))
+    for i, o in enumerate(outs):
+        C_, np_ = int(n_chains[i]), models[i].n_params
+        o.draws = o.draws.reshape(np_, C_, n_iter).transpose(2, 1, 0).copy()
+        o.accept_hist = o.accept_hist.reshape(nb, C_)
+        o.final_sd = o.final_sd.reshape(C_, np_)
+        o.final_state = o.final_state.reshape(C_, np_)
+    return outs, {"kernel_ms": opts.kernel_ms,
+                  "n_workgroups": opts.n_workgroups}

@@ -14,10 +14,11 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgpum_hip.so")
 
-SOURCES = ["gpum_hip.cpp", "lower.cpp", "da.cpp", "chain_kernel.hip",
-           "mlp.hip", "da_kernels.hip"]
+SOURCES = ["gpum_hip.cpp", "lower.cpp", "da.cpp", "batch.cpp",
+           "chain_kernel.hip", "mlp.hip", "da_kernels.hip", "batch_kernels.hip"]
 HEADERS = ["vm.h", "lower.h", "chain_kernel.h", "device_vm.h", "math64.h",
-           "philox.h", "runtime.h", "step_common.h", "bf16.h", "mlp.h", "da.h"]
+           "philox.h", "runtime.h", "step_common.h", "bf16.h", "mlp.h", "da.h",
+           "batch.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -63,6 +63,7 @@ struct ChainArgs {
   double* trace_prop;
   uint8_t* trace_accept;
   double* trace_ll;        // data term of each proposal
+  uint32_t draws_ld;       // iteration stride of `draws` (0: n_iter)
   // tiling
   uint32_t cpb;  // chains per workgroup
   uint32_t wpc;  // waves per chain
@@ -105,6 +106,25 @@ constexpr size_t kStreamStaticLds = 2 * (size_t)kTileF4 * 16;
 // Copies per wave that fill one buffer with 1024 threads.
 constexpr uint32_t kTileIssues = (kTileF4 + 1023) / 1024;
 
+// Batched launches: one workgroup per (model, first chain) item.
+struct BatchWork {
+  uint32_t model;
+  uint32_t chain0;
+};
+// Per-launch values of a batched run (one batch of iterations).
+struct BatchStep {
+  uint32_t seed_add;   // added to each model's seed
+  uint32_t n_iter;     // iterations of this batch
+  uint32_t t0;         // first draw row of this batch
+  uint32_t draws_ld;   // iteration stride of the models' draws
+};
+// Static LDS of the batched kernel beyond the tile buffers (its copy of
+// the model's arguments).
+constexpr size_t kBatchArgsLds = 512;
+// Static LDS of the points kernel beyond the tile buffers (item and plan).
+constexpr size_t kPointsExtraLds = 64;
+static_assert(sizeof(ChainArgs) <= kBatchArgsLds, "ChainArgs too large");
+
 // The data term of a list of parameter points, summed per observation
 // slice by a persistent grid of n_workgroups that takes (slot group, slice)
 // items from a queue; one wave per slot.
@@ -126,6 +146,11 @@ struct PointsArgs {
 hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
                                int ns, int opt, size_t lds_bytes,
                                hipStream_t s);
+hipError_t launch_chain_kernel_batch(const BatchWork* work, uint32_t n_wg,
+                                     const ChainArgs* args,
+                                     const ChainHot* hots, BatchStep step,
+                                     int ns, int opt, size_t lds_bytes,
+                                     hipStream_t s);
 // `p` is a device copy of the arguments.
 hipError_t launch_points_kernel(const PointsArgs* p, uint32_t n_workgroups,
                                 const ChainHot& h, int ns, int opt,

@@ -106,8 +106,9 @@ __device__ __noinline__ void chain_finish_iter(const ChainArgs* __restrict__ A,
     r.batch_acc() += 1.0;
   }
   if (A->trace_accept) A->trace_accept[(size_t)t * C + chain] = acc_ok ? 1 : 0;
+  const size_t ld = A->draws_ld ? A->draws_ld : T_iter;
   for (int j = 0; j < np; ++j)
-    A->draws[t + (size_t)T_iter * (chain + (size_t)C * j)] = (float)r.state[j];
+    A->draws[t + ld * (chain + (size_t)C * j)] = (float)r.state[j];
   if (A->adapt_batches > 0 && A->swap_every > 0) {
     const uint32_t b = min(t / A->swap_every + 1, A->n_batches);
     if (b <= A->adapt_batches) {
@@ -346,9 +347,12 @@ __device__ __forceinline__ double tile_pass(const ChainHot& H,
   return acc;
 }
 
+// One workgroup's chains chain0 .. chain0 + cpb - 1 through the whole run.
 template <int NS, int OPT, bool STREAM>
-__global__ void __launch_bounds__(1024)
-chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
+// H by value: a reference to the kernel argument made the register
+// allocator spill around the tile copies (a wait on the copy itself).
+__device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
+                                           ChainHot H, uint32_t chain0) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int SW = (int)H.state_words;
   const int cpb = (int)H.cpb, wpc = (int)H.wpc;
@@ -364,7 +368,7 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int c_local = wave / wpc;
   const int part = wave - c_local * wpc;
-  const uint32_t chain = blockIdx.x * (uint32_t)cpb + (uint32_t)c_local;
+  const uint32_t chain = chain0 + (uint32_t)c_local;
   const bool active = chain < H.n_chains;
   const bool lead = active && part == 0 && lane == 0;
   double* st = st_all + c_local * SW;
@@ -383,8 +387,18 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
   const uint32_t wave_base = (uint32_t)(wave * 64);
   float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
 
+  // The copy of issue_tile_dma written out here: as a call it raises the
+  // scalar register pressure of the iteration loop.
   auto issue_tile = [&](uint32_t ti, float4* dst) {
-    issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti, dst);
+#pragma unroll
+    for (uint32_t k = 0; k < kTileIssues; ++k) {
+      const uint32_t i0 = min(wave_base + k * 1024u, tile4 - 64u);
+      const uint32_t col = i0 / n4, off0 = i0 - col * n4;
+      const float4* base = src + ((size_t)col * ld4 + (size_t)ti * n4 + off0);
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(base + lane),
+          (__attribute__((address_space(3))) void*)(dst + i0), 16, 0, 0);
+    }
   };
 
   if constexpr (!STREAM) {
@@ -466,8 +480,7 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
         if (lead) chain_adapt(Ap, b, bsize, chain, st);
         if (H.swap_group > 1) {
           __syncthreads();
-          if (tid == 0)
-            pt_sweep(Ap, b, blockIdx.x * (uint32_t)cpb, cpb, st_all, SW);
+          if (tid == 0) pt_sweep(Ap, b, chain0, cpb, st_all, SW);
         }
         __syncthreads();
       }
@@ -476,6 +489,35 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
     __syncthreads();
   }
   if (lead) chain_finish(Ap, chain, st);
+}
+
+template <int NS, int OPT, bool STREAM>
+__global__ void __launch_bounds__(1024)
+chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
+  chain_body<NS, OPT, STREAM>(Ap, H, blockIdx.x * H.cpb);
+}
+
+// Many models in one launch: workgroup w runs chains work[w].chain0 .. of
+// model work[w].model, with that model's arguments (copied to LDS and
+// patched with this batch's seed, length and draw offset) and hot values.
+template <int NS, int OPT>
+__global__ void __launch_bounds__(1024)
+chain_kernel_batch(const BatchWork* __restrict__ work,
+                   const ChainArgs* __restrict__ args,
+                   const ChainHot* __restrict__ hots, BatchStep step) {
+  __shared__ ChainArgs a_sh;
+  const BatchWork w = work[blockIdx.x];
+  ChainHot H = hots[w.model];
+  H.n_iter = step.n_iter;
+  if (threadIdx.x == 0) {
+    a_sh = args[w.model];
+    a_sh.seed += step.seed_add;
+    a_sh.n_iter = step.n_iter;
+    a_sh.draws += step.t0;
+    a_sh.draws_ld = step.draws_ld;
+  }
+  __syncthreads();
+  chain_body<NS, OPT, true>(&a_sh, H, w.chain0);
 }
 
 // Data term of a list of parameter points (no chain state).  A persistent
@@ -623,7 +665,9 @@ points_kernel(const PointsArgs* __restrict__ P, ChainHot H) {
   template __global__ void chain_kernel<NS, OPT, true>(const ChainArgs*, ChainHot); \
   template __global__ void points_kernel<NS, OPT, false>(const PointsArgs*,  \
                                                          ChainHot);         \
-  template __global__ void points_kernel<NS, OPT, true>(const PointsArgs*, ChainHot);
+  template __global__ void points_kernel<NS, OPT, true>(const PointsArgs*, ChainHot); \
+  template __global__ void chain_kernel_batch<NS, OPT>(                      \
+      const BatchWork*, const ChainArgs*, const ChainHot*, BatchStep);
 GPUM_INST(0, 16)
 GPUM_INST(1, 16)
 GPUM_INST(3, 16)
@@ -684,6 +728,27 @@ hipError_t launch_points_kernel(const PointsArgs* p, uint32_t n_workgroups,
   return hipErrorInvalidValue;
 }
 
+hipError_t launch_chain_kernel_batch(const BatchWork* work, uint32_t n_wg,
+                                     const ChainArgs* args,
+                                     const ChainHot* hots, BatchStep step,
+                                     int ns, int opt, size_t lds_bytes,
+                                     hipStream_t s) {
+#define GPUM_CASE(NS_, OPT_)                                                 \
+  if (ns == NS_ && opt == OPT_) {                                            \
+    hipLaunchKernelGGL((chain_kernel_batch<NS_, OPT_>), dim3(n_wg),          \
+                       dim3(1024), lds_bytes, s, work, args, hots, step);    \
+    return hipGetLastError();                                                \
+  }
+  GPUM_CASE(0, 16)
+  GPUM_CASE(1, 16)
+  GPUM_CASE(3, 16)
+  GPUM_CASE(7, 8)
+  GPUM_CASE(15, 4)
+  GPUM_CASE(31, 4)
+#undef GPUM_CASE
+  return hipErrorInvalidValue;
+}
+
 hipError_t set_chain_kernel_lds_limit(size_t bytes) {
   hipError_t e = hipSuccess;
 #define GPUM_ATTR(NS_, OPT_)                                                 \
@@ -694,6 +759,18 @@ hipError_t set_chain_kernel_lds_limit(size_t bytes) {
   e = hipFuncSetAttribute((const void*)chain_kernel<NS_, OPT_, true>,        \
                           hipFuncAttributeMaxDynamicSharedMemorySize,        \
                           (int)(bytes - kStreamStaticLds));                  \
+  if (e != hipSuccess) return e;                                             \
+  e = hipFuncSetAttribute((const void*)points_kernel<NS_, OPT_, false>,      \
+                          hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                          (int)(bytes - kPointsExtraLds));                   \
+  if (e != hipSuccess) return e;                                             \
+  e = hipFuncSetAttribute((const void*)points_kernel<NS_, OPT_, true>,       \
+                          hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                          (int)(bytes - kStreamStaticLds - kPointsExtraLds)); \
+  if (e != hipSuccess) return e;                                             \
+  e = hipFuncSetAttribute((const void*)chain_kernel_batch<NS_, OPT_>,        \
+                          hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                          (int)(bytes - kStreamStaticLds - kBatchArgsLds));  \
   if (e != hipSuccess) return e;
   GPUM_ATTR(0, 16)
   GPUM_ATTR(1, 16)

@@ -86,8 +86,13 @@ bool device_init() {
   g_dev.lds_max = p.sharedMemPerBlock;
   if (g_dev.lds_max < 65536) g_dev.lds_max = 65536;
   if (g_dev.lds_max > 160 * 1024) g_dev.lds_max = 160 * 1024;
-  if (set_chain_kernel_lds_limit(g_dev.lds_max) != hipSuccess) {
-    g_dev.lds_max = 65536;
+  hipError_t le = set_chain_kernel_lds_limit(g_dev.lds_max);
+  if (le != hipSuccess) {
+    g_dev.why = std::string("cannot raise the kernels' LDS limit to ") +
+                std::to_string(g_dev.lds_max) + " bytes: kernel/device error (" +
+                hipGetErrorString(le) + ")";
+    set_err("%s", g_dev.why.c_str());
+    return false;
   }
   if (hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking) != hipSuccess) {
     g_dev.why = "cannot create a HIP stream on the device";
@@ -212,8 +217,12 @@ namespace gpum {
 namespace rt {
 
 bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
-                 const gpum_run_ext* ext, Tiling* t, bool points) {
-  t->v = pick_variant(m->L.vm_depth);
+                 const gpum_run_ext* ext, Tiling* t, bool points,
+                 const TilingForce* force) {
+  t->v = force ? force->v : pick_variant(m->L.vm_depth);
+  const size_t static_lds =
+      kStreamStaticLds + (force ? force->extra_static
+                                : (points ? kPointsExtraLds : 0));
   const uint32_t opt = (uint32_t)t->v.opt;
   t->chunk = 64u * opt;
   const uint32_t ncu = (uint32_t)g_dev.n_cu;
@@ -265,8 +274,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   t->wpc = wpc;
   const uint32_t ncol = std::max(1u, m->n_cols_total);
   auto head_bytes = [&](uint32_t c, uint32_t w) {
-    // points: uniform tables, and 16 bytes for the kernel's static item slot
-    if (points) return ((size_t)c * m->n_uniform_pad * 4 + 15) / 16 * 16 + 16;
+    if (points) return ((size_t)c * m->n_uniform_pad * 4 + 15) / 16 * 16;
     const size_t state_bytes = (size_t)c * (6 * m->n_params + 6) * 8;
     const size_t part_bytes = (size_t)c * w * 8;
     const size_t u_bytes = (size_t)c * m->n_uniform_pad * 4;
@@ -276,7 +284,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   // streamed tile buffers.
   const bool pinned = ext && ext->chains_per_block > 0;
   while (!pinned && cpb > 1 &&
-         head_bytes(cpb, wpc) + kStreamStaticLds > g_dev.lds_max) {
+         head_bytes(cpb, wpc) + static_lds > g_dev.lds_max) {
     cpb /= 2;
   }
   t->cpb = cpb;
@@ -285,8 +293,9 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   const uint32_t step = t->chunk * wpc;  // one chunk per wave of a chain
   const size_t tile_floats = (size_t)kTileF4 * 4;  // one static buffer
   uint32_t n_res = round_up(std::max(1u, m->n_obs), step);
-  if ((size_t)n_res * ncol * 4 <= (size_t)96 * 1024 &&
-      head + (size_t)n_res * ncol * 4 <= g_dev.lds_max) {
+  if (!force && (size_t)n_res * ncol * 4 <= (size_t)96 * 1024 &&
+      head + (size_t)n_res * ncol * 4 + (points ? kPointsExtraLds : 0) <=
+          g_dev.lds_max) {
     // Resident: the whole padded data set in dynamic LDS.
     t->resident = 1;
     t->stream_mode = 0;
@@ -295,7 +304,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
     t->lds_bytes = head + (size_t)n_res * ncol * 4;
     return true;
   }
-  if (head + kStreamStaticLds > g_dev.lds_max) {
+  if (head + static_lds > g_dev.lds_max) {
     set_err("per-chain state too large for the LDS budget (%u chains per "
             "block, %u parameters)", cpb, m->n_params);
     return false;

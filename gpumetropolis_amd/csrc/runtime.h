@@ -114,10 +114,19 @@ struct gpum_hip_model {
 namespace gpum {
 namespace rt {
 
+// Constraints a batched launch puts on each model's tiling.
+struct TilingForce {
+  Variant v;            // VM variant shared by the launch
+  size_t extra_static;  // static LDS beyond the two tile buffers
+};
+
 // Tiling of a launch over n_chains chains (or, with `points`, of the points
 // kernel over n_chains slots: one wave per slot, 16 slots per workgroup).
+// With `force`: the given variant, always streamed, room for extra static
+// LDS.
 bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
-                 const gpum_run_ext* ext, Tiling* t, bool points = false);
+                 const gpum_run_ext* ext, Tiling* t, bool points = false,
+                 const TilingForce* force = nullptr);
 // Lay the model's columns out on the device with column stride n_pad.
 bool ensure_data(gpum_hip_model* m, uint32_t n_pad);
 

@@ -15,6 +15,12 @@ as the reference's R layer hands data to its backends.
                   params mu1, mu2; 16 temperatures x 256 replicas.
   logistic16      16-dim Bayesian logistic regression, n = 4 000 000 (FP32),
                   equicorrelated (0.3) N(0, 1) covariates, N(0, 5) priors.
+  regressions     512 independent linear regressions, n_m log-uniform in
+                  [1e3, 1e6] (FP32), x1..x8 ~ N(0, 1), beta ~ N(0, 1),
+                  sigma = 0.5; params b0..b8, log_sigma, flat priors;
+                  DE-MC with 64 chains per model.  Closed-form posterior:
+                  b ~ t_{n-p}(beta_hat, s^2 (X'X)^-1), sigma^2 ~
+                  Inv-Gamma((n-p)/2, (n-p) s^2 / 2), p = 9.
 """
 from __future__ import annotations
 
@@ -111,6 +117,54 @@ def logistic(n: int = 4_000_000, d: int = 16, seed: int = 15,
                    prior=_logistic_prior(d))
     return Workload("logistic16", m, data, n_chains, n_iter,
                     meta={"true": beta})
+
+
+REGRESSION = ("~ -((y - b0 - b1 * x1 - b2 * x2 - b3 * x3 - b4 * x4 - b5 * x5"
+              " - b6 * x6 - b7 * x7 - b8 * x8)^2) / (2 * exp(2 * log_sigma))"
+              " - log_sigma")
+REGRESSION_PARAMS = [f"b{j}" for j in range(9)] + ["log_sigma"]
+
+
+def regression_posterior(data: np.ndarray) -> Dict:
+    """Closed-form posterior moments of one regression (flat priors on b
+    and log sigma): b ~ t_{n-p}(beta_hat, s^2 (X'X)^-1) with variance
+    s^2 (X'X)^-1 (n-p)/(n-p-2); sigma^2 ~ Inv-Gamma(a, b), a = (n-p)/2,
+    b = (n-p) s^2 / 2, so E log sigma = (log b - digamma(a)) / 2 and
+    Var log sigma = trigamma(a) / 4."""
+    from scipy.special import digamma, polygamma
+    y = data[:, 0]
+    X = np.column_stack([np.ones(len(y)), data[:, 1:]])
+    n, p = X.shape
+    XtX = X.T @ X
+    beta = np.linalg.solve(XtX, X.T @ y)
+    rss = float(np.sum((y - X @ beta) ** 2))
+    s2 = rss / (n - p)
+    cov = s2 * np.linalg.inv(XtX) * (n - p) / (n - p - 2)
+    a, b = (n - p) / 2.0, rss / 2.0
+    mean = np.append(beta, 0.5 * (np.log(b) - digamma(a)))
+    sd = np.append(np.sqrt(np.diag(cov)), 0.5 * np.sqrt(polygamma(1, a)))
+    return {"mean": mean, "sd": sd}
+
+
+def regressions(n_models: int = 512, seed: int = 16, n_min: int = 1000,
+                n_max: int = 1_000_000, n_chains: int = 64,
+                n_iter: int = 4000) -> List[Workload]:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ns = np.floor(np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_models)))
+    m = gpum_model(REGRESSION, params=REGRESSION_PARAMS,
+                   data=["y"] + [f"x{j}" for j in range(1, 9)])
+    out = []
+    for k, n in enumerate(ns.astype(int)):
+        X = rng.standard_normal((n, 8), dtype=np.float32)
+        beta = rng.standard_normal(9)
+        y = (beta[0] + X.astype(np.float64) @ beta[1:] +
+             0.5 * rng.standard_normal(n)).astype(np.float32)
+        data = np.empty((n, 9), np.float64)
+        data[:, 0] = y
+        data[:, 1:] = X
+        out.append(Workload(f"regression{k}", m, data, n_chains, n_iter,
+                            meta={"true": np.append(beta, np.log(0.5))}))
+    return out
 
 
 def by_name(name: str, **kw) -> Workload:

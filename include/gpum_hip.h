@@ -277,6 +277,53 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* s,
                           const double* proposal_l, gpum_da_ext* ext,
                           gpum_chain_result* out);
 
+/* ---- Many models in one run -------------------------------------------
+ * No single-call counterpart in the reference, whose host runs one model
+ * per rust_gpu_metropolis call (R/api.R:247-599).  Each model keeps its own
+ * program, data, chains, seed and Philox stream id; every batch of
+ * iterations of every model is one launch, the between-batch step of the
+ * reference's orchestrator (R/adaptive.R:427-524 for DE path A: the
+ * population snapshot is refreshed from the last f32 draws, and during the
+ * warmup batches each chain's sd becomes sqrt(diag cov) of its draws so
+ * far) is a second launch, and with use_graph all batches are one HIP
+ * graph.  Batch b (1-based) draws its uniforms with seed + b - 1, as the
+ * reference reseeds each batch (R/adaptive.R:471). */
+typedef struct gpum_batch_model {
+  gpum_hip_model* model;
+  uint32_t n_chains;
+  const double* init;         /* n_chains * n_params */
+  const double* proposal_sd;  /* n_chains * n_params */
+  uint32_t seed;
+  uint32_t stream_id;         /* Philox key word 1 */
+  double gamma;               /* DE difference scale (proposal_mode 1) */
+  double de_noise;
+  /* Host outputs filled when non-NULL: */
+  float* draws;               /* n_iter * n_chains * n_params, all iterations,
+                                 draws[t + n_iter * (c + n_chains * j)] */
+  float* accept_hist;         /* n_batches * n_chains */
+  double* final_sd;           /* n_chains * n_params */
+  double* last_log_post;      /* n_chains */
+  double* final_state;        /* n_chains * n_params: the last snapshot */
+} gpum_batch_model;
+
+typedef struct gpum_batch_opts {
+  uint32_t struct_size;
+  uint32_t proposal_mode;     /* 0 random walk, 1 DE against the snapshot */
+  uint32_t n_iter;            /* all iterations, warmup included */
+  uint32_t batch_len;         /* iterations per batch (de_every); the last
+                                 batch absorbs the remainder */
+  uint32_t warmup_batches;    /* leading batches that adapt */
+  int32_t adapt;
+  int32_t chains_per_block;   /* 0 = 16; chains_per_block * waves_per_chain = 16 */
+  int32_t waves_per_chain;
+  int32_t use_graph;          /* capture all batches in one HIP graph */
+  double kernel_ms;           /* out: device time of the run */
+  uint32_t n_workgroups;      /* out: workgroups per batch launch */
+} gpum_batch_opts;
+
+int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
+                       gpum_batch_opts* opts);
+
 #ifdef __cplusplus
 }
 #endif
