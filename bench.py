@@ -47,6 +47,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target duration of the CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--jit", action="store_true",
+                   help="run every model on its compiled kernels instead of "
+                        "the interpreter")
     p.add_argument("--no-da", action="store_true",
                    help="skip the surrogate delayed-acceptance comparison")
     p.add_argument("--da-obs", type=int, default=4_000_000)
@@ -119,8 +122,6 @@ def da_section(a):
     same proposal; plain MH runs on the faster of the two plain engines
     (the stepwise sampler splits the observations over every CU; the fused
     chain kernel's 1024 chains fill a quarter of them)."""
-    import torch
-
     from gpumetropolis_amd import _native, adaptive as ad, workloads
     from gpumetropolis_amd.api import _StepCaller, adaptive_warmup
     from gpumetropolis_amd.diagnostics import rhat, summarize
@@ -131,7 +132,7 @@ def da_section(a):
     m = w.model
     d, C, N = m.n_params, a.da_chains, w.n_obs
     dm = _native.DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
-                             m.prior.consts, d, w.data, w.n_cols, N)
+                             m.prior.consts, d, w.data, w.n_cols, N, jit=a.jit)
     info = dm.info()
     print(f"[bench-da] data + model {time.perf_counter() - t0:.1f} s: {info}",
           file=sys.stderr, flush=True)
@@ -175,11 +176,10 @@ def da_section(a):
         draws, state, wall, kms, exact, t = [], wu.init, 0.0, 0.0, 0.0, 0
         s1, acc = [], []
         while True:
-            torch.cuda.synchronize()
+            # run_da returns after its stream has drained
             t0 = time.perf_counter()
             r = dm.run_da(net, state, np.ones(C), a.da_iters, seed2,
                           proposal_l=wu.L * f, final_state=True, iter_offset=t)
-            torch.cuda.synchronize()
             wall += time.perf_counter() - t0
             kms += r.kernel_ms
             exact += r.exact_evals
@@ -270,7 +270,7 @@ def batch_section(a):
     bf = gpu_metropolis_batch([w.model for w in wls], [w.data for w in wls],
                               n_chains=64, n_iter=a.batch_iters,
                               init=inits, proposal_sd=0.003, seed=a.seed,
-                              tiling="auto")
+                              tiling="auto", jit=a.jit)
     wall = time.perf_counter() - t0
     evals = n_tot * 64 * a.batch_iters
     # closed-form check on the first models
@@ -306,22 +306,20 @@ def main():
                           "be 1 (multi-GPU sharding is described in "
                           "DESIGN.md, not launched)"}))
         sys.exit(2)
-    import torch
-
+    # torch is not imported: it bundles its own HIP runtime, and the
+    # library's calls already return after their stream has drained.
     from gpumetropolis_amd import _native, adaptive as ad, workloads
     from gpumetropolis_amd.api import adaptive_warmup
     from gpumetropolis_amd.diagnostics import summarize
 
-    assert torch.cuda.is_available(), "bench needs a GPU"
-    torch.cuda.set_device(0)
     if not _native.available():
-        raise RuntimeError(_native.last_error())
+        raise RuntimeError("bench needs a gfx950 GPU: " + _native.last_error())
 
     w = workloads.gamma(n=a.n_obs, n_chains=a.chains)
     m = w.model
     C, P, N = a.chains, m.n_params, w.n_obs
     dm = _native.DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
-                             m.prior.consts, P, w.data, 1, N)
+                             m.prior.consts, P, w.data, 1, N, jit=a.jit)
     info = dm.info()
     print(f"[bench] model lowered: {info}", file=sys.stderr, flush=True)
     state = {"kernel_ms": []}
@@ -352,7 +350,6 @@ def main():
     # final_state of the last warmup batch is the exact FP64 state; the
     # warmup hands back the f32 draws, so continue from them as R does.
     mode = 2 if wu.L is not None else 0
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(a.steps):
         r = call(cur, wu.sd, a.iters_per_step, a.seed + wu.batches_run + k,
@@ -361,7 +358,6 @@ def main():
         draws.append(r.draws)
         print(f"[bench] step {k + 1}/{a.steps} kernel {r.kernel_ms:.1f} ms",
               file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
     iters = a.steps * a.iters_per_step

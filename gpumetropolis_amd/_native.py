@@ -150,6 +150,9 @@ def lib():
     L.gpum_hip_model_loglik.restype = C.c_int
     L.gpum_hip_model_free.argtypes = [C.c_void_p]
     L.gpum_hip_model_free.restype = None
+    L.gpum_hip_model_set_jit.argtypes = [C.c_void_p, C.c_int32,
+                                         C.POINTER(C.c_double)]
+    L.gpum_hip_model_set_jit.restype = C.c_int
     _f32p = C.POINTER(C.c_float)
     L.gpum_hip_surrogate_n_weights.restype = C.c_uint32
     L.gpum_hip_surrogate_create.argtypes = [C.c_uint32, C.c_uint32]
@@ -363,7 +366,7 @@ class DeviceModel:
     """A model whose lowered program and data stay on the device."""
 
     def __init__(self, code, consts, prior_code, prior_consts, n_params, data,
-                 n_cols, n_obs, precompute_data=None, fuse=None):
+                 n_cols, n_obs, precompute_data=None, fuse=None, jit=False):
         L = lib()
         self._L = L
         code, prior_code = _u32(code), _u32(prior_code)
@@ -382,6 +385,20 @@ class DeviceModel:
         if not h:
             raise BackendError(last_error())
         self._h = h
+        self.jit = False
+        self.jit_compile_ms = 0.0
+        if jit:
+            self.set_jit(True)
+
+    def set_jit(self, enable: bool) -> float:
+        """Compiled per-model kernels (True) or the interpreter (False);
+        returns the compile time in ms (0 when cached)."""
+        ms = C.c_double(0.0)
+        _check(self._L.gpum_hip_model_set_jit(self._h, 1 if enable else 0,
+                                              C.byref(ms)))
+        self.jit = bool(enable)
+        self.jit_compile_ms = ms.value
+        return ms.value
 
     def info(self) -> dict:
         inf = ModelInfo()

@@ -92,7 +92,7 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                    de_noise: float = 1e-3, de_every=None, de_sync=False,
                    seed: int = 1, backend: str = "hip",
                    tiling=None, ladder_size=None, surrogate=None,
-                   engine: str = "auto") -> Fit:
+                   engine: str = "auto", jit: bool = False) -> Fit:
     if backend not in ("hip", "auto"):
         raise ValueError(f"backend '{backend}' is not available here; this "
                          "package is the hip backend")
@@ -125,7 +125,7 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
     sd_mat = _sd_matrix(proposal_sd, n_chains, np_)
 
     dm = DeviceModel(model.loglik.code, model.loglik.consts, model.prior.code,
-                     model.prior.consts, np_, X, model.n_cols, n_obs)
+                     model.prior.consts, np_, X, model.n_cols, n_obs, jit=jit)
     if engine not in ("auto", "fused", "stepwise"):
         raise ValueError(f"unknown engine '{engine}'")
     if engine == "auto":

@@ -41,7 +41,8 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
                          de_noise: float = 1e-3, de_every=None,
                          adapt: bool = True, seed: int = 1, tiling="auto",
                          use_graph: bool = True,
-                         tile_cache: Optional[str] = None) -> BatchFit:
+                         tile_cache: Optional[str] = None,
+                         jit: bool = False) -> BatchFit:
     if method not in ("de", "rwm"):
         raise ValueError('batched runs take method "de" or "rwm"')
     if len(models) != len(datas) or not models:
@@ -72,7 +73,8 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
             X = _data_matrix(m, data)
             np_ = m.n_params
             dms.append(DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
-                                   m.prior.consts, np_, X, m.n_cols, X.shape[0]))
+                                   m.prior.consts, np_, X, m.n_cols, X.shape[0],
+                                   jit=jit))
             C = chains[k]
             if init is None:
                 im = normals(seed, C * np_, stream=k).reshape(C, np_)
@@ -88,7 +90,7 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
         mode = 1 if method == "de" else 0
         infos = [dm.info() for dm in dms]
         key = autotune.tile_key([i["n_vm_instr"] for i in infos],
-                                [d.n_obs for d in dms])
+                                [d.n_obs for d in dms]) + ("-jit" if jit else "")
         timings = {}
         if tiling == "auto":
             def time_candidate(t):

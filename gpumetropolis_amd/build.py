@@ -14,15 +14,24 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgpum_hip.so")
 
-SOURCES = ["gpum_hip.cpp", "lower.cpp", "da.cpp", "batch.cpp",
+SOURCES = ["gpum_hip.cpp", "lower.cpp", "da.cpp", "batch.cpp", "jit.cpp",
            "chain_kernel.hip", "mlp.hip", "da_kernels.hip", "batch_kernels.hip"]
 HEADERS = ["vm.h", "lower.h", "chain_kernel.h", "device_vm.h", "math64.h",
            "philox.h", "runtime.h", "step_common.h", "bf16.h", "mlp.h", "da.h",
-           "batch.h"]
+           "batch.h", "chain_body.h", "jit.h"]
+# Device headers embedded into the library for the per-model kernels
+# compiled at run time (csrc/jit.cpp).
+JIT_HEADERS = ["chain_body.h", "chain_kernel.h", "device_vm.h", "math64.h",
+               "philox.h", "step_common.h", "vm.h"]
+JIT_INC = os.path.join(CSRC, "jit_sources.inc")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Wno-unused-result", "-Wno-unused-value",
+    # Contract a*b+c only within one expression, never across statements:
+    # the per-model kernels (straight-line code) then round exactly as the
+    # interpreter, which executes one VM instruction at a time.
+    "-ffp-contract=on",
     # Packed FP32 arithmetic stays off (ordinary float code is emitted as
     # scalar v_*_f32).
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -36,11 +45,35 @@ def hipcc() -> str:
     return "hipcc"
 
 
+def write_jit_sources() -> None:
+    """Embed the device headers as raw string literals (jit_sources.inc)."""
+    names, bodies = [], []
+    for h in JIT_HEADERS:
+        with open(os.path.join(CSRC, h)) as f:
+            text = f.read()
+        if ")GPUMJIT" in text:
+            raise RuntimeError(f"{h} contains the raw-string delimiter")
+        names.append(f'"{h}"')
+        bodies.append(f'R"GPUMJIT({text})GPUMJIT"')
+    out = ("// Generated by gpumetropolis_amd/build.py from the device headers;"
+           " do not edit.\n"
+           f"static const int kJitHeaderCount = {len(names)};\n"
+           "static const char* const kJitHeaderNames[] = {" + ", ".join(names)
+           + "};\n"
+           "static const char* const kJitHeaderSources[] = {\n"
+           + ",\n".join(bodies) + "};\n")
+    old = open(JIT_INC).read() if os.path.exists(JIT_INC) else None
+    if old != out:
+        with open(JIT_INC, "w") as f:
+            f.write(out)
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
+    write_jit_sources()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), *HIPCC_FLAGS, *[os.path.join(CSRC, s) for s in SOURCES],
-           "-o", tmp]
+           "-L/opt/rocm/lib", "-lhiprtc", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

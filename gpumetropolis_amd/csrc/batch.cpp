@@ -21,6 +21,7 @@
 #include "../../include/gpum_hip.h"
 #include "batch.h"
 #include "chain_kernel.h"
+#include "jit.h"
 #include "runtime.h"
 
 using namespace gpum;
@@ -236,6 +237,21 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
 
   hipStream_t st = g_dev.stream;
   const uint32_t n_wg = (uint32_t)work.size();
+  // One compiled kernel serves the launch when every model runs the same
+  // compiled program with the launch's variant.
+  hipFunction_t jf = nullptr;
+  {
+    const Variant v0 = pick_variant(models[0].model->L.vm_depth);
+    bool same = v0.ns == force.v.ns && v0.opt == force.v.opt;
+    for (uint32_t k = 0; k < n_models && same; ++k)
+      same = !models[k].model->jit_eval.empty() &&
+             models[k].model->jit_eval == models[0].model->jit_eval;
+    if (same) {
+      bool failed = false;
+      jf = model_jit_kernel(models[0].model, JIT_BATCH, lds + kBatchArgsLds, &failed);
+      if (failed) return 1;
+    }
+  }
   auto enqueue = [&]() -> hipError_t {
     uint32_t t0 = 0;
     for (uint32_t b = 1; b <= nb; ++b) {
@@ -244,8 +260,17 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
       step.n_iter = sizes[b - 1];
       step.t0 = t0;
       step.draws_ld = o.n_iter;
-      hipError_t e = launch_chain_kernel_batch(d_work.p, n_wg, d_args.p, d_hots.p,
-                                               step, force.v.ns, force.v.opt, lds, st);
+      hipError_t e;
+      if (jf) {
+        const BatchWork* wp = d_work.p;
+        const ChainArgs* ap = d_args.p;
+        const ChainHot* hp = d_hots.p;
+        void* kargs[] = {(void*)&wp, (void*)&ap, (void*)&hp, (void*)&step};
+        e = launch_module(jf, dim3(n_wg), dim3(1024), lds, st, kargs);
+      } else {
+        e = launch_chain_kernel_batch(d_work.p, n_wg, d_args.p, d_hots.p, step,
+                                      force.v.ns, force.v.opt, lds, st);
+      }
       if (e != hipSuccess) return e;
       e = launch_batch_end(d_info.p, chain_model.p, n_chains, t0, sizes[b - 1], b,
                            (o.adapt && b <= o.warmup_batches) ? 1 : 0, st);

@@ -143,6 +143,8 @@ struct PointsArgs {
   double* partial;         // slices x n_points; slices <= min(tiles, target_items)
 };
 
+// Host launchers (not seen by the run-time compiler of per-model kernels).
+#ifndef __HIPCC_RTC__
 hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
                                int ns, int opt, size_t lds_bytes,
                                hipStream_t s);
@@ -156,5 +158,6 @@ hipError_t launch_points_kernel(const PointsArgs* p, uint32_t n_workgroups,
                                 const ChainHot& h, int ns, int opt,
                                 size_t lds_bytes, hipStream_t s);
 hipError_t set_chain_kernel_lds_limit(size_t bytes);
+#endif  // __HIPCC_RTC__
 
 }  // namespace gpum

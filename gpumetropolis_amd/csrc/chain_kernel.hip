@@ -21,642 +21,28 @@
 #include <math.h>
 #include <stdint.h>
 
-#include "chain_kernel.h"
-#include "device_vm.h"
-#include "math64.h"
-#include "philox.h"
-#include "step_common.h"
+#include "chain_body.h"
 
 namespace gpum {
-
-// Per-chain record in LDS (doubles):
-//   state[np] prop[np] z[np] sd[np] wmean[np] wm2[np]
-//   cur prior_lp accepts wn scale batch_accepts
-struct Rec {
-  double* state;
-  double* prop;
-  double* z;
-  double* sd;
-  double* wmean;
-  double* wm2;
-  double* tail;  // cur, prior_lp, accepts, wn, scale, batch_accepts
-  __device__ double& cur() { return tail[0]; }
-  __device__ double& prior_lp() { return tail[1]; }
-  __device__ double& accepts() { return tail[2]; }
-  __device__ double& wn() { return tail[3]; }
-  __device__ double& scale() { return tail[4]; }
-  __device__ double& batch_acc() { return tail[5]; }
-};
-
-__device__ __forceinline__ Rec make_rec(double* st, int np) {
-  Rec r;
-  r.state = st;
-  r.prop = st + np;
-  r.z = st + 2 * np;
-  r.sd = st + 3 * np;
-  r.wmean = st + 4 * np;
-  r.wm2 = st + 5 * np;
-  r.tail = st + 6 * np;
-  return r;
-}
-
-// Batch structure of a run split into batches of `every` iterations, the
-// last absorbing the remainder (R/adaptive.R:284-289).  Returns the 1-based
-// batch that ends after iteration t, or 0.
-__device__ __forceinline__ uint32_t batch_ending_at(uint32_t t, uint32_t n_iter,
-                                                    uint32_t every,
-                                                    uint32_t n_batches) {
-  if (every == 0) return 0;
-  if (t + 1 == n_iter) return n_batches;
-  if ((t + 1) % every == 0 && (t + 1) / every < n_batches) return (t + 1) / every;
-  return 0;
-}
-
-// Finish iteration it-1 with the data term `ll` of the proposal
-// (cpu_native.rs:785-799): tempered Metropolis decision in FP64, record the
-// f32 draw; during adaptation batches, fold the draw into the chain's
-// Welford accumulator (R/adaptive.R:31-42, diagonal part).
-__device__ __noinline__ void chain_finish_iter(const ChainArgs* __restrict__ A,
-                                               uint32_t it, uint32_t chain,
-                                               double* st, double ll) {
-  const int np = (int)A->n_params;
-  const uint32_t C = A->n_chains;
-  const uint32_t T_iter = A->n_iter;
-  const uint32_t spi = A->mode == 1 ? 2u * np + 4u : 2u * np + 1u;
-  Rec r = make_rec(st, np);
-  double plp = ll + r.prior_lp();
-  if (it == 0) {
-    r.cur() = plp;
-    if (A->init_log_post) A->init_log_post[chain] = plp;
-    return;
-  }
-  const uint32_t t = it - 1;
-  StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, chain};
-  double u = us.get(spi - 1);
-  bool acc_ok = log(u) < (plp - r.cur()) / A->temperatures[chain];
-  if (A->trace_plp) A->trace_plp[(size_t)t * C + chain] = plp;
-  if (A->trace_ll) A->trace_ll[(size_t)t * C + chain] = ll;
-  if (A->trace_prop)
-    for (int j = 0; j < np; ++j)
-      A->trace_prop[((size_t)t * C + chain) * np + j] = r.prop[j];
-  if (acc_ok) {
-    for (int j = 0; j < np; ++j) r.state[j] = r.prop[j];
-    r.cur() = plp;
-    r.accepts() += 1.0;
-    r.batch_acc() += 1.0;
-  }
-  if (A->trace_accept) A->trace_accept[(size_t)t * C + chain] = acc_ok ? 1 : 0;
-  const size_t ld = A->draws_ld ? A->draws_ld : T_iter;
-  for (int j = 0; j < np; ++j)
-    A->draws[t + ld * (chain + (size_t)C * j)] = (float)r.state[j];
-  if (A->adapt_batches > 0 && A->swap_every > 0) {
-    const uint32_t b = min(t / A->swap_every + 1, A->n_batches);
-    if (b <= A->adapt_batches) {
-      // Welford over the recorded f32 draws, as the reference's host sees
-      // them (R/adaptive.R:31-42).
-      r.wn() += 1.0;
-      for (int j = 0; j < np; ++j) {
-        double x = (double)(float)r.state[j];
-        double d1 = x - r.wmean[j];
-        r.wmean[j] += d1 / r.wn();
-        r.wm2[j] += d1 * (x - r.wmean[j]);
-      }
-    }
-  }
-}
-
-// End of batch b for one chain: Robbins-Monro on the batch acceptance and
-// the per-dimension scale sqrt(diag(cov)) (R/adaptive.R:342-356), while b is
-// an adaptation batch; then reset the batch counter.
-__device__ __noinline__ void chain_adapt(const ChainArgs* __restrict__ A,
-                                         uint32_t b, uint32_t bsize,
-                                         uint32_t chain, double* st) {
-  const int np = (int)A->n_params;
-  Rec r = make_rec(st, np);
-  // accept_rate crosses the reference's FFI as f32 (chain_kernel.rs:353).
-  const double acc = (double)((float)r.batch_acc() / (float)bsize);
-  if (A->acc_hist) A->acc_hist[(size_t)(b - 1) * A->n_chains + chain] = (float)acc;
-  r.batch_acc() = 0.0;
-  if (b > A->adapt_batches) return;
-  const double g = pow((double)(A->batch0 + b), -2.0 / 3.0);
-  r.scale() = exp(log(r.scale()) + g * (acc - target_accept(np)));
-  for (int j = 0; j < np; ++j) {
-    // .am_covariance: the seed Sigma (initial sd^2) until two draws.
-    double var = r.wn() >= 2.0 ? r.wm2[j] / (r.wn() - 1.0)
-                               : A->proposal_sd[(size_t)chain * np + j] *
-                                     A->proposal_sd[(size_t)chain * np + j];
-    r.sd[j] = r.scale() * sqrt(fmax(var, 1e-12));
-  }
-}
-
-// Replica-exchange sweep of batch b over the ladder groups of this
-// workgroup (R/adaptive.R:362-391): odd batches try pairs (0,1), (2,3), ...,
-// even batches (1,2), (3,4), ...; a pair swaps states and log-posteriors
-// when log u < (lp[i+1] - lp[i]) (1/T_i - 1/T_{i+1}).  The uniform for the
-// k-th pair of the sweep of group g is word 0 of Philox block
-// (b, k, g, TAG_SWAP).
-__device__ __noinline__ void pt_sweep(const ChainArgs* __restrict__ A,
-                                      uint32_t b, uint32_t chain0, int n_local,
-                                      double* st_all, int SW) {
-  const int np = (int)A->n_params;
-  const uint32_t G = A->swap_group;
-  const uint32_t bg = A->batch0 + b;  // global batch index
-  for (int g0 = 0; g0 + (int)G <= n_local; g0 += (int)G) {
-    const uint32_t gchain0 = chain0 + (uint32_t)g0;
-    if (gchain0 + G > A->n_chains) break;
-    const uint32_t group = gchain0 / G;
-    const int pairs = (int)G - 1;
-    if (pairs < 1) continue;
-    int start = (bg % 2u == 1u) ? 0 : (pairs >= 2 ? 1 : 0);
-    int k = 0;
-    for (int i = start; i < pairs; i += 2, ++k) {
-      U4 w = philox4x32_10(U4{bg, (uint32_t)k, group, TAG_SWAP}, A->seed,
-                           A->stream_id);
-      double u = u32_to_unit(w.x);
-      Rec lo = make_rec(st_all + (g0 + i) * SW, np);
-      Rec hi = make_rec(st_all + (g0 + i + 1) * SW, np);
-      const double t_lo = A->temperatures[gchain0 + i];
-      const double t_hi = A->temperatures[gchain0 + i + 1];
-      double delta = (hi.cur() - lo.cur()) * (1.0 / t_lo - 1.0 / t_hi);
-      bool ok = isfinite(delta) && log(u) < delta;
-      if (ok) {
-        for (int j = 0; j < np; ++j) {
-          double x = lo.state[j];
-          lo.state[j] = hi.state[j];
-          hi.state[j] = x;
-        }
-        double c = lo.cur();
-        lo.cur() = hi.cur();
-        hi.cur() = c;
-      }
-      if (A->swap_hist)
-        A->swap_hist[(size_t)(b - 1) * A->n_chains + gchain0 + i] = ok ? 1 : 0;
-    }
-  }
-}
-
-// Prepare iteration it: Philox uniforms, Box-Muller, the proposal
-// (cpu_native.rs:726-784), the hoisted parameter program into the uniform
-// table and the prior of the proposal.
-__device__ __noinline__ void chain_prepare_iter(const ChainArgs* __restrict__ A,
-                                                uint32_t it, uint32_t chain,
-                                                double* st, float* U) {
-  const int np = (int)A->n_params;
-  const uint32_t C = A->n_chains;
-  const uint32_t mode = A->mode;
-  const double two_pi = 6.283185307179586;
-  Rec r = make_rec(st, np);
-  StepUniforms us{A->seed, A->stream_id, A->iter_offset + it, chain};
-  const double* sd = r.sd;
-  if (mode == 0) {
-    for (int j = 0; j < np; ++j) {
-      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-      double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-      r.prop[j] = r.state[j] + sd[j] * z;
-    }
-  } else if (mode == 2) {
-    const double* L = A->proposal_l + (size_t)chain * np * np;
-    for (int j = 0; j < np; ++j) {
-      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-      r.z[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-    }
-    for (int k = 0; k < np; ++k) {
-      double a = r.state[k];
-      for (int j = 0; j <= k; ++j) a += L[k * np + j] * r.z[j];
-      r.prop[k] = a;
-    }
-  } else {
-    // DE path A: the pair comes from the frozen batch-start population in
-    // `init`; with probability 0.1 the scale is 1.
-    uint32_t ia = (uint32_t)(us.get(0) * (double)C);
-    if (ia >= C) ia = C - 1;
-    uint32_t ib = (uint32_t)(us.get(1) * (double)C);
-    if (ib >= C) ib = C - 1;
-    if (ib == ia) ib = (ib + 1) % C;
-    double g = us.get(2) < 0.1 ? 1.0 : A->gamma;
-    for (int j = 0; j < np; ++j) {
-      double u1 = us.get(3 + 2 * j), u2 = us.get(4 + 2 * j);
-      double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-      double diff = A->init[(size_t)ia * np + j] - A->init[(size_t)ib * np + j];
-      r.prop[j] = r.state[j] + g * diff + A->de_noise * sd[j] * z;
-    }
-  }
-  eval_params64(A->hoist, A->n_hoist, A->consts, r.prop, U);
-  r.prior_lp() = eval_params64(A->prior, A->n_prior, A->prior_consts, r.prop, nullptr);
-}
-
-__device__ __noinline__ void chain_init(const ChainArgs* __restrict__ A,
-                                        uint32_t chain, double* st, float* U) {
-  const int np = (int)A->n_params;
-  Rec r = make_rec(st, np);
-  for (int j = 0; j < np; ++j) {
-    double v = A->init[(size_t)chain * np + j];
-    r.state[j] = v;
-    r.prop[j] = v;
-    r.sd[j] = A->proposal_sd[(size_t)chain * np + j];
-    r.wmean[j] = 0.0;
-    r.wm2[j] = 0.0;
-  }
-  r.accepts() = 0.0;
-  r.wn() = 0.0;
-  r.scale() = A->init_scale ? A->init_scale[chain] : 1.0;
-  r.batch_acc() = 0.0;
-  eval_params64(A->hoist, A->n_hoist, A->consts, r.prop, U);
-  r.prior_lp() = eval_params64(A->prior, A->n_prior, A->prior_consts, r.prop, nullptr);
-}
-
-__device__ __noinline__ void chain_finish(const ChainArgs* __restrict__ A,
-                                          uint32_t chain, double* st) {
-  const int np = (int)A->n_params;
-  Rec r = make_rec(st, np);
-  const uint32_t T_iter = A->n_iter;
-  A->accept_rate[chain] =
-      T_iter == 0 ? 0.0f : (float)r.accepts() / (float)T_iter;
-  A->last_log_post[chain] = r.cur();
-  if (A->final_state)
-    for (int j = 0; j < np; ++j) A->final_state[(size_t)chain * np + j] = r.state[j];
-  if (A->final_sd)
-    for (int j = 0; j < np; ++j) A->final_sd[(size_t)chain * np + j] = r.sd[j];
-  if (A->final_scale) A->final_scale[chain] = r.scale();
-}
-
-// Streamed tiles live in two static LDS buffers.  Being distinct LDS
-// objects, the compiler can tell that the waves' reads of one buffer do not
-// depend on the asynchronous copy into the other, so the copy overlaps the
-// evaluation instead of being waited for at the first LDS read.
-__shared__ float4 g_tile_a[kTileF4];
-__shared__ float4 g_tile_b[kTileF4];
-
-// Asynchronous tile copy: global_load_lds moves 16 bytes per lane from
-// HBM straight into LDS (no VGPR staging); lane l of wave w writes float4
-// i0 + l where i0 = k*blockDim + 64*w is wave-uniform.  A tile holds a
-// multiple of 64 float4s per column, so a wave's 64 slots are all inside
-// the tile or all outside.  Streaming launches use 1024 threads, so a tile
-// takes at most kTileIssues copies per wave; the loop is unrolled so that
-// every copy in flight keeps its own address registers.
-__device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
-                                               uint32_t ld4, uint32_t n4,
-                                               uint32_t tile4,
-                                               uint32_t wave_base, int lane,
-                                               uint32_t ti, float4* dst) {
-#pragma unroll
-  for (uint32_t k = 0; k < kTileIssues; ++k) {
-    // Slots past the tile repeat its last 64 float4s (identical bytes to
-    // identical addresses), which keeps the sequence free of branches.
-    // n4 is a multiple of 64, so the column is uniform across the wave and
-    // the address is a scalar base plus the lane's 16-byte offset.
-    const uint32_t i0 = min(wave_base + k * 1024u, tile4 - 64u);
-    const uint32_t col = i0 / n4, off0 = i0 - col * n4;
-    const float4* base = src + ((size_t)col * ld4 + (size_t)ti * n4 + off0);
-    __builtin_amdgcn_global_load_lds(
-        (const void*)(base + lane),
-        (__attribute__((address_space(3))) void*)(dst + i0), 16, 0, 0);
-  }
-}
-
-// The chain's share of one tile: chunks part, part + wpc, ... of 64*OPT
-// observations; FP32 sum per chunk, FP64 across chunks.
-template <int NS, int OPT>
-__device__ __forceinline__ double tile_pass(const ChainHot& H,
-                                            const float* __restrict__ tile,
-                                            uint32_t t0, uint32_t part,
-                                            uint32_t wpc, const float* U,
-                                            int lane) {
-  const uint32_t chunk = 64u * OPT;
-  const uint32_t nchunks = H.tile_len / chunk;
-  double acc = 0.0;
-  for (uint32_t ck = part; ck < nchunks; ck += wpc) {
-    const uint32_t base = ck * chunk;
-    float T[OPT];
-    vm_run<NS, OPT>((vm_prog_t)H.vm, H.n_vm, U, tile + base, H.tile_len, lane,
-                    T);
-    const uint32_t first = t0 + base;
-    float s = 0.0f;
-    if (first + chunk <= H.n_obs) {
-#pragma unroll
-      for (int k = 0; k < OPT; ++k) s += T[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < OPT; ++k) {
-        uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
-        s += obs < H.n_obs ? T[k] : 0.0f;
-      }
-    }
-    acc += (double)s;
-  }
-  return acc;
-}
-
-// One workgroup's chains chain0 .. chain0 + cpb - 1 through the whole run.
-template <int NS, int OPT, bool STREAM>
-// H by value: a reference to the kernel argument made the register
-// allocator spill around the tile copies (a wait on the copy itself).
-__device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
-                                           ChainHot H, uint32_t chain0) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int SW = (int)H.state_words;
-  const int cpb = (int)H.cpb, wpc = (int)H.wpc;
-  const int NU = (int)H.n_uniform_pad;
-  double* st_all = reinterpret_cast<double*>(smem);
-  double* partials = st_all + cpb * SW;
-  float* Uall = reinterpret_cast<float*>(partials + cpb * wpc);
-
-  const int tid = (int)threadIdx.x;
-  const int lane = tid & 63;
-  // Wave-uniform indices: readfirstlane tells the compiler so, and the
-  // chunk loops become scalar loops instead of exec-masked ones.
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int c_local = wave / wpc;
-  const int part = wave - c_local * wpc;
-  const uint32_t chain = chain0 + (uint32_t)c_local;
-  const bool active = chain < H.n_chains;
-  const bool lead = active && part == 0 && lane == 0;
-  double* st = st_all + c_local * SW;
-  float* U = Uall + c_local * NU;
-
-  for (int i = tid; i < cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
-  __syncthreads();
-  if (lead) chain_init(Ap, chain, st, U);
-
-  const uint32_t tile_len = H.tile_len;
-  const uint32_t n4 = tile_len / 4;      // float4s per column of a tile
-  const uint32_t tile4 = H.n_cols * n4;  // float4s per tile
-  const uint32_t ld4 = H.ld / 4;
-  const uint32_t n_tiles = H.n_pad / tile_len;
-  const float4* src = reinterpret_cast<const float4*>(H.data);
-  const uint32_t wave_base = (uint32_t)(wave * 64);
-  float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
-
-  // The copy of issue_tile_dma written out here: as a call it raises the
-  // scalar register pressure of the iteration loop.
-  auto issue_tile = [&](uint32_t ti, float4* dst) {
-#pragma unroll
-    for (uint32_t k = 0; k < kTileIssues; ++k) {
-      const uint32_t i0 = min(wave_base + k * 1024u, tile4 - 64u);
-      const uint32_t col = i0 / n4, off0 = i0 - col * n4;
-      const float4* base = src + ((size_t)col * ld4 + (size_t)ti * n4 + off0);
-      __builtin_amdgcn_global_load_lds(
-          (const void*)(base + lane),
-          (__attribute__((address_space(3))) void*)(dst + i0), 16, 0, 0);
-    }
-  };
-
-  if constexpr (!STREAM) {
-    // The whole (padded) data set stays in LDS for every iteration.
-    for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-      uint32_t col = i / n4, off = i - col * n4;
-      res[i] = src[(size_t)col * ld4 + off];
-    }
-  } else {
-    if (H.stream_mode == 1 && H.n_vm > 0) issue_tile(0, g_tile_a);
-  }
-  __syncthreads();
-
-  for (uint32_t it = 0; it <= H.n_iter; ++it) {
-    // ---- data pass for the chain's current proposal -------------------
-    double acc = 0.0;
-    if (H.n_vm > 0) {
-      if constexpr (!STREAM) {
-        if (active)
-          acc = tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(res), 0,
-                                   part, wpc, U, lane);
-      } else if (H.stream_mode == 1) {
-        // Double buffering over an even number of tiles: tile ti is in A
-        // when the pair starts; the copy of ti+1 into B overlaps the pass
-        // over A, and the copy of ti+2 (or tile 0 of the next iteration,
-        // the data never change) into A overlaps the pass over B.
-        for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
-          issue_tile(ti + 1, g_tile_b);
-          if (active)
-            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
-                                      ti * tile_len, part, wpc, U, lane);
-          __syncthreads();
-          if (ti + 2 < n_tiles)
-            issue_tile(ti + 2, g_tile_a);
-          else if (it < H.n_iter)
-            issue_tile(0, g_tile_a);
-          if (active)
-            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_b),
-                                      (ti + 1) * tile_len, part, wpc, U, lane);
-          __syncthreads();
-        }
-      } else {
-        // Rows too wide for two buffers: synchronous copy into A.
-        for (uint32_t ti = 0; ti < n_tiles; ++ti) {
-          __syncthreads();
-          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-            uint32_t col = i / n4, off = i - col * n4;
-            g_tile_a[i] = src[(size_t)col * ld4 + (size_t)ti * n4 + off];
-          }
-          __syncthreads();
-          if (active)
-            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
-                                      ti * tile_len, part, wpc, U, lane);
-        }
-      }
-    }
-    // ---- reduction ----------------------------------------------------
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) partials[wave] = acc;
-    __syncthreads();
-    // ---- fused step ---------------------------------------------------
-    if (lead) {
-      double ll = 0.0;
-      for (int p = 0; p < wpc; ++p) ll += partials[c_local * wpc + p];
-      chain_finish_iter(Ap, it, chain, st, ll);
-    }
-    if (it > 0 && H.swap_every > 0) {
-      const uint32_t b = batch_ending_at(it - 1, H.n_iter, H.swap_every,
-                                         H.n_batches);
-      if (b > 0) {
-        // End of a batch: per-chain adaptation, then the replica-exchange
-        // sweep of the workgroup's ladder groups, before any proposal of
-        // the next batch is drawn.
-        const uint32_t bsize =
-            b == H.n_batches ? H.n_iter - (H.n_batches - 1) * H.swap_every
-                             : H.swap_every;
-        __syncthreads();
-        if (lead) chain_adapt(Ap, b, bsize, chain, st);
-        if (H.swap_group > 1) {
-          __syncthreads();
-          if (tid == 0) pt_sweep(Ap, b, chain0, cpb, st_all, SW);
-        }
-        __syncthreads();
-      }
-    }
-    if (lead && it < H.n_iter) chain_prepare_iter(Ap, it, chain, st, U);
-    __syncthreads();
-  }
-  if (lead) chain_finish(Ap, chain, st);
-}
 
 template <int NS, int OPT, bool STREAM>
 __global__ void __launch_bounds__(1024)
 chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
-  chain_body<NS, OPT, STREAM>(Ap, H, blockIdx.x * H.cpb);
+  chain_body<VmEval<NS, OPT>, STREAM>(Ap, H, blockIdx.x * H.cpb);
 }
 
-// Many models in one launch: workgroup w runs chains work[w].chain0 .. of
-// model work[w].model, with that model's arguments (copied to LDS and
-// patched with this batch's seed, length and draw offset) and hot values.
 template <int NS, int OPT>
 __global__ void __launch_bounds__(1024)
 chain_kernel_batch(const BatchWork* __restrict__ work,
                    const ChainArgs* __restrict__ args,
                    const ChainHot* __restrict__ hots, BatchStep step) {
-  __shared__ ChainArgs a_sh;
-  const BatchWork w = work[blockIdx.x];
-  ChainHot H = hots[w.model];
-  H.n_iter = step.n_iter;
-  if (threadIdx.x == 0) {
-    a_sh = args[w.model];
-    a_sh.seed += step.seed_add;
-    a_sh.n_iter = step.n_iter;
-    a_sh.draws += step.t0;
-    a_sh.draws_ld = step.draws_ld;
-  }
-  __syncthreads();
-  chain_body<NS, OPT, true>(&a_sh, H, w.chain0);
-}
-
-// Data term of a list of parameter points (no chain state).  A persistent
-// grid takes work items from a queue: item (g, s) sums observation slice s
-// for slots g*cpb .. g*cpb+cpb-1, one wave per slot, and writes one FP64
-// partial per (slice, slot); the caller adds the slices in order.  The
-// number of slices is chosen on the device from the active slot count so
-// that about `target_items` items exist whatever the count, and published
-// in *n_slices_out.  The last workgroup to finish resets the queue.
-// The uniform table of slot k's point (lead lane of the slot's wave).
-__device__ __noinline__ void points_hoist(const PointsArgs* __restrict__ P,
-                                          uint32_t k, float* U) {
-  const uint32_t pi = P->index ? P->index[k] : k;
-  eval_params64(P->hoist, P->n_hoist, P->consts,
-                P->points + (size_t)pi * P->n_params, U);
-}
-
-// Item count and slicing of one launch (identical in every workgroup).
-struct PointsPlan {
-  uint32_t count, groups, n_sl, tps, total;
-};
-
-__device__ __noinline__ PointsPlan points_plan(const PointsArgs* __restrict__ P,
-                                               uint32_t cpb, uint32_t n_tiles) {
-  PointsPlan q;
-  q.count = P->count ? *P->count : P->n_points;
-  q.groups = (q.count + cpb - 1) / cpb;
-  uint32_t n_sl = q.groups ? (P->target_items + q.groups - 1) / q.groups : 1u;
-  n_sl = max(1u, min(n_sl, n_tiles));
-  q.tps = (n_tiles + n_sl - 1) / n_sl;
-  q.n_sl = (n_tiles + q.tps - 1) / q.tps;
-  q.total = q.groups * q.n_sl;
-  return q;
+  batch_body<VmEval<NS, OPT>>(work, args, hots, step);
 }
 
 template <int NS, int OPT, bool STREAM>
 __global__ void __launch_bounds__(1024)
 points_kernel(const PointsArgs* __restrict__ P, ChainHot H) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint32_t item_sh;
-  __shared__ PointsPlan plan_sh;
-  const int tid = (int)threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t tile_len = H.tile_len;
-  const uint32_t n_tiles = H.n_pad / tile_len;
-  if (tid == 0) {
-    plan_sh = points_plan(P, H.cpb, n_tiles);
-    if (blockIdx.x == 0) *P->n_slices_out = plan_sh.n_sl;
-  }
-  const int NU = (int)H.n_uniform_pad;
-  float* Uall = reinterpret_cast<float*>(smem);
-  float* U = Uall + wave * NU;
-  bool loaded = false;  // resident layout: data loaded with the first item
-  for (;;) {
-    __syncthreads();  // previous item's reads of U and the tiles are done
-    if (tid == 0) item_sh = atomicAdd(&P->queue[0], 1u);
-    __syncthreads();
-    const uint32_t item = item_sh;
-    if (item >= plan_sh.total) break;  // uniform: every wave leaves here
-    // Slice-major order: the items in flight together read few slices, so
-    // the slot groups of one slice share its tiles in the caches.
-    const uint32_t groups = plan_sh.groups, tps = plan_sh.tps;
-    const uint32_t sl = item / groups, g = item - sl * groups;
-    const uint32_t k = g * H.cpb + (uint32_t)wave;
-    const bool active = k < plan_sh.count;
-    for (int i = tid; i < (int)H.cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
-    __syncthreads();
-    if (active && lane == 0) points_hoist(P, k, U);
-    const uint32_t ti0 = min(sl * tps, n_tiles);
-    const uint32_t ti1 = min(ti0 + tps, n_tiles);
-    const uint32_t n4 = tile_len / 4;
-    const uint32_t tile4 = H.n_cols * n4;
-    const uint32_t ld4 = H.ld / 4;
-    const float4* src = reinterpret_cast<const float4*>(H.data);
-    const uint32_t wave_base = (uint32_t)(wave * 64);
-    double acc = 0.0;
-    if (H.n_vm > 0) {
-      if constexpr (!STREAM) {
-        // One tile holds the whole data set; load it once per workgroup.
-        if (!loaded) {
-          float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
-          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-            uint32_t col = i / n4, off = i - col * n4;
-            res[i] = src[(size_t)col * ld4 + off];
-          }
-          loaded = true;
-        }
-        __syncthreads();
-        if (active && ti0 < ti1)
-          acc = tile_pass<NS, OPT>(
-              H, reinterpret_cast<const float*>(smem + H.lds_tile_offset), 0,
-              0, 1, U, lane);
-      } else if (H.stream_mode == 1) {
-        if (ti0 < ti1)
-          issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti0, g_tile_a);
-        __syncthreads();
-        for (uint32_t ti = ti0; ti < ti1; ti += 2) {
-          const bool two = ti + 1 < ti1;
-          if (two)
-            issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 1, g_tile_b);
-          if (active)
-            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
-                                      ti * tile_len, 0, 1, U, lane);
-          __syncthreads();
-          if (ti + 2 < ti1)
-            issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 2, g_tile_a);
-          if (two && active)
-            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_b),
-                                      (ti + 1) * tile_len, 0, 1, U, lane);
-          __syncthreads();
-        }
-      } else {
-        for (uint32_t ti = ti0; ti < ti1; ++ti) {
-          __syncthreads();
-          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-            uint32_t col = i / n4, off = i - col * n4;
-            g_tile_a[i] = src[(size_t)col * ld4 + (size_t)ti * n4 + off];
-          }
-          __syncthreads();
-          if (active)
-            acc += tile_pass<NS, OPT>(H, reinterpret_cast<const float*>(g_tile_a),
-                                      ti * tile_len, 0, 1, U, lane);
-        }
-      }
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (active && lane == 0) P->partial[(size_t)sl * P->n_points + k] = acc;
-  }
-  // The last workgroup out leaves the queue at zero for the next launch.
-  if (tid == 0) {
-    __threadfence();
-    if (atomicAdd(&P->queue[1], 1u) == gridDim.x - 1) {
-      P->queue[0] = 0u;
-      P->queue[1] = 0u;
-      __threadfence();
-    }
-  }
+  points_body<VmEval<NS, OPT>, STREAM>(P, H);
 }
 
 #define GPUM_INST(NS, OPT)                                                   \

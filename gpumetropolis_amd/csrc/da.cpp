@@ -9,6 +9,7 @@
 
 #include "../../include/gpum_hip.h"
 #include "da.h"
+#include "jit.h"
 #include "mlp.h"
 #include "philox.h"
 #include "runtime.h"
@@ -613,7 +614,17 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   if (!d_pinit.upload(&pinit, 1, "upload launch arguments") ||
       !d_pa.upload(&pa, 1, "upload launch arguments"))
     return 1;
-  chk(launch_points_kernel(d_pinit.p, n_wg, h, t.v.ns, t.v.opt, t.lds_bytes, st));
+  bool jit_failed = false;
+  hipFunction_t jf = model_jit_kernel(
+      m, t.stream_mode ? JIT_POINTS_STREAM : JIT_POINTS_RESIDENT,
+      t.lds_bytes + kPointsExtraLds, &jit_failed);
+  if (jit_failed) return 1;
+  auto points = [&](const PointsArgs* p) -> hipError_t {
+    if (!jf) return launch_points_kernel(p, n_wg, h, t.v.ns, t.v.opt, t.lds_bytes, st);
+    void* kargs[] = {(void*)&p, (void*)&h};
+    return launch_module(jf, dim3(n_wg), dim3(h.cpb * 64), t.lds_bytes, st, kargs);
+  };
+  chk(points(d_pinit.p));
   chk(launch_da_init_finish(d_args.p, C, st));
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -627,7 +638,7 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
       return 1;
     }
     chk(launch_da_stage1(d_args.p, it, st));
-    chk(launch_points_kernel(d_pa.p, n_wg, h, t.v.ns, t.v.opt, t.lds_bytes, st));
+    chk(points(d_pa.p));
     chk(launch_da_stage2(d_args.p, C, it, st));
   }
   hipEventRecord(e1, st);

@@ -139,6 +139,142 @@ __device__ __forceinline__ VmIns4 vm_fetch(vm_prog_t prog, uint32_t pc) {
   return r;
 }
 
+// One VM instruction on one chunk.  The interpreter calls it with the
+// fetched opcode; code generated per model calls it with constants, so the
+// switch folds away and each program becomes straight-line code.
+template <int NS, int OPT>
+__device__ __forceinline__ void vm_step(uint32_t op, uint32_t slot, uint32_t a,
+                                        uint32_t b, const float* __restrict__ U,
+                                        const float* __restrict__ tile,
+                                        uint32_t ld, int lane, float (&T)[OPT],
+                                        float (&S)[NS > 0 ? NS : 1][OPT],
+                                        float (&D)[OPT]) {
+  switch (op) {
+    case VM_LOAD_U: {
+      float u = U[a];
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) T[k] = u;
+      break;
+    }
+    case VM_LOAD_D:
+      vm_load_col<OPT>(tile + a * ld, lane, T);
+      break;
+    case VM_PUSH_U:
+      if constexpr (NS > 0) {
+        vm_spill<0, NS, OPT>(slot, S, T);
+        float u = U[a];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[k] = u;
+      }
+      break;
+    case VM_PUSH_D:
+      if constexpr (NS > 0) {
+        vm_spill<0, NS, OPT>(slot, S, T);
+        vm_load_col<OPT>(tile + a * ld, lane, T);
+      }
+      break;
+    case VM_ADD_S:
+      if constexpr (NS > 0)
+        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s + t; });
+      break;
+    case VM_SUB_S:
+      if constexpr (NS > 0)
+        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s - t; });
+      break;
+    case VM_MUL_S:
+      if constexpr (NS > 0)
+        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s * t; });
+      break;
+    case VM_DIV_S:
+      if constexpr (NS > 0)
+        vm_bin_s<0, NS, OPT>(slot, S, T,
+                             [](float s, float t) { return s * vm_rcp(t); });
+      break;
+    case VM_POW_S:
+      if constexpr (NS > 0)
+        vm_bin_s<0, NS, OPT>(slot, S, T,
+                             [](float s, float t) { return vm_pow(s, t); });
+      break;
+#define GPUM_UOP(NAME, EXPR)                          \
+case NAME: {                                        \
+  float u = U[a];                                   \
+  _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
+    float t = T[k];                                 \
+    T[k] = (EXPR);                                  \
+  }                                                 \
+  break;                                            \
+}
+    GPUM_UOP(VM_ADD_U, t + u)
+    GPUM_UOP(VM_SUB_U, t - u)
+    GPUM_UOP(VM_MUL_U, t * u)
+    GPUM_UOP(VM_POW_U, vm_pow(t, u))
+    GPUM_UOP(VM_RSUB_U, u - t)
+    GPUM_UOP(VM_RDIV_U, u * vm_rcp(t))
+    GPUM_UOP(VM_RPOW_U, vm_pow(u, t))
+#undef GPUM_UOP
+#define GPUM_DOP(NAME, EXPR)                          \
+case NAME: {                                        \
+  vm_load_col<OPT>(tile + a * ld, lane, D);         \
+  _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
+    float t = T[k];                                 \
+    float d = D[k];                                 \
+    T[k] = (EXPR);                                  \
+  }                                                 \
+  break;                                            \
+}
+    GPUM_DOP(VM_ADD_D, t + d)
+    GPUM_DOP(VM_SUB_D, t - d)
+    GPUM_DOP(VM_MUL_D, t * d)
+    GPUM_DOP(VM_DIV_D, t * vm_rcp(d))
+    GPUM_DOP(VM_POW_D, vm_pow(t, d))
+    GPUM_DOP(VM_RSUB_D, d - t)
+    GPUM_DOP(VM_RDIV_D, d * vm_rcp(t))
+    GPUM_DOP(VM_RPOW_D, vm_pow(d, t))
+#undef GPUM_DOP
+#define GPUM_UNOP(NAME, EXPR)                         \
+case NAME: {                                        \
+  _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
+    float t = T[k];                                 \
+    T[k] = (EXPR);                                  \
+  }                                                 \
+  break;                                            \
+}
+    GPUM_UNOP(VM_NEG, -t)
+    GPUM_UNOP(VM_EXP, vm_exp(t))
+    GPUM_UNOP(VM_LOG, vm_log(t))
+    GPUM_UNOP(VM_SQRT, vm_sqrt(t))
+    GPUM_UNOP(VM_LGAMMA, vm_lgamma(t))
+    GPUM_UNOP(VM_SQR, t * t)
+    GPUM_UNOP(VM_CUBE, t * t * t)
+    GPUM_UNOP(VM_POW4, (t * t) * (t * t))
+    GPUM_UNOP(VM_RCP, vm_rcp(t))
+    GPUM_UNOP(VM_SOFTPLUS, vm_softplus(t))
+#undef GPUM_UNOP
+    case VM_FMA_UU: {
+      float ua = U[a], ub = U[b];
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], ua, ub);
+      break;
+    }
+    case VM_FMA_UD: {
+      float ua = U[a];
+      vm_load_col<OPT>(tile + b * ld, lane, D);
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) T[k] = fmaf(ua, D[k], T[k]);
+      break;
+    }
+    case VM_FMA_DU: {
+      float ub = U[b];
+      vm_load_col<OPT>(tile + a * ld, lane, D);
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], D[k], ub);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
 template <int NS, int OPT>
 __device__ __forceinline__ void vm_run(vm_prog_t prog,
                                        uint32_t n_instr,
@@ -158,130 +294,7 @@ __device__ __forceinline__ void vm_run(vm_prog_t prog,
   for (uint32_t pc = 0; pc < n_instr; ++pc) {
     const uint32_t op = nxt.op, slot = nxt.slot, a = nxt.a, b = nxt.b;
     nxt = vm_fetch(prog, pc + 1);
-    switch (op) {
-      case VM_LOAD_U: {
-        float u = U[a];
-#pragma unroll
-        for (int k = 0; k < OPT; ++k) T[k] = u;
-        break;
-      }
-      case VM_LOAD_D:
-        vm_load_col<OPT>(tile + a * ld, lane, T);
-        break;
-      case VM_PUSH_U:
-        if constexpr (NS > 0) {
-          vm_spill<0, NS, OPT>(slot, S, T);
-          float u = U[a];
-#pragma unroll
-          for (int k = 0; k < OPT; ++k) T[k] = u;
-        }
-        break;
-      case VM_PUSH_D:
-        if constexpr (NS > 0) {
-          vm_spill<0, NS, OPT>(slot, S, T);
-          vm_load_col<OPT>(tile + a * ld, lane, T);
-        }
-        break;
-      case VM_ADD_S:
-        if constexpr (NS > 0)
-          vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s + t; });
-        break;
-      case VM_SUB_S:
-        if constexpr (NS > 0)
-          vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s - t; });
-        break;
-      case VM_MUL_S:
-        if constexpr (NS > 0)
-          vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s * t; });
-        break;
-      case VM_DIV_S:
-        if constexpr (NS > 0)
-          vm_bin_s<0, NS, OPT>(slot, S, T,
-                               [](float s, float t) { return s * vm_rcp(t); });
-        break;
-      case VM_POW_S:
-        if constexpr (NS > 0)
-          vm_bin_s<0, NS, OPT>(slot, S, T,
-                               [](float s, float t) { return vm_pow(s, t); });
-        break;
-#define GPUM_UOP(NAME, EXPR)                          \
-  case NAME: {                                        \
-    float u = U[a];                                   \
-    _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
-      float t = T[k];                                 \
-      T[k] = (EXPR);                                  \
-    }                                                 \
-    break;                                            \
-  }
-      GPUM_UOP(VM_ADD_U, t + u)
-      GPUM_UOP(VM_SUB_U, t - u)
-      GPUM_UOP(VM_MUL_U, t * u)
-      GPUM_UOP(VM_POW_U, vm_pow(t, u))
-      GPUM_UOP(VM_RSUB_U, u - t)
-      GPUM_UOP(VM_RDIV_U, u * vm_rcp(t))
-      GPUM_UOP(VM_RPOW_U, vm_pow(u, t))
-#undef GPUM_UOP
-#define GPUM_DOP(NAME, EXPR)                          \
-  case NAME: {                                        \
-    vm_load_col<OPT>(tile + a * ld, lane, D);         \
-    _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
-      float t = T[k];                                 \
-      float d = D[k];                                 \
-      T[k] = (EXPR);                                  \
-    }                                                 \
-    break;                                            \
-  }
-      GPUM_DOP(VM_ADD_D, t + d)
-      GPUM_DOP(VM_SUB_D, t - d)
-      GPUM_DOP(VM_MUL_D, t * d)
-      GPUM_DOP(VM_DIV_D, t * vm_rcp(d))
-      GPUM_DOP(VM_POW_D, vm_pow(t, d))
-      GPUM_DOP(VM_RSUB_D, d - t)
-      GPUM_DOP(VM_RDIV_D, d * vm_rcp(t))
-      GPUM_DOP(VM_RPOW_D, vm_pow(d, t))
-#undef GPUM_DOP
-#define GPUM_UNOP(NAME, EXPR)                         \
-  case NAME: {                                        \
-    _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
-      float t = T[k];                                 \
-      T[k] = (EXPR);                                  \
-    }                                                 \
-    break;                                            \
-  }
-      GPUM_UNOP(VM_NEG, -t)
-      GPUM_UNOP(VM_EXP, vm_exp(t))
-      GPUM_UNOP(VM_LOG, vm_log(t))
-      GPUM_UNOP(VM_SQRT, vm_sqrt(t))
-      GPUM_UNOP(VM_LGAMMA, vm_lgamma(t))
-      GPUM_UNOP(VM_SQR, t * t)
-      GPUM_UNOP(VM_CUBE, t * t * t)
-      GPUM_UNOP(VM_POW4, (t * t) * (t * t))
-      GPUM_UNOP(VM_RCP, vm_rcp(t))
-      GPUM_UNOP(VM_SOFTPLUS, vm_softplus(t))
-#undef GPUM_UNOP
-      case VM_FMA_UU: {
-        float ua = U[a], ub = U[b];
-#pragma unroll
-        for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], ua, ub);
-        break;
-      }
-      case VM_FMA_UD: {
-        float ua = U[a];
-        vm_load_col<OPT>(tile + b * ld, lane, D);
-#pragma unroll
-        for (int k = 0; k < OPT; ++k) T[k] = fmaf(ua, D[k], T[k]);
-        break;
-      }
-      case VM_FMA_DU: {
-        float ub = U[b];
-        vm_load_col<OPT>(tile + a * ld, lane, D);
-#pragma unroll
-        for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], D[k], ub);
-        break;
-      }
-      default:
-        break;
-    }
+    vm_step<NS, OPT>(op, slot, a, b, U, tile, ld, lane, T, S, D);
   }
 }
 

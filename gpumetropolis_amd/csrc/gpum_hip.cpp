@@ -20,6 +20,7 @@
 
 #include "../../include/gpum_hip.h"
 #include "chain_kernel.h"
+#include "jit.h"
 #include "lower.h"
 #include "runtime.h"
 
@@ -331,6 +332,25 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   return true;
 }
 
+hipFunction_t model_jit_kernel(const gpum_hip_model* m, int kind,
+                               size_t lds_bytes, bool* failed) {
+  *failed = false;
+  if (m->jit_eval.empty() || lds_bytes > kJitMaxDynamicLds) return nullptr;
+  std::string err;
+  hipFunction_t f = jit_kernel(m->jit_eval, (JitKind)kind, &err);
+  if (!f) {
+    set_err("%s", err.c_str());
+    *failed = true;
+  }
+  return f;
+}
+
+hipError_t launch_module(hipFunction_t f, dim3 grid, dim3 block,
+                         size_t lds_bytes, hipStream_t s, void** args) {
+  return hipModuleLaunchKernel(f, grid.x, grid.y, grid.z, block.x, block.y,
+                               block.z, (unsigned)lds_bytes, s, args, nullptr);
+}
+
 bool ensure_data(gpum_hip_model* m, uint32_t n_pad) {
   if (m->data.p && m->data_ld == n_pad) return true;
   const uint32_t ncol = std::max(1u, m->n_cols_total);
@@ -425,6 +445,34 @@ gpum_hip_model* gpum_hip_model_create(
     return nullptr;
   }
   return m;
+}
+
+int gpum_hip_model_set_jit(gpum_hip_model* m, int32_t enable,
+                           double* compile_ms) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!m) {
+    set_err("null model");
+    return 1;
+  }
+  if (!device_init()) return 1;
+  hipSetDevice(g_dev.id);
+  if (compile_ms) *compile_ms = 0.0;
+  if (!enable || m->L.empty) {
+    m->jit_eval.clear();
+    return 0;
+  }
+  const Variant v = pick_variant(m->L.vm_depth);
+  std::string src = jit_evaluator_source(m->L, v.ns, v.opt);
+  std::string err;
+  double ms = 0.0;
+  // Compile the streamed chain kernel now; the other kinds on first use.
+  if (!jit_kernel(src, JIT_CHAIN_STREAM, &err, &ms)) {
+    set_err("%s", err.c_str());
+    return 1;
+  }
+  m->jit_eval = src;
+  if (compile_ms) *compile_ms = ms;
+  return 0;
 }
 
 void gpum_hip_model_free(gpum_hip_model* m) {
@@ -618,9 +666,22 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
+  bool jit_failed = false;
+  hipFunction_t jf = model_jit_kernel(
+      m, t.stream_mode ? JIT_CHAIN_STREAM : JIT_CHAIN_RESIDENT, t.lds_bytes,
+      &jit_failed);
+  if (jit_failed) return 1;
   hipEventRecord(e0, g_dev.stream);
-  hipError_t e = launch_chain_kernel(d_args.p, h, t.v.ns, t.v.opt, t.lds_bytes,
-                                     g_dev.stream);
+  hipError_t e;
+  if (jf) {
+    const ChainArgs* ap = d_args.p;
+    void* kargs[] = {(void*)&ap, (void*)&h};
+    e = launch_module(jf, dim3((h.n_chains + h.cpb - 1) / h.cpb),
+                      dim3(h.cpb * h.wpc * 64), t.lds_bytes, g_dev.stream, kargs);
+  } else {
+    e = launch_chain_kernel(d_args.p, h, t.v.ns, t.v.opt, t.lds_bytes,
+                            g_dev.stream);
+  }
   hipEventRecord(e1, g_dev.stream);
   if (e != hipSuccess) {
     hipEventDestroy(e0);

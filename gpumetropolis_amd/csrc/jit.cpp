@@ -1,0 +1,179 @@
+// jit.cpp -- compile per-model kernels with hiprtc and cache them.
+#include "jit.h"
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_version.h>
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <vector>
+
+#include "lower.h"
+#include "vm.h"
+
+// kJitHeaderNames / kJitHeaderSources / kJitHeaderCount: the device headers
+// the kernels are built from, embedded at build time.
+#include "jit_sources.inc"
+
+namespace gpum {
+namespace {
+
+// hiprtc predefines the HIP runtime and the device math functions but ships
+// no system headers; these stand in for the three the device headers name.
+const char* kStubRuntime = "#pragma once\n";
+const char* kStubMath = "#pragma once\n";
+const char* kStubStdint =
+    "#pragma once\n"
+    "typedef __hip_internal::uint8_t uint8_t;\n"
+    "typedef __hip_internal::uint16_t uint16_t;\n"
+    "typedef __hip_internal::uint32_t uint32_t;\n"
+    "typedef __hip_internal::uint64_t uint64_t;\n"
+    "typedef __hip_internal::int8_t int8_t;\n"
+    "typedef __hip_internal::int16_t int16_t;\n"
+    "typedef __hip_internal::int32_t int32_t;\n"
+    "typedef __hip_internal::int64_t int64_t;\n";
+
+const char* kKernels[JIT_NUM_KINDS] = {
+    "extern \"C\" __global__ void __launch_bounds__(1024)\n"
+    "gpum_jit_kernel(const gpum::ChainArgs* A, gpum::ChainHot H) {\n"
+    "  gpum::chain_body<gpum::JitEval, true>(A, H, blockIdx.x * H.cpb);\n}\n",
+    "extern \"C\" __global__ void __launch_bounds__(1024)\n"
+    "gpum_jit_kernel(const gpum::ChainArgs* A, gpum::ChainHot H) {\n"
+    "  gpum::chain_body<gpum::JitEval, false>(A, H, blockIdx.x * H.cpb);\n}\n",
+    "extern \"C\" __global__ void __launch_bounds__(1024)\n"
+    "gpum_jit_kernel(const gpum::PointsArgs* P, gpum::ChainHot H) {\n"
+    "  gpum::points_body<gpum::JitEval, true>(P, H);\n}\n",
+    "extern \"C\" __global__ void __launch_bounds__(1024)\n"
+    "gpum_jit_kernel(const gpum::PointsArgs* P, gpum::ChainHot H) {\n"
+    "  gpum::points_body<gpum::JitEval, false>(P, H);\n}\n",
+    "extern \"C\" __global__ void __launch_bounds__(1024)\n"
+    "gpum_jit_kernel(const gpum::BatchWork* w, const gpum::ChainArgs* a,\n"
+    "                const gpum::ChainHot* h, gpum::BatchStep s) {\n"
+    "  gpum::batch_body<gpum::JitEval>(w, a, h, s);\n}\n",
+};
+
+struct Entry {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+
+std::mutex g_jit_mu;
+std::map<std::string, Entry> g_cache;  // by full source text
+
+bool compile(const std::string& src, Entry* out, std::string* err,
+             double* ms) {
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<const char*> names, texts;
+  names.push_back("hip/hip_runtime.h");
+  texts.push_back(kStubRuntime);
+  names.push_back("math.h");
+  texts.push_back(kStubMath);
+  names.push_back("stdint.h");
+  texts.push_back(kStubStdint);
+  for (int i = 0; i < kJitHeaderCount; ++i) {
+    names.push_back(kJitHeaderNames[i]);
+    texts.push_back(kJitHeaderSources[i]);
+  }
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "gpum_jit.hip", (int)names.size(),
+                          texts.data(), names.data()) != HIPRTC_SUCCESS) {
+    *err = "hiprtc: cannot create the program";
+    return false;
+  }
+  // The library's own code-generation flags (build.py).
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                        "-ffp-contract=on", "-Xclang", "-target-feature",
+                        "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof *opts), opts);
+  if (r != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    *err = std::string("hiprtc compile failed: ") + hiprtcGetErrorString(r) +
+           "\n" + log.substr(0, 2000);
+    return false;
+  }
+  size_t n = 0;
+  hiprtcGetCodeSize(prog, &n);
+  std::vector<char> code(n);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  hipError_t e = hipModuleLoadData(&out->mod, code.data());
+  if (e == hipSuccess) e = hipModuleGetFunction(&out->fn, out->mod, "gpum_jit_kernel");
+  if (e != hipSuccess) {
+    *err = std::string("load compiled kernel: kernel/device error (") +
+           hipGetErrorString(e) + ")";
+    return false;
+  }
+  if (ms)
+    *ms = std::chrono::duration<double, std::milli>(
+              std::chrono::steady_clock::now() - t0).count();
+  return true;
+}
+
+// hiprtc writes code objects for the ROCm release this library is built
+// against; a process whose HIP runtime came from another release (a
+// framework that bundles its own runtime, loaded first) may reject them, so
+// the loaded runtime must be the release the library was compiled with.
+bool runtime_matches(std::string* why) {
+  int rt = 0;
+  if (hipRuntimeGetVersion(&rt) != hipSuccess) {
+    *why = "cannot query the HIP runtime version";
+    return false;
+  }
+  const int rt_major = rt / 10000000, rt_minor = (rt / 100000) % 100;
+  if (rt_major != HIP_VERSION_MAJOR || rt_minor != HIP_VERSION_MINOR) {
+    std::ostringstream o;
+    o << "compiled kernels need the HIP runtime this library was built with ("
+      << HIP_VERSION_MAJOR << "." << HIP_VERSION_MINOR << "); runtime "
+      << rt_major << "." << rt_minor << " is loaded in this process (e.g. "
+         "bundled by another library imported first); use the interpreter";
+    *why = o.str();
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+std::string jit_evaluator_source(const Lowered& L, int ns, int opt) {
+  std::ostringstream o;
+  o << "#include \"chain_body.h\"\n"
+    << "namespace gpum {\n"
+    << "struct JitEval {\n"
+    << "  static constexpr int kOpt = " << opt << ";\n"
+    << "  static __device__ __forceinline__ void run(const ChainHot& H,\n"
+    << "      const float* __restrict__ U, const float* __restrict__ tile,\n"
+    << "      uint32_t ld, int lane, float (&T)[" << opt << "]) {\n"
+    << "    float S[" << (ns > 0 ? ns : 1) << "][" << opt << "];\n"
+    << "    float D[" << opt << "];\n";
+  for (const VmInstr& ins : L.vm)
+    o << "    vm_step<" << ns << ", " << opt << ">(" << vm_op(ins) << "u, "
+      << vm_slot(ins) << "u, " << vm_a(ins) << "u, " << vm_b(ins)
+      << "u, U, tile, ld, lane, T, S, D);\n";
+  o << "  }\n};\n}  // namespace gpum\n";
+  return o.str();
+}
+
+hipFunction_t jit_kernel(const std::string& evaluator, JitKind kind,
+                         std::string* err, double* compile_ms) {
+  const std::string src = evaluator + kKernels[kind];
+  std::lock_guard<std::mutex> lk(g_jit_mu);
+  auto it = g_cache.find(src);
+  if (it != g_cache.end()) {
+    if (compile_ms) *compile_ms = 0.0;
+    return it->second.fn;
+  }
+  if (!runtime_matches(err)) return nullptr;
+  Entry e;
+  if (!compile(src, &e, err, compile_ms)) return nullptr;
+  g_cache[src] = e;
+  return e.fn;
+}
+
+}  // namespace gpum

@@ -1,0 +1,33 @@
+// jit.h -- per-model kernels compiled at run time (hiprtc).
+//
+// The reference compiles each model's log-likelihood to native code for its
+// CPU path (src/rust/src/jit.rs:441-631, Cranelift).  This is the same idea
+// on the GPU: the lowered VM program becomes a straight-line evaluator
+// (the interpreter's own vm_step calls with constant operands, so the
+// arithmetic is the interpreter's, operation for operation) instantiated
+// into the chain, points and batched kernel bodies.  Modules are cached per
+// process by source text, so models sharing a program share kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+namespace gpum {
+
+struct Lowered;
+
+enum JitKind { JIT_CHAIN_STREAM = 0, JIT_CHAIN_RESIDENT, JIT_POINTS_STREAM,
+               JIT_POINTS_RESIDENT, JIT_BATCH, JIT_NUM_KINDS };
+
+// Source of the evaluator for a lowered program run with variant (ns, opt).
+std::string jit_evaluator_source(const Lowered& L, int ns, int opt);
+
+// The kernel of `kind` for an evaluator source (compiled on first use).
+// On failure returns nullptr and fills *err.
+hipFunction_t jit_kernel(const std::string& evaluator, JitKind kind,
+                         std::string* err, double* compile_ms = nullptr);
+
+// Dynamic LDS a module kernel may use without an attribute call.
+constexpr size_t kJitMaxDynamicLds = 65536;
+
+}  // namespace gpum

@@ -109,6 +109,8 @@ struct gpum_hip_model {
   gpum::rt::DevBuf<float> data;
   uint32_t data_ld = 0;
   uint32_t data_chunk = 0;  // layout granularity the data was padded to
+  // Per-model compiled evaluator (empty: the interpreter runs).
+  std::string jit_eval;
 };
 
 namespace gpum {
@@ -129,6 +131,14 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
                  const TilingForce* force = nullptr);
 // Lay the model's columns out on the device with column stride n_pad.
 bool ensure_data(gpum_hip_model* m, uint32_t n_pad);
+
+// The model's compiled kernel of `kind` when it has one and the launch can
+// use it (module kernels get at most 64 KB of dynamic LDS), else nullptr;
+// sets *failed (and the error) when compilation fails.
+hipFunction_t model_jit_kernel(const gpum_hip_model* m, int kind,
+                               size_t lds_bytes, bool* failed);
+hipError_t launch_module(hipFunction_t f, dim3 grid, dim3 block,
+                         size_t lds_bytes, hipStream_t s, void** args);
 
 }  // namespace rt
 }  // namespace gpum

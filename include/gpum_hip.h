@@ -186,6 +186,17 @@ int gpum_hip_model_run(gpum_hip_model* m, const double* init,
 int gpum_hip_model_loglik(gpum_hip_model* m, const double* points,
                           uint32_t n_points, double* out);
 
+/* Run this model on kernels compiled for its program (enable = 1) or on
+ * the interpreter (0, the default).  The lowered program becomes
+ * straight-line code calling the interpreter's own per-instruction
+ * arithmetic, compiled at run time for gfx950 (hiprtc) and cached per
+ * process by program; results equal the interpreter's.  No counterpart on
+ * the reference's GPU path; its CPU path compiles each model the same way
+ * (src/rust/src/jit.rs:441-631).  compile_ms (may be NULL): compile time of
+ * the first kernel, 0 when cached. */
+int gpum_hip_model_set_jit(gpum_hip_model* m, int32_t enable,
+                           double* compile_ms);
+
 void gpum_hip_model_free(gpum_hip_model* m);
 
 /* ---- Surrogate delayed acceptance -------------------------------------

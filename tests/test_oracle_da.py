@@ -9,12 +9,12 @@ import math
 
 import numpy as np
 import pytest
-import torch
 
 from oracle import mlp
 
 
 def test_bf16_matches_torch_round_to_nearest_even():
+    import torch  # imported here: GPU test runs never load its runtime
     rng = np.random.default_rng(0)
     x = np.concatenate([
         rng.standard_normal(20000) * 10.0 ** rng.integers(-30, 30, 20000),
@@ -42,6 +42,7 @@ def _net(seed=0, scale=1.0):
 
 
 def test_exact_network_matches_torch_autograd():
+    import torch
     w = _net(1)
     rng = np.random.default_rng(2)
     X = rng.standard_normal((37, 16))
