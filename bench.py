@@ -47,9 +47,12 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target duration of the CPU-baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--jit", action="store_true",
-                   help="run every model on its compiled kernels instead of "
-                        "the interpreter")
+    p.add_argument("--no-jit", dest="jit", action="store_false",
+                   help="run every model on the interpreter instead of its "
+                        "compiled kernels")
+    p.add_argument("--vm-launches", type=int, default=2,
+                   help="interpreter launches timed beside the compiled "
+                        "headline (0: skip)")
     p.add_argument("--no-da", action="store_true",
                    help="skip the surrogate delayed-acceptance comparison")
     p.add_argument("--da-obs", type=int, default=4_000_000)
@@ -410,12 +413,29 @@ def main():
             "traffic": None,
             "w_exec": w_exec,
             "w_bytecode": info["w_bytecode"],
-            "kernel": "gpum::chain_kernel",
+            "kernel": ("gpum_jit_kernel (chain_body, compiled gamma program)"
+                       if a.jit else "gpum::chain_kernel<0,16,true>"),
             "avg_launch_ms": kms.mean(),
             "stream_bytes_per_launch": bytes_per_launch,
             "stream_GBps": bytes_per_launch / avg_launch_s / 1e9,
         },
     }
+    if a.jit and a.vm_launches > 0:
+        # The same launches on the interpreter (identical draws), for the
+        # record of what compiling the model buys.
+        dm.set_jit(False)
+        vm_ms = []
+        for k in range(a.vm_launches):
+            r = dm.run(wu.init, wu.sd, np.ones(C), a.iters_per_step,
+                       a.seed + wu.batches_run + k, proposal_mode=mode,
+                       proposal_l=wu.L, final_state=True)
+            vm_ms.append(r.kernel_ms)
+        vm_avg = float(np.mean(vm_ms))
+        out["interpreter"] = {
+            "avg_launch_ms": vm_avg,
+            "obs_evals_per_s": evals_per_launch / (vm_avg / 1e3),
+            "valu_frac": w_exec * evals_per_launch / (vm_avg / 1e3) / VALU_PEAK}
+    out["engine"] = "compiled per model" if a.jit else "interpreter"
     dm.close()
     if not a.no_da:
         out["surrogate_da"] = da_section(a)
