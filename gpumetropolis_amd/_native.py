@@ -47,7 +47,8 @@ class RunExt(C.Structure):
                 ("acc_hist", C.POINTER(C.c_float)),
                 ("final_sd", C.POINTER(C.c_double)),
                 ("final_scale", C.POINTER(C.c_double)),
-                ("trace_ll", C.POINTER(C.c_double))]
+                ("trace_ll", C.POINTER(C.c_double)),
+                ("chains_per_wave", C.c_int32)]
 
 
 class DaExt(C.Structure):
@@ -61,7 +62,8 @@ class DaExt(C.Structure):
                 ("trace_stage", C.POINTER(C.c_uint8)),
                 ("kernel_ms", C.c_double), ("exact_evals", C.c_double),
                 ("trace_prop", C.POINTER(C.c_double)),
-                ("trace_ll", C.POINTER(C.c_double))]
+                ("trace_ll", C.POINTER(C.c_double)),
+                ("chains_per_wave", C.c_uint32)]
 
 
 class BatchModel(C.Structure):
@@ -83,7 +85,8 @@ class BatchOpts(C.Structure):
                 ("warmup_batches", C.c_uint32), ("adapt", C.c_int32),
                 ("chains_per_block", C.c_int32),
                 ("waves_per_chain", C.c_int32), ("use_graph", C.c_int32),
-                ("kernel_ms", C.c_double), ("n_workgroups", C.c_uint32)]
+                ("kernel_ms", C.c_double), ("n_workgroups", C.c_uint32),
+                ("chains_per_wave", C.c_int32)]
 
 
 class TrainOpts(C.Structure):
@@ -98,7 +101,8 @@ class ModelInfo(C.Structure):
     _fields_ = [("n_vm_instr", C.c_uint32), ("vm_depth", C.c_uint32),
                 ("ref_depth", C.c_uint32), ("n_uniform", C.c_uint32),
                 ("n_cols_total", C.c_uint32), ("n_hoist_words", C.c_uint32),
-                ("w_exec", C.c_double), ("w_bytecode", C.c_double)]
+                ("w_exec", C.c_double), ("w_bytecode", C.c_double),
+                ("sum_mode", C.c_uint32), ("n_sum_tail", C.c_uint32)]
 
 
 _u32p = C.POINTER(C.c_uint32)
@@ -292,7 +296,10 @@ def _ext(n_iter, n_chains, n_params, trace, final_state, stream_id, tiling,
         keep["final"] = np.zeros(n_chains * n_params, np.float64)
         ext.final_state = _p(keep["final"], C.c_double)
     if tiling:
-        ext.chains_per_block, ext.waves_per_chain = tiling
+        # (chains per block, waves per chain[, chains per wave])
+        ext.chains_per_block, ext.waves_per_chain = tiling[:2]
+        if len(tiling) > 2:
+            ext.chains_per_wave = tiling[2]
     ext.precompute_data = -1 if precompute_data is None else int(precompute_data)
     ext.fuse = -1 if fuse is None else int(fuse)
     return ext, keep
@@ -426,7 +433,7 @@ class DeviceModel:
     def run_da(self, surrogate, init, temperatures, n_iter, seed,
                proposal_sd=None, proposal_l=None, trace=False,
                final_state=False, stream_id=0, iter_offset=0,
-               work_items=0, work_groups=0) -> RunOutput:
+               work_items=0, work_groups=0, chains_per_wave=0) -> RunOutput:
         """Stepwise random-walk Metropolis; with `surrogate` (a Surrogate)
         two-stage delayed acceptance, without it plain Metropolis with the
         uniforms of `run`.  Mode 2 when proposal_l is given, else mode 0."""
@@ -442,6 +449,7 @@ class DeviceModel:
         ext.iter_offset = iter_offset
         ext.work_items = work_items
         ext.work_groups = work_groups
+        ext.chains_per_wave = chains_per_wave
         keep = {"rate": np.zeros(n_chains, np.float64)}
         ext.stage1_rate = _p(keep["rate"], C.c_double)
         if final_state:
@@ -622,7 +630,10 @@ def batch_run(models, n_chains, inits, sds, seeds, stream_ids, gammas,
     opts.warmup_batches = warmup_batches
     opts.adapt = 1 if adapt else 0
     if tiling:
-        opts.chains_per_block, opts.waves_per_chain = tiling
+        # (chains per block, waves per chain[, chains per wave])
+        opts.chains_per_block, opts.waves_per_chain = tiling[:2]
+        if len(tiling) > 2:
+            opts.chains_per_wave = tiling[2]
     opts.use_graph = 1 if use_graph else 0
     _check(L.gpum_hip_batch_run(arr, k, C.byref(opts)))
     for i, o in enumerate(outs):

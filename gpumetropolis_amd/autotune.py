@@ -1,15 +1,18 @@
 """Tile autotuner for batched launches.
 
 A batched launch gives every workgroup 16 waves, split as chains per block
-x waves per chain.  Many chains per block share each data tile among more
-chains (less data traffic per evaluation); more waves per chain shorten a
-chain's pass over a long data set and keep more workgroups busy when the
-batch has few chains.  Which wins depends on the program (its length sets
-the arithmetic per tile) and on the observation counts, so the choice is
-measured: each candidate runs one short batch of the real workload, the
-fastest is kept, and later batches with the same key reuse it.  The key is
-(longest program in VM instructions, median and largest observation count
-rounded to powers of two).
+x waves per chain, with one or two chains per wave.  Many chains per block
+share each data tile among more chains (less data traffic per evaluation);
+more waves per chain shorten a chain's pass over a long data set and keep
+more workgroups busy when the batch has few chains; two chains per wave
+(programs without a stack only) read each observation from LDS once for
+both.  Which wins depends on the program (its length sets the arithmetic
+per tile) and on the observation counts, so the choice is measured: each
+candidate runs one short batch of the real workload, the fastest is kept,
+and later batches with the same key reuse it.  The key is (longest program
+in VM instructions, median and largest observation count rounded to powers
+of two).  A tiling is (chains per block, waves per chain[, chains per
+wave]).
 """
 from __future__ import annotations
 
@@ -18,9 +21,12 @@ import math
 import os
 from typing import Dict, Optional, Sequence, Tuple
 
-CANDIDATES: Tuple[Tuple[int, int], ...] = ((16, 1), (8, 2), (4, 4))
+CANDIDATES: Tuple[Tuple[int, ...], ...] = ((16, 1), (8, 2), (4, 4))
+# Two chains per wave, for compiled batches whose programs all have depth 1.
+PAIR_CANDIDATES: Tuple[Tuple[int, ...], ...] = ((32, 1, 2), (16, 2, 2),
+                                                (8, 4, 2))
 
-_cache: Dict[str, Tuple[int, int]] = {}
+_cache: Dict[str, Tuple[int, ...]] = {}
 
 
 def tile_key(n_vm_instr: Sequence[int], n_obs: Sequence[int]) -> str:

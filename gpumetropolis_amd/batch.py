@@ -99,7 +99,11 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
                                    8, 8, 0, False, mode, tiling=t,
                                    use_graph=False)
                 return inf["kernel_ms"]
+            cands = autotune.CANDIDATES
+            if jit and all(i["vm_depth"] <= 1 for i in infos):
+                cands = cands + autotune.PAIR_CANDIDATES
             tiling, timings = autotune.choose(key, time_candidate,
+                                              candidates=cands,
                                               cache_path=tile_cache)
         outs, info = batch_run(dms, chains, inits, sds, [seed] * K,
                                list(range(K)), gammas, [de_noise] * K, n_iter,

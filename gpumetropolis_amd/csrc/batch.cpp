@@ -61,10 +61,15 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
   const uint32_t nb = std::max(1u, o.n_iter / every);
   std::vector<uint32_t> sizes(nb, every);
   sizes[nb - 1] += o.n_iter - nb * every;
-  const uint32_t cpb = o.chains_per_block > 0 ? (uint32_t)o.chains_per_block : 16u;
-  const uint32_t wpc = o.waves_per_chain > 0 ? (uint32_t)o.waves_per_chain : 16u / cpb;
-  if (cpb * wpc != 16) {
-    set_err("batched runs need chains_per_block * waves_per_chain == 16");
+  const uint32_t cw = o.chains_per_wave > 0 ? (uint32_t)o.chains_per_wave : 1u;
+  const uint32_t cpb =
+      o.chains_per_block > 0 ? (uint32_t)o.chains_per_block : 16u * cw;
+  const uint32_t wpc = o.waves_per_chain > 0 ? (uint32_t)o.waves_per_chain
+                                             : 16u / std::max(1u, cpb / cw);
+  if (cw > 2 || cpb % cw != 0 || cpb / cw * wpc != 16) {
+    set_err("batched runs need chains_per_wave 1 or 2, chains_per_block a "
+            "multiple of it and chains_per_block / chains_per_wave * "
+            "waves_per_chain == 16");
     return 1;
   }
   int max_depth = 0;
@@ -83,6 +88,11 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
   }
   TilingForce force;
   force.v = pick_variant(max_depth);
+  if (cw == 2 && force.v.ns != 0) {
+    set_err("chains_per_wave 2 needs every model's program at depth 1");
+    return 1;
+  }
+  force.v.cw = (int)cw;
   force.extra_static = kBatchArgsLds;
   gpum_run_ext ext;
   memset(&ext, 0, sizeof ext);
@@ -208,6 +218,7 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     h.wpc = t.wpc;
     h.tile_len = t.tile_len;
     h.stream_mode = 1;
+    set_sum_fields(m, &h);
     h.lds_tile_offset = t.lds_tile_offset;
     h.state_words = 6 * np + 6;
     BatchModelInfo& in = info[k];
@@ -241,14 +252,17 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
   // compiled program with the launch's variant.
   hipFunction_t jf = nullptr;
   {
-    const Variant v0 = pick_variant(models[0].model->L.vm_depth);
-    bool same = v0.ns == force.v.ns && v0.opt == force.v.opt;
+    const std::string src0 = jit_evaluator_source(
+        models[0].model->L, force.v.ns, force.v.opt, force.v.cw);
+    bool same = true;
     for (uint32_t k = 0; k < n_models && same; ++k)
-      same = !models[k].model->jit_eval.empty() &&
-             models[k].model->jit_eval == models[0].model->jit_eval;
+      same = models[k].model->jit &&
+             (k == 0 || jit_evaluator_source(models[k].model->L, force.v.ns,
+                                             force.v.opt, force.v.cw) == src0);
     if (same) {
       bool failed = false;
-      jf = model_jit_kernel(models[0].model, JIT_BATCH, lds + kBatchArgsLds, &failed);
+      jf = model_jit_kernel(models[0].model, JIT_BATCH, force.v,
+                            lds + kBatchArgsLds, &failed);
       if (failed) return 1;
     }
   }
@@ -269,7 +283,8 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
         e = launch_module(jf, dim3(n_wg), dim3(1024), lds, st, kargs);
       } else {
         e = launch_chain_kernel_batch(d_work.p, n_wg, d_args.p, d_hots.p, step,
-                                      force.v.ns, force.v.opt, lds, st);
+                                      force.v.ns, force.v.opt, force.v.cw, lds,
+                                      st);
       }
       if (e != hipSuccess) return e;
       e = launch_batch_end(d_info.p, chain_model.p, n_chains, t0, sizes[b - 1], b,

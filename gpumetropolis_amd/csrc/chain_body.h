@@ -1,8 +1,17 @@
 // chain_body.h -- device code of the chain, points and batched kernels,
 // generic over the per-observation evaluator E:
 //   E::kOpt         observations per lane per chunk;
+//   E::kCw          chains per wave: one wave evaluates kCw chains on every
+//                   observation it reads from LDS (chain j's uniform table
+//                   is U + j * H.n_uniform_pad);
 //   E::run(H, U, tile, ld, lane, T)
-//                   evaluate the program on one chunk into T[kOpt].
+//                   evaluate the program on one chunk into T[kCw][kOpt];
+//   E::mode(H)      how a chunk is summed (SumMode);
+//   E::run_acc(H, U, tile, ld, lane, s)
+//                   SUM_AFFINE: the chunk's sums s[kCw], accumulated
+//                   directly;
+//   E::finish(H, U, s, with_const)
+//                   apply the sum tail to one chain's data-pass sum.
 // The interpreter (VmEval) and code generated per model share this file.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -303,37 +312,60 @@ __device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
   }
 }
 
-// The chain's share of one tile: chunks part, part + wpc, ... of 64*OPT
-// observations; FP32 sum per chunk, FP64 across chunks.
+// The wave's share of one tile for its kCw chains: chunks part, part + wpc,
+// ... of 64*OPT observations; FP32 sum per chunk, FP64 across chunks, added
+// to acc[j].
 template <class E>
-__device__ __forceinline__ double tile_pass(const ChainHot& H,
-                                            const float* __restrict__ tile,
-                                            uint32_t t0, uint32_t part,
-                                            uint32_t wpc, const float* U,
-                                            int lane) {
-  constexpr int OPT = E::kOpt;
+__device__ __forceinline__ void tile_pass(const ChainHot& H,
+                                          const float* __restrict__ tile,
+                                          uint32_t t0, uint32_t part,
+                                          uint32_t wpc, const float* U,
+                                          int lane, double (&acc)[E::kCw]) {
+  constexpr int OPT = E::kOpt, CW = E::kCw;
   const uint32_t chunk = 64u * OPT;
   const uint32_t nchunks = H.tile_len / chunk;
-  double acc = 0.0;
+  const uint32_t mode = E::mode(H);
   for (uint32_t ck = part; ck < nchunks; ck += wpc) {
     const uint32_t base = ck * chunk;
-    float T[OPT];
-    E::run(H, U, tile + base, H.tile_len, lane, T);
     const uint32_t first = t0 + base;
-    float s = 0.0f;
+    float s[CW];
     if (first + chunk <= H.n_obs) {
+      if (mode == SUM_AFFINE) {
+        E::run_acc(H, U, tile + base, H.tile_len, lane, s);
+      } else {
+        float T[CW][OPT];
+        E::run(H, U, tile + base, H.tile_len, lane, T);
 #pragma unroll
-      for (int k = 0; k < OPT; ++k) s += T[k];
+        for (int j = 0; j < CW; ++j) {
+          if (mode == SUM_SQUARE) {
+            s[j] = 0.0f;
+#pragma unroll
+            for (int k = 0; k < OPT; ++k) s[j] = fmaf(T[j][k], T[j][k], s[j]);
+          } else {
+            s[j] = T[j][0];
+#pragma unroll
+            for (int k = 1; k < OPT; ++k) s[j] += T[j][k];
+          }
+        }
+      }
     } else {
+      // The last chunk of the data: observations past n_obs count zero.
+      float T[CW][OPT];
+      E::run(H, U, tile + base, H.tile_len, lane, T);
 #pragma unroll
-      for (int k = 0; k < OPT; ++k) {
-        uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
-        s += obs < H.n_obs ? T[k] : 0.0f;
+      for (int j = 0; j < CW; ++j) {
+        s[j] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) {
+          uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
+          const float v = obs < H.n_obs ? T[j][k] : 0.0f;
+          s[j] = mode == SUM_SQUARE ? fmaf(v, v, s[j]) : s[j] + v;
+        }
       }
     }
-    acc += (double)s;
+#pragma unroll
+    for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
   }
-  return acc;
 }
 
 // One workgroup's chains chain0 .. chain0 + cpb - 1 through the whole run.
@@ -350,18 +382,24 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
   double* partials = st_all + cpb * SW;
   float* Uall = reinterpret_cast<float*>(partials + cpb * wpc);
 
+  constexpr int CW = E::kCw;
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63;
   // Wave-uniform indices: readfirstlane tells the compiler so, and the
-  // chunk loops become scalar loops instead of exec-masked ones.
+  // chunk loops become scalar loops instead of exec-masked ones.  Wave
+  // (grp, part) evaluates chains grp*CW .. grp*CW + CW-1 on data share
+  // `part`; lane j < CW of the part-0 wave is chain grp*CW + j's lead.
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int c_local = wave / wpc;
-  const int part = wave - c_local * wpc;
+  const int grp = wave / wpc;
+  const int part = wave - grp * wpc;
+  const int c_first = grp * CW;
+  const int c_local = c_first + (lane < CW ? lane : 0);
   const uint32_t chain = chain0 + (uint32_t)c_local;
-  const bool active = chain < H.n_chains;
-  const bool lead = active && part == 0 && lane == 0;
+  const bool active = chain0 + (uint32_t)c_first < H.n_chains;
+  const bool lead = part == 0 && lane < CW && chain < H.n_chains;
   double* st = st_all + c_local * SW;
   float* U = Uall + c_local * NU;
+  const float* Uw = Uall + c_first * NU;
 
   for (int i = tid; i < cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
   __syncthreads();
@@ -403,12 +441,14 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
 
   for (uint32_t it = 0; it <= H.n_iter; ++it) {
     // ---- data pass for the chain's current proposal -------------------
-    double acc = 0.0;
+    double acc[CW];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) acc[j] = 0.0;
     if (H.n_vm > 0) {
       if constexpr (!STREAM) {
         if (active)
-          acc = tile_pass<E>(H, reinterpret_cast<const float*>(res), 0,
-                                   part, wpc, U, lane);
+          tile_pass<E>(H, reinterpret_cast<const float*>(res), 0, part, wpc,
+                       Uw, lane, acc);
       } else if (H.stream_mode == 1) {
         // Double buffering over an even number of tiles: tile ti is in A
         // when the pair starts; the copy of ti+1 into B overlaps the pass
@@ -417,16 +457,16 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
         for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
           issue_tile(ti + 1, g_tile_b);
           if (active)
-            acc += tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
-                                      ti * tile_len, part, wpc, U, lane);
+            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
+                         ti * tile_len, part, wpc, Uw, lane, acc);
           __syncthreads();
           if (ti + 2 < n_tiles)
             issue_tile(ti + 2, g_tile_a);
           else if (it < H.n_iter)
             issue_tile(0, g_tile_a);
           if (active)
-            acc += tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_b),
-                                      (ti + 1) * tile_len, part, wpc, U, lane);
+            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_b),
+                         (ti + 1) * tile_len, part, wpc, Uw, lane, acc);
           __syncthreads();
         }
       } else {
@@ -439,20 +479,25 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
           }
           __syncthreads();
           if (active)
-            acc += tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
-                                      ti * tile_len, part, wpc, U, lane);
+            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
+                         ti * tile_len, part, wpc, Uw, lane, acc);
         }
       }
     }
     // ---- reduction ----------------------------------------------------
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) partials[wave] = acc;
+    for (int j = 0; j < CW; ++j) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1)
+        acc[j] += __shfl_xor(acc[j], off, 64);
+      if (lane == 0) partials[(c_first + j) * wpc + part] = acc[j];
+    }
     __syncthreads();
     // ---- fused step ---------------------------------------------------
     if (lead) {
       double ll = 0.0;
       for (int p = 0; p < wpc; ++p) ll += partials[c_local * wpc + p];
+      if (H.n_vm > 0) ll = E::finish(H, U, ll, true);
       chain_finish_iter(Ap, it, chain, st, ll);
     }
     if (it > 0 && H.swap_every > 0) {
@@ -543,8 +588,10 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t item_sh;
   __shared__ PointsPlan plan_sh;
+  constexpr int CW = E::kCw;
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63;
+  // Wave w evaluates slots w*CW .. w*CW + CW-1 of the item's group.
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t tile_len = H.tile_len;
   const uint32_t n_tiles = H.n_pad / tile_len;
@@ -554,7 +601,7 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
   }
   const int NU = (int)H.n_uniform_pad;
   float* Uall = reinterpret_cast<float*>(smem);
-  float* U = Uall + wave * NU;
+  float* Uw = Uall + wave * CW * NU;
   bool loaded = false;  // resident layout: data loaded with the first item
   for (;;) {
     __syncthreads();  // previous item's reads of U and the tiles are done
@@ -566,11 +613,12 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
     // the slot groups of one slice share its tiles in the caches.
     const uint32_t groups = plan_sh.groups, tps = plan_sh.tps;
     const uint32_t sl = item / groups, g = item - sl * groups;
-    const uint32_t k = g * H.cpb + (uint32_t)wave;
-    const bool active = k < plan_sh.count;
+    const uint32_t k0 = g * H.cpb + (uint32_t)(wave * CW);
+    const bool active = k0 < plan_sh.count;
+    const uint32_t kl = k0 + (uint32_t)(lane < CW ? lane : 0);  // lane's slot
     for (int i = tid; i < (int)H.cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
     __syncthreads();
-    if (active && lane == 0) points_hoist(P, k, U);
+    if (lane < CW && kl < plan_sh.count) points_hoist(P, kl, Uw + lane * NU);
     const uint32_t ti0 = min(sl * tps, n_tiles);
     const uint32_t ti1 = min(ti0 + tps, n_tiles);
     const uint32_t n4 = tile_len / 4;
@@ -578,7 +626,9 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
     const uint32_t ld4 = H.ld / 4;
     const float4* src = reinterpret_cast<const float4*>(H.data);
     const uint32_t wave_base = (uint32_t)(wave * 64);
-    double acc = 0.0;
+    double acc[CW];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) acc[j] = 0.0;
     if (H.n_vm > 0) {
       if constexpr (!STREAM) {
         // One tile holds the whole data set; load it once per workgroup.
@@ -592,9 +642,8 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
         }
         __syncthreads();
         if (active && ti0 < ti1)
-          acc = tile_pass<E>(
-              H, reinterpret_cast<const float*>(smem + H.lds_tile_offset), 0,
-              0, 1, U, lane);
+          tile_pass<E>(H, reinterpret_cast<const float*>(smem + H.lds_tile_offset),
+                       0, 0, 1, Uw, lane, acc);
       } else if (H.stream_mode == 1) {
         if (ti0 < ti1)
           issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti0, g_tile_a);
@@ -604,14 +653,14 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
           if (two)
             issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 1, g_tile_b);
           if (active)
-            acc += tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
-                                      ti * tile_len, 0, 1, U, lane);
+            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
+                         ti * tile_len, 0, 1, Uw, lane, acc);
           __syncthreads();
           if (ti + 2 < ti1)
             issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 2, g_tile_a);
           if (two && active)
-            acc += tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_b),
-                                      (ti + 1) * tile_len, 0, 1, U, lane);
+            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_b),
+                         (ti + 1) * tile_len, 0, 1, Uw, lane, acc);
           __syncthreads();
         }
       } else {
@@ -623,14 +672,24 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
           }
           __syncthreads();
           if (active)
-            acc += tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
-                                      ti * tile_len, 0, 1, U, lane);
+            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
+                         ti * tile_len, 0, 1, Uw, lane, acc);
         }
       }
     }
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (active && lane == 0) P->partial[(size_t)sl * P->n_points + k] = acc;
+    for (int j = 0; j < CW; ++j) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1)
+        acc[j] += __shfl_xor(acc[j], off, 64);
+      const uint32_t k = k0 + (uint32_t)j;
+      // The sum tail's constant joins the first slice's partial only.
+      if (lane == 0 && k < plan_sh.count) {
+        double v = acc[j];
+        if (H.n_vm > 0) v = E::finish(H, Uw + j * NU, v, sl == 0);
+        P->partial[(size_t)sl * P->n_points + k] = v;
+      }
+    }
   }
   // The last workgroup out leaves the queue at zero for the next launch.
   if (tid == 0) {
@@ -644,13 +703,37 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
 }
 
 // The interpreter as an evaluator.
-template <int NS, int OPT>
+template <int NS, int OPT, int CW>
 struct VmEval {
   static constexpr int kOpt = OPT;
+  static constexpr int kCw = CW;
   static __device__ __forceinline__ void run(const ChainHot& H, const float* U,
                                              const float* tile, uint32_t ld,
-                                             int lane, float (&T)[OPT]) {
-    vm_run<NS, OPT>((vm_prog_t)H.vm, H.n_vm, U, tile, ld, lane, T);
+                                             int lane, float (&T)[CW][OPT]) {
+    vm_run<NS, OPT, CW>((vm_prog_t)H.vm, H.n_vm, U, H.n_uniform_pad, tile, ld,
+                        lane, T);
+  }
+  static __device__ __forceinline__ uint32_t mode(const ChainHot& H) {
+    return H.sum_mode;
+  }
+  static __device__ __forceinline__ void run_acc(const ChainHot& H,
+                                                 const float* U,
+                                                 const float* tile,
+                                                 uint32_t ld, int lane,
+                                                 float (&s)[CW]) {
+    vm_run_acc<OPT, CW>((vm_prog_t)H.vm, H.n_vm, U, H.n_uniform_pad, tile, ld,
+                        lane, s);
+  }
+  static __device__ __forceinline__ double finish(const ChainHot& H,
+                                                  const float* U, double s,
+                                                  bool with_const) {
+    if (H.n_sum_tail == 0) return s;
+    double alpha = 1.0, beta = 0.0;
+    const VmIns4* tail = H.vm + H.n_vm + 1;
+    for (uint32_t i = 0; i < H.n_sum_tail; ++i)
+      vm_tail_step(tail[i].op, (double)U[tail[i].a], (double)U[tail[i].b],
+                   alpha, beta);
+    return vm_tail_apply(alpha, beta, s, H.n_obs, with_const);
   }
 };
 

@@ -87,8 +87,8 @@ struct ChainHot {
   uint32_t ld;
   uint32_t n_chains;
   uint32_t n_iter;
-  uint32_t cpb;
-  uint32_t wpc;
+  uint32_t cpb;  // chains per workgroup (a multiple of the chains per wave)
+  uint32_t wpc;  // waves sharing one wave's chains' data pass
   uint32_t tile_len;
   // 0: whole data set resident in LDS; 1: double-buffered streaming through
   // LDS DMA copies; 2: synchronous single-buffer streaming.
@@ -98,6 +98,10 @@ struct ChainHot {
   uint32_t swap_every;
   uint32_t n_batches;
   uint32_t swap_group;
+  // Summation of the program (SumMode) and the length of its sum tail,
+  // stored in `vm` after the program's padding instruction.
+  uint32_t sum_mode;
+  uint32_t n_sum_tail;
 };
 
 // Streamed tiles: two static LDS buffers of kTileF4 float4s each.
@@ -127,7 +131,7 @@ static_assert(sizeof(ChainArgs) <= kBatchArgsLds, "ChainArgs too large");
 
 // The data term of a list of parameter points, summed per observation
 // slice by a persistent grid of n_workgroups that takes (slot group, slice)
-// items from a queue; one wave per slot.
+// items from a queue; one wave per `chains per wave` slots.
 struct PointsArgs {
   const double* points;    // FP64, np per point
   const uint32_t* index;   // slot k evaluates point index[k] (null: point k)
@@ -145,17 +149,19 @@ struct PointsArgs {
 
 // Host launchers (not seen by the run-time compiler of per-model kernels).
 #ifndef __HIPCC_RTC__
+// Variant (ns, opt, cw): spill slots, observations per lane, chains per
+// wave; a workgroup has h.cpb / cw * h.wpc waves.
 hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
-                               int ns, int opt, size_t lds_bytes,
+                               int ns, int opt, int cw, size_t lds_bytes,
                                hipStream_t s);
 hipError_t launch_chain_kernel_batch(const BatchWork* work, uint32_t n_wg,
                                      const ChainArgs* args,
                                      const ChainHot* hots, BatchStep step,
-                                     int ns, int opt, size_t lds_bytes,
-                                     hipStream_t s);
+                                     int ns, int opt, int cw,
+                                     size_t lds_bytes, hipStream_t s);
 // `p` is a device copy of the arguments.
 hipError_t launch_points_kernel(const PointsArgs* p, uint32_t n_workgroups,
-                                const ChainHot& h, int ns, int opt,
+                                const ChainHot& h, int ns, int opt, int cw,
                                 size_t lds_bytes, hipStream_t s);
 hipError_t set_chain_kernel_lds_limit(size_t bytes);
 #endif  // __HIPCC_RTC__

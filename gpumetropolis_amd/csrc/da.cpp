@@ -464,7 +464,8 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
     return 1;
   }
   Tiling t;
-  if (!pick_tiling(m, C, nullptr, &t, true)) return 1;
+  if (!pick_tiling(m, C, nullptr, &t, true, nullptr, (int)X.chains_per_wave))
+    return 1;
   if (!ensure_data(m, t.n_pad)) return 1;
   const uint32_t n_tiles = t.n_pad / t.tile_len;
   // Exact-stage work items per launch: every CU twice by default.
@@ -581,6 +582,7 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   h.wpc = 1;
   h.tile_len = t.tile_len;
   h.stream_mode = t.stream_mode;
+  set_sum_fields(m, &h);
   h.lds_tile_offset = t.lds_tile_offset;
   PointsArgs pa;
   memset(&pa, 0, sizeof pa);
@@ -616,13 +618,16 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
     return 1;
   bool jit_failed = false;
   hipFunction_t jf = model_jit_kernel(
-      m, t.stream_mode ? JIT_POINTS_STREAM : JIT_POINTS_RESIDENT,
+      m, t.stream_mode ? JIT_POINTS_STREAM : JIT_POINTS_RESIDENT, t.v,
       t.lds_bytes + kPointsExtraLds, &jit_failed);
   if (jit_failed) return 1;
   auto points = [&](const PointsArgs* p) -> hipError_t {
-    if (!jf) return launch_points_kernel(p, n_wg, h, t.v.ns, t.v.opt, t.lds_bytes, st);
+    if (!jf)
+      return launch_points_kernel(p, n_wg, h, t.v.ns, t.v.opt, t.v.cw,
+                                  t.lds_bytes, st);
     void* kargs[] = {(void*)&p, (void*)&h};
-    return launch_module(jf, dim3(n_wg), dim3(h.cpb * 64), t.lds_bytes, st, kargs);
+    return launch_module(jf, dim3(n_wg), dim3(tiling_waves(t) * 64),
+                         t.lds_bytes, st, kargs);
   };
   chk(points(d_pinit.p));
   chk(launch_da_init_finish(d_args.p, C, st));

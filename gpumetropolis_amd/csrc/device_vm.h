@@ -139,16 +139,30 @@ __device__ __forceinline__ VmIns4 vm_fetch(vm_prog_t prog, uint32_t pc) {
   return r;
 }
 
-// One VM instruction on one chunk.  The interpreter calls it with the
-// fetched opcode; code generated per model calls it with constants, so the
-// switch folds away and each program becomes straight-line code.
+// The data column an instruction reads, or -1.
+__device__ __forceinline__ int vm_col(uint32_t op, uint32_t a, uint32_t b) {
+  switch (op) {
+    case VM_LOAD_D: case VM_PUSH_D:
+    case VM_ADD_D: case VM_SUB_D: case VM_MUL_D: case VM_DIV_D: case VM_POW_D:
+    case VM_RSUB_D: case VM_RDIV_D: case VM_RPOW_D:
+    case VM_FMA_DU:
+      return (int)a;
+    case VM_FMA_UD:
+      return (int)b;
+    default:
+      return -1;
+  }
+}
+
+// One VM instruction on one chunk of one chain, its data column (if any)
+// already in D.  The interpreter calls it with the fetched opcode; code
+// generated per model calls it with constants, so the switch folds away and
+// each program becomes straight-line code.
 template <int NS, int OPT>
-__device__ __forceinline__ void vm_step(uint32_t op, uint32_t slot, uint32_t a,
-                                        uint32_t b, const float* __restrict__ U,
-                                        const float* __restrict__ tile,
-                                        uint32_t ld, int lane, float (&T)[OPT],
-                                        float (&S)[NS > 0 ? NS : 1][OPT],
-                                        float (&D)[OPT]) {
+__device__ __forceinline__ void vm_apply(uint32_t op, uint32_t slot, uint32_t a,
+                                         uint32_t b, const float* __restrict__ U,
+                                         const float (&D)[OPT], float (&T)[OPT],
+                                         float (&S)[NS > 0 ? NS : 1][OPT]) {
   switch (op) {
     case VM_LOAD_U: {
       float u = U[a];
@@ -157,7 +171,8 @@ __device__ __forceinline__ void vm_step(uint32_t op, uint32_t slot, uint32_t a,
       break;
     }
     case VM_LOAD_D:
-      vm_load_col<OPT>(tile + a * ld, lane, T);
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) T[k] = D[k];
       break;
     case VM_PUSH_U:
       if constexpr (NS > 0) {
@@ -170,7 +185,8 @@ __device__ __forceinline__ void vm_step(uint32_t op, uint32_t slot, uint32_t a,
     case VM_PUSH_D:
       if constexpr (NS > 0) {
         vm_spill<0, NS, OPT>(slot, S, T);
-        vm_load_col<OPT>(tile + a * ld, lane, T);
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[k] = D[k];
       }
       break;
     case VM_ADD_S:
@@ -214,7 +230,6 @@ case NAME: {                                        \
 #undef GPUM_UOP
 #define GPUM_DOP(NAME, EXPR)                          \
 case NAME: {                                        \
-  vm_load_col<OPT>(tile + a * ld, lane, D);         \
   _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
     float t = T[k];                                 \
     float d = D[k];                                 \
@@ -258,14 +273,12 @@ case NAME: {                                        \
     }
     case VM_FMA_UD: {
       float ua = U[a];
-      vm_load_col<OPT>(tile + b * ld, lane, D);
 #pragma unroll
       for (int k = 0; k < OPT; ++k) T[k] = fmaf(ua, D[k], T[k]);
       break;
     }
     case VM_FMA_DU: {
       float ub = U[b];
-      vm_load_col<OPT>(tile + a * ld, lane, D);
 #pragma unroll
       for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], D[k], ub);
       break;
@@ -275,15 +288,96 @@ case NAME: {                                        \
   }
 }
 
-template <int NS, int OPT>
-__device__ __forceinline__ void vm_run(vm_prog_t prog,
-                                       uint32_t n_instr,
+// One VM instruction on one chunk of CW chains: the data column is read
+// from LDS once and used by every chain (chain j's uniform table is
+// U + j * nu).  Each chain's arithmetic is the single-chain arithmetic.
+template <int NS, int OPT, int CW>
+__device__ __forceinline__ void vm_step(uint32_t op, uint32_t slot, uint32_t a,
+                                        uint32_t b, const float* __restrict__ U,
+                                        uint32_t nu,
+                                        const float* __restrict__ tile,
+                                        uint32_t ld, int lane,
+                                        float (&T)[CW][OPT],
+                                        float (&S)[CW][NS > 0 ? NS : 1][OPT],
+                                        float (&D)[OPT]) {
+  const int col = vm_col(op, a, b);
+  if (col >= 0) vm_load_col<OPT>(tile + (uint32_t)col * ld, lane, D);
+#pragma unroll
+  for (int j = 0; j < CW; ++j)
+    vm_apply<NS, OPT>(op, slot, a, b, U + j * nu, D, T[j], S[j]);
+}
+
+// SUM_AFFINE: s[j] += U_j[a] * column b over the chunk, one fused operation
+// per observation, in the order vm_load_col lays the observations out.
+template <int OPT, int CW>
+__device__ __forceinline__ void vm_acc_fma(const float* __restrict__ U,
+                                           uint32_t nu,
+                                           const float* __restrict__ tile,
+                                           uint32_t ld, int lane, uint32_t a,
+                                           uint32_t b, float (&s)[CW]) {
+  float D[OPT];
+  vm_load_col<OPT>(tile + b * ld, lane, D);
+#pragma unroll
+  for (int j = 0; j < CW; ++j) {
+    const float ua = U[j * nu + a];
+#pragma unroll
+    for (int k = 0; k < OPT; ++k) s[j] = fmaf(ua, D[k], s[j]);
+  }
+}
+
+// The chunk sums of an affine program: its FMA_UD instructions (pc >= 1;
+// instruction 0 loads the zero the per-observation form starts from).
+template <int OPT, int CW>
+__device__ __forceinline__ void vm_run_acc(vm_prog_t prog, uint32_t n_instr,
+                                           const float* __restrict__ U,
+                                           uint32_t nu,
+                                           const float* __restrict__ tile,
+                                           uint32_t ld, int lane,
+                                           float (&s)[CW]) {
+#pragma unroll
+  for (int j = 0; j < CW; ++j) s[j] = 0.0f;
+  VmIns4 nxt = vm_fetch(prog, 1);
+  for (uint32_t pc = 1; pc < n_instr; ++pc) {
+    const uint32_t a = nxt.a, b = nxt.b;
+    nxt = vm_fetch(prog, pc + 1);
+    vm_acc_fma<OPT, CW>(U, nu, tile, ld, lane, a, b, s);
+  }
+}
+
+// One entry of the sum tail (operands U[a], U[b]): the data pass's sum
+// becomes alpha * s + n * beta.  One rounding per statement.
+__device__ __forceinline__ void vm_tail_step(uint32_t op, double ua, double ub,
+                                             double& alpha, double& beta) {
+  if (op == VM_ADD_U) {
+    beta = beta + ua;
+  } else if (op == VM_SUB_U) {
+    beta = beta - ua;
+  } else {  // MUL_U; FMA_UU: T * U[a] + U[b]
+    alpha = alpha * ua;
+    beta = beta * ua;
+    if (op == VM_FMA_UU) beta = beta + ub;
+  }
+}
+
+// alpha * s + n * beta (n * beta only where the caller's sum covers the
+// constant's share: once per data pass).
+__device__ __forceinline__ double vm_tail_apply(double alpha, double beta,
+                                                double s, uint32_t n,
+                                                bool with_const) {
+  const double nb = with_const ? (double)n * beta : 0.0;
+  return __builtin_fma(alpha, s, nb);
+}
+
+template <int NS, int OPT, int CW>
+__device__ __forceinline__ void vm_run(vm_prog_t prog, uint32_t n_instr,
                                        const float* __restrict__ U,
+                                       uint32_t nu,
                                        const float* __restrict__ tile,
-                                       uint32_t ld, int lane, float (&T)[OPT]) {
+                                       uint32_t ld, int lane,
+                                       float (&T)[CW][OPT]) {
   // NS == 0: a program of depth 1 (no PUSH, no binary op on the stack);
   // those cases compile to nothing and the spill registers do not exist.
-  float S[NS > 0 ? NS : 1][OPT];
+  float S[CW][NS > 0 ? NS : 1][OPT];
   float D[OPT];
   // The first instruction of every program is a LOAD (the host lowering
   // starts from an empty stack), so T needs no initialisation.
@@ -294,7 +388,7 @@ __device__ __forceinline__ void vm_run(vm_prog_t prog,
   for (uint32_t pc = 0; pc < n_instr; ++pc) {
     const uint32_t op = nxt.op, slot = nxt.slot, a = nxt.a, b = nxt.b;
     nxt = vm_fetch(prog, pc + 1);
-    vm_step<NS, OPT>(op, slot, a, b, U, tile, ld, lane, T, S, D);
+    vm_step<NS, OPT, CW>(op, slot, a, b, U, nu, tile, ld, lane, T, S, D);
   }
 }
 

@@ -106,12 +106,12 @@ bool device_init() {
 
 Variant pick_variant(int vm_depth) {
   int need = vm_depth > 1 ? vm_depth - 1 : 0;
-  if (need == 0) return {0, 16};
-  if (need <= 1) return {1, 16};
-  if (need <= 3) return {3, 16};
-  if (need <= 7) return {7, 8};
-  if (need <= 15) return {15, 4};
-  return {31, 4};
+  if (need == 0) return {0, 16, 1};
+  if (need <= 1) return {1, 16, 1};
+  if (need <= 3) return {3, 16, 1};
+  if (need <= 7) return {7, 8, 1};
+  if (need <= 15) return {15, 4, 1};
+  return {31, 4, 1};
 }
 
 uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
@@ -196,6 +196,13 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
   }
   // One padding instruction: the kernel prefetches pc + 1.
   for (int k = 0; k < 4; ++k) vm4.push_back(k == 0 ? (uint32_t)VM_NUM_OPS : 0u);
+  // The sum tail follows the padding.
+  for (const VmInstr& ins : m->L.sum_tail) {
+    vm4.push_back(vm_op(ins));
+    vm4.push_back(vm_slot(ins));
+    vm4.push_back(vm_a(ins));
+    vm4.push_back(vm_b(ins));
+  }
   m->n_uniform_pad = (uint32_t)std::max(1, m->L.n_uniform);
   std::vector<float> u0(m->n_uniform_pad, 0.0f);
   for (auto& kv : m->L.const_uniforms) u0[kv.first] = kv.second;
@@ -217,9 +224,45 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
 namespace gpum {
 namespace rt {
 
+static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
+                           const gpum_run_ext* ext, Tiling* t, bool points,
+                           int cw_in, const TilingForce* force);
+
 bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
                  const gpum_run_ext* ext, Tiling* t, bool points,
-                 const TilingForce* force) {
+                 const TilingForce* force, int cw_req) {
+  if (!force) {
+    if (ext && ext->struct_size >= offsetof(gpum_run_ext, chains_per_wave) + 4 &&
+        ext->chains_per_wave > 0)
+      cw_req = ext->chains_per_wave;
+    // Automatic: two chains per wave when the program keeps no stack, the
+    // chains fill every CU with pairs and the launch runs a compiled kernel
+    // (the interpreter's dispatch holds both chains' registers at every
+    // opcode merge and spills).  A ladder or a pinned workgroup of odd size
+    // keeps one.
+    if (cw_req == 0) {
+      const bool has_tiling =
+          ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4;
+      const bool odd_ladder =
+          ext && ext->swap_every > 0 && ext->swap_group > 1 && ext->swap_group % 2u;
+      const bool odd_pin =
+          has_tiling && ext->chains_per_block > 0 && ext->chains_per_block % 2;
+      if (m->jit && pick_variant(m->L.vm_depth).ns == 0 &&
+          n_chains >= 2u * (uint32_t)g_dev.n_cu && !odd_ladder && !odd_pin &&
+          pick_tiling_cw(m, n_chains, ext, t, points, 2, nullptr) &&
+          t->lds_bytes <= kJitMaxDynamicLds)
+        return true;
+      cw_req = 1;
+    }
+  }
+  return pick_tiling_cw(m, n_chains, ext, t, points, force ? 0 : cw_req, force);
+}
+
+// pick_tiling with the chains per wave fixed (cw_in; with `force`, the
+// forced variant's).
+static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
+                           const gpum_run_ext* ext, Tiling* t, bool points,
+                           int cw_in, const TilingForce* force) {
   t->v = force ? force->v : pick_variant(m->L.vm_depth);
   const size_t static_lds =
       kStreamStaticLds + (force ? force->extra_static
@@ -227,20 +270,37 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   const uint32_t opt = (uint32_t)t->v.opt;
   t->chunk = 64u * opt;
   const uint32_t ncu = (uint32_t)g_dev.n_cu;
+  const bool has_tiling =
+      ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4;
+  const bool pinned = has_tiling && ext->chains_per_block > 0;
+  const bool ladder = ext && ext->swap_every > 0 && ext->swap_group > 1;
+  // Chains per wave: two chains share every observation a wave reads from
+  // LDS (half the LDS traffic per evaluation).
+  if (!force) {
+    if (cw_in != 1 && !(cw_in == 2 && t->v.ns == 0)) {
+      set_err("chains_per_wave must be 1, or 2 for a program of depth 1");
+      return false;
+    }
+    t->v.cw = cw_in;
+  }
+  const uint32_t cw = (uint32_t)t->v.cw;
+  const uint32_t groups = (n_chains + cw - 1) / cw;  // waves' worth of chains
   uint32_t wpc, cpb;
-  if (n_chains >= 16u * ncu) {
+  if (groups >= 16u * ncu) {
     wpc = 1;
   } else {
-    uint32_t w = 16u * ncu / std::max(1u, n_chains);
+    uint32_t w = 16u * ncu / std::max(1u, groups);
     wpc = 1;
     while (wpc * 2 <= w && wpc * 2 <= 16) wpc *= 2;
   }
-  // No more waves per chain than there are chunks to share.
+  // No more waves per chain group than there are chunks to share.
   uint32_t nchunks = std::max(1u, (m->n_obs + t->chunk - 1) / t->chunk);
   while (wpc > 1 && wpc > nchunks) wpc /= 2;
-  cpb = std::max(1u, 16u / wpc);
-  if (cpb > n_chains) cpb = std::max(1u, n_chains);
-  if (ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4) {
+  auto fill = [&](uint32_t w) {  // chains of a full workgroup of w-wave groups
+    return std::min(std::max(1u, 16u / w), std::max(1u, groups)) * cw;
+  };
+  cpb = fill(wpc);
+  if (has_tiling) {
     if (ext->waves_per_chain > 0) wpc = (uint32_t)ext->waves_per_chain;
     if (ext->chains_per_block > 0) cpb = (uint32_t)ext->chains_per_block;
   }
@@ -250,25 +310,26 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
     const size_t tile_cap = (size_t)kTileF4 * 16;
     while (wpc > 1 && (size_t)t->chunk * wpc * ncol0 * 4 > tile_cap) {
       wpc /= 2;
-      if (!(ext && ext->chains_per_block > 0))
-        cpb = std::min(std::max(1u, 16u / wpc), std::max(1u, n_chains));
+      if (!pinned) cpb = fill(wpc);
     }
   }
   // A temperature ladder lives inside one workgroup.
-  if (ext && ext->swap_every > 0 && ext->swap_group > 1) {
+  if (ladder) {
     const uint32_t G = ext->swap_group;
-    while (wpc > 1 && G * wpc > 16) wpc /= 2;
-    cpb = G * std::max(1u, (16u / wpc) / G);
+    while (wpc > 1 && G / cw * wpc > 16) wpc /= 2;
+    cpb = G * std::max(1u, (16u / wpc * cw) / G);
     while (cpb > G && cpb > n_chains) cpb -= G;
   }
   if (points) {
-    // One wave per slot: the points kernel splits observations over
+    // One wave per cw slots: the points kernel splits observations over
     // workgroups instead of waves.
     wpc = 1;
-    cpb = std::min(16u, std::max(1u, n_chains));
+    cpb = std::min(16u * cw, groups * cw);
   }
-  if (cpb * wpc > 16 || cpb == 0 || wpc == 0) {
-    set_err("invalid tiling: chains_per_block * waves_per_chain must be <= 16");
+  if (cpb == 0 || wpc == 0 || cpb % cw != 0 || cpb / cw * wpc > 16) {
+    set_err("invalid tiling: chains_per_block must be a multiple of "
+            "chains_per_wave and chains_per_block / chains_per_wave * "
+            "waves_per_chain at most 16");
     return false;
   }
   t->cpb = cpb;
@@ -283,10 +344,9 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   };
   // Per-chain records and uniform tables must leave room for the two
   // streamed tile buffers.
-  const bool pinned = ext && ext->chains_per_block > 0;
-  while (!pinned && cpb > 1 &&
+  while (!pinned && cpb > cw &&
          head_bytes(cpb, wpc) + static_lds > g_dev.lds_max) {
-    cpb /= 2;
+    cpb = std::max(1u, cpb / cw / 2) * cw;
   }
   t->cpb = cpb;
   size_t head = head_bytes(cpb, wpc);
@@ -317,7 +377,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
   t->resident = 0;
   // The double-buffered copy is laid out for 1024-thread workgroups; other
   // shapes copy synchronously.
-  t->stream_mode = cpb * wpc == 16 ? 1 : 2;
+  t->stream_mode = cpb / cw * wpc == 16 ? 1 : 2;
   // Tiles of as many steps as one static buffer holds, an even number of
   // them (the double buffer alternates in pairs), as little padding as
   // that allows.
@@ -333,11 +393,13 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
 }
 
 hipFunction_t model_jit_kernel(const gpum_hip_model* m, int kind,
-                               size_t lds_bytes, bool* failed) {
+                               const Variant& v, size_t lds_bytes,
+                               bool* failed) {
   *failed = false;
-  if (m->jit_eval.empty() || lds_bytes > kJitMaxDynamicLds) return nullptr;
+  if (!m->jit || lds_bytes > kJitMaxDynamicLds) return nullptr;
   std::string err;
-  hipFunction_t f = jit_kernel(m->jit_eval, (JitKind)kind, &err);
+  hipFunction_t f = jit_kernel(jit_evaluator_source(m->L, v.ns, v.opt, v.cw),
+                               (JitKind)kind, &err);
   if (!f) {
     set_err("%s", err.c_str());
     *failed = true;
@@ -458,19 +520,21 @@ int gpum_hip_model_set_jit(gpum_hip_model* m, int32_t enable,
   hipSetDevice(g_dev.id);
   if (compile_ms) *compile_ms = 0.0;
   if (!enable || m->L.empty) {
-    m->jit_eval.clear();
+    m->jit = false;
     return 0;
   }
-  const Variant v = pick_variant(m->L.vm_depth);
-  std::string src = jit_evaluator_source(m->L, v.ns, v.opt);
+  // Compile the streamed chain kernel of the variant of large launches now
+  // (a failure is reported here); other kinds and variants on first use.
+  Variant v = pick_variant(m->L.vm_depth);
+  if (v.ns == 0) v.cw = 2;
+  std::string src = jit_evaluator_source(m->L, v.ns, v.opt, v.cw);
   std::string err;
   double ms = 0.0;
-  // Compile the streamed chain kernel now; the other kinds on first use.
   if (!jit_kernel(src, JIT_CHAIN_STREAM, &err, &ms)) {
     set_err("%s", err.c_str());
     return 1;
   }
-  m->jit_eval = src;
+  m->jit = true;
   if (compile_ms) *compile_ms = ms;
   return 0;
 }
@@ -493,6 +557,8 @@ int gpum_hip_model_info(const gpum_hip_model* m, gpum_model_info* info) {
   info->n_hoist_words = (uint32_t)m->L.hoist.size();
   info->w_exec = m->L.w_exec;
   info->w_bytecode = m->L.w_bytecode;
+  info->sum_mode = m->L.empty ? 0u : m->L.sum_mode;
+  info->n_sum_tail = m->L.empty ? 0u : (uint32_t)m->L.sum_tail.size();
   return 0;
 }
 
@@ -662,14 +728,15 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   h.swap_every = a.swap_every;
   h.n_batches = a.n_batches;
   h.swap_group = a.swap_group;
+  set_sum_fields(m, &h);
 
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   bool jit_failed = false;
   hipFunction_t jf = model_jit_kernel(
-      m, t.stream_mode ? JIT_CHAIN_STREAM : JIT_CHAIN_RESIDENT, t.lds_bytes,
-      &jit_failed);
+      m, t.stream_mode ? JIT_CHAIN_STREAM : JIT_CHAIN_RESIDENT, t.v,
+      t.lds_bytes, &jit_failed);
   if (jit_failed) return 1;
   hipEventRecord(e0, g_dev.stream);
   hipError_t e;
@@ -677,9 +744,10 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
     const ChainArgs* ap = d_args.p;
     void* kargs[] = {(void*)&ap, (void*)&h};
     e = launch_module(jf, dim3((h.n_chains + h.cpb - 1) / h.cpb),
-                      dim3(h.cpb * h.wpc * 64), t.lds_bytes, g_dev.stream, kargs);
+                      dim3(tiling_waves(t) * 64), t.lds_bytes, g_dev.stream,
+                      kargs);
   } else {
-    e = launch_chain_kernel(d_args.p, h, t.v.ns, t.v.opt, t.lds_bytes,
+    e = launch_chain_kernel(d_args.p, h, t.v.ns, t.v.opt, t.v.cw, t.lds_bytes,
                             g_dev.stream);
   }
   hipEventRecord(e1, g_dev.stream);

@@ -141,21 +141,49 @@ bool runtime_matches(std::string* why) {
 
 }  // namespace
 
-std::string jit_evaluator_source(const Lowered& L, int ns, int opt) {
+std::string jit_evaluator_source(const Lowered& L, int ns, int opt, int cw) {
   std::ostringstream o;
+  const int nsd = ns > 0 ? ns : 1;
   o << "#include \"chain_body.h\"\n"
     << "namespace gpum {\n"
     << "struct JitEval {\n"
     << "  static constexpr int kOpt = " << opt << ";\n"
+    << "  static constexpr int kCw = " << cw << ";\n"
     << "  static __device__ __forceinline__ void run(const ChainHot& H,\n"
     << "      const float* __restrict__ U, const float* __restrict__ tile,\n"
-    << "      uint32_t ld, int lane, float (&T)[" << opt << "]) {\n"
-    << "    float S[" << (ns > 0 ? ns : 1) << "][" << opt << "];\n"
-    << "    float D[" << opt << "];\n";
+    << "      uint32_t ld, int lane, float (&T)[" << cw << "][" << opt << "]) {\n"
+    << "    float S[" << cw << "][" << nsd << "][" << opt << "];\n"
+    << "    float D[" << opt << "];\n"
+    << "    const uint32_t nu = H.n_uniform_pad;\n";
   for (const VmInstr& ins : L.vm)
-    o << "    vm_step<" << ns << ", " << opt << ">(" << vm_op(ins) << "u, "
-      << vm_slot(ins) << "u, " << vm_a(ins) << "u, " << vm_b(ins)
-      << "u, U, tile, ld, lane, T, S, D);\n";
+    o << "    vm_step<" << ns << ", " << opt << ", " << cw << ">(" << vm_op(ins)
+      << "u, " << vm_slot(ins) << "u, " << vm_a(ins) << "u, " << vm_b(ins)
+      << "u, U, nu, tile, ld, lane, T, S, D);\n";
+  o << "  }\n"
+    << "  static __device__ __forceinline__ uint32_t mode(const ChainHot&) {\n"
+    << "    return " << L.sum_mode << "u;\n  }\n"
+    << "  static __device__ __forceinline__ void run_acc(const ChainHot& H,\n"
+    << "      const float* __restrict__ U, const float* __restrict__ tile,\n"
+    << "      uint32_t ld, int lane, float (&s)[" << cw << "]) {\n"
+    << "    const uint32_t nu = H.n_uniform_pad;\n"
+    << "    for (int j = 0; j < " << cw << "; ++j) s[j] = 0.0f;\n";
+  // The interpreter's vm_run_acc order: instructions 1.. of the program.
+  if (L.sum_mode == SUM_AFFINE)
+    for (size_t pc = 1; pc < L.vm.size(); ++pc)
+      o << "    vm_acc_fma<" << opt << ", " << cw << ">(U, nu, tile, ld, lane, "
+        << vm_a(L.vm[pc]) << "u, " << vm_b(L.vm[pc]) << "u, s);\n";
+  o << "  }\n"
+    << "  static __device__ __forceinline__ double finish(const ChainHot& H,\n"
+    << "      const float* U, double s, bool with_const) {\n";
+  if (L.sum_tail.empty()) {
+    o << "    return s;\n";
+  } else {
+    o << "    double alpha = 1.0, beta = 0.0;\n";
+    for (const VmInstr& t : L.sum_tail)
+      o << "    vm_tail_step(" << vm_op(t) << "u, (double)U[" << vm_a(t)
+        << "], (double)U[" << vm_b(t) << "], alpha, beta);\n";
+    o << "    return vm_tail_apply(alpha, beta, s, H.n_obs, with_const);\n";
+  }
   o << "  }\n};\n}  // namespace gpum\n";
   return o.str();
 }

@@ -19,8 +19,10 @@ struct Lowered;
 enum JitKind { JIT_CHAIN_STREAM = 0, JIT_CHAIN_RESIDENT, JIT_POINTS_STREAM,
                JIT_POINTS_RESIDENT, JIT_BATCH, JIT_NUM_KINDS };
 
-// Source of the evaluator for a lowered program run with variant (ns, opt).
-std::string jit_evaluator_source(const Lowered& L, int ns, int opt);
+// Source of the evaluator for a lowered program run with variant
+// (ns, opt, cw): ns spill slots, opt observations per lane per chunk, cw
+// chains per wave.
+std::string jit_evaluator_source(const Lowered& L, int ns, int opt, int cw);
 
 // The kernel of `kind` for an evaluator source (compiled on first use).
 // On failure returns nullptr and fills *err.

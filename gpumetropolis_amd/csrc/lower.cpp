@@ -10,6 +10,7 @@
 
 #include <math.h>
 
+#include <cmath>
 #include <map>
 #include <sstream>
 
@@ -18,6 +19,116 @@
 namespace gpum {
 
 namespace {
+
+VmInstr make_instr(uint32_t op, uint32_t slot = 0, uint32_t a = 0,
+                   uint32_t b = 0) {
+  VmInstr ins;
+  ins.w0 = op | (slot << 8);
+  ins.w1 = (a & 0xFFFFu) | (b << 16);
+  return ins;
+}
+
+// Nominal VALU slots of one VM instruction per observation.
+double vm_weight(uint32_t op) {
+  switch (op) {
+    case VM_LOAD_U: case VM_LOAD_D: case VM_PUSH_U: case VM_PUSH_D:
+      return 0.0;
+    case VM_DIV_S: case VM_DIV_D: case VM_RDIV_U: case VM_RDIV_D:
+    case VM_RCP:
+      return 5.0;
+    case VM_POW_S: case VM_POW_U: case VM_POW_D: case VM_RPOW_U:
+    case VM_RPOW_D:
+      return 9.0;
+    case VM_EXP: case VM_LOG: case VM_SQRT: return 4.0;
+    case VM_LGAMMA: return 32.0;
+    case VM_CUBE: case VM_POW4: return 2.0;
+    case VM_SOFTPLUS: return 9.0;
+    default: return 1.0;
+  }
+}
+
+// A uniform slot holding the constant v (one is added if none holds it).
+int const_slot(Lowered* L, float v) {
+  for (const auto& kv : L->const_uniforms)
+    if (kv.second == v && std::signbit(kv.second) == std::signbit(v))
+      return kv.first;
+  const int slot = L->n_uniform++;
+  L->const_uniforms.push_back({slot, v});
+  return slot;
+}
+
+// Move work from the per-observation program into the summation (SumMode,
+// vm.h).  Trailing ADD_U / SUB_U / MUL_U / FMA_UU go to the sum tail; a
+// program
+// left affine in the data columns, c + sum_i U[a_i] D[b_i] (heads LOAD_U c,
+// LOAD_U a; MUL_D b, LOAD_D b; MUL_U a, or LOAD_D b; body FMA_UD, ADD_D,
+// SUB_D, ADD_U, SUB_U),
+// is rewritten as LOAD_U zero followed by one FMA_UD per product, summed
+// by accumulation, with its constants at the front of the tail; otherwise a
+// trailing SQR becomes the summation's square.
+void fold_sum(Lowered* L) {
+  std::vector<VmInstr>& P = L->vm;
+  size_t end = P.size();
+  while (end > 1) {
+    const uint32_t op = vm_op(P[end - 1]);
+    if (op != VM_ADD_U && op != VM_SUB_U && op != VM_MUL_U && op != VM_FMA_UU)
+      break;
+    --end;
+  }
+  std::vector<VmInstr> tail(P.begin() + (long)end, P.end());
+  P.resize(end);
+
+  // Affine form: products (a, b) and constants (ADD_U / SUB_U entries).
+  std::vector<std::pair<int, int>> prods;
+  std::vector<VmInstr> consts;
+  bool affine = true;
+  size_t pc = 1;
+  const uint32_t op0 = vm_op(P[0]);
+  if (op0 == VM_LOAD_U && P.size() > 1 && vm_op(P[1]) == VM_MUL_D) {
+    prods.push_back({(int)vm_a(P[0]), (int)vm_a(P[1])});
+    pc = 2;
+  } else if (op0 == VM_LOAD_D && P.size() > 1 && vm_op(P[1]) == VM_MUL_U) {
+    prods.push_back({(int)vm_a(P[1]), (int)vm_a(P[0])});
+    pc = 2;
+  } else if (op0 == VM_LOAD_U) {
+    consts.push_back(make_instr(VM_ADD_U, 0, vm_a(P[0])));
+  } else if (op0 == VM_LOAD_D) {
+    prods.push_back({-1, (int)vm_a(P[0])});
+  } else {
+    affine = false;
+  }
+  for (; affine && pc < P.size(); ++pc) {
+    const uint32_t op = vm_op(P[pc]);
+    if (op == VM_FMA_UD)
+      prods.push_back({(int)vm_a(P[pc]), (int)vm_b(P[pc])});
+    else if (op == VM_ADD_D)
+      prods.push_back({-1, (int)vm_a(P[pc])});
+    else if (op == VM_SUB_D)
+      prods.push_back({-2, (int)vm_a(P[pc])});
+    else if (op == VM_ADD_U || op == VM_SUB_U)
+      consts.push_back(make_instr(op, 0, vm_a(P[pc])));
+    else
+      affine = false;
+  }
+  if (affine) {
+    std::vector<VmInstr> Q;
+    Q.push_back(make_instr(VM_LOAD_U, 0, (uint32_t)const_slot(L, 0.0f)));
+    for (auto& pr : prods) {
+      int a = pr.first;
+      if (a == -1) a = const_slot(L, 1.0f);
+      if (a == -2) a = const_slot(L, -1.0f);
+      Q.push_back(make_instr(VM_FMA_UD, 0, (uint32_t)a, (uint32_t)pr.second));
+    }
+    P.swap(Q);
+    L->vm_depth = 1;
+    L->sum_mode = SUM_AFFINE;
+    tail.insert(tail.begin(), consts.begin(), consts.end());
+  } else if (P.size() > 1 && vm_op(P.back()) == VM_SQR) {
+    P.pop_back();
+    L->sum_mode = SUM_SQUARE;
+  }
+  L->sum_tail = tail;
+}
 
 enum Kind { K_CONST = 0, K_PARAM = 1, K_DATA = 2, K_MIXED = 3 };
 
@@ -292,27 +403,7 @@ struct Emitter {
   }
 
   void put(uint32_t op, uint32_t slot = 0, uint32_t a = 0, uint32_t bb = 0) {
-    VmInstr ins;
-    ins.w0 = op | (slot << 8);
-    ins.w1 = (a & 0xFFFFu) | (bb << 16);
-    out->vm.push_back(ins);
-    double w = 0.0;
-    switch (op) {
-      case VM_LOAD_U: case VM_LOAD_D: case VM_PUSH_U: case VM_PUSH_D:
-        w = 0.0; break;
-      case VM_DIV_S: case VM_DIV_D: case VM_RDIV_U: case VM_RDIV_D:
-      case VM_RCP:
-        w = 5.0; break;
-      case VM_POW_S: case VM_POW_U: case VM_POW_D: case VM_RPOW_U:
-      case VM_RPOW_D:
-        w = 9.0; break;
-      case VM_EXP: case VM_LOG: case VM_SQRT: w = 4.0; break;
-      case VM_LGAMMA: w = 32.0; break;
-      case VM_CUBE: case VM_POW4: w = 2.0; break;
-      case VM_SOFTPLUS: w = 9.0; break;
-      default: w = 1.0;
-    }
-    out->w_exec += w;
+    out->vm.push_back(make_instr(op, slot, a, bb));
   }
 
   void bump(int depth) {
@@ -602,7 +693,11 @@ std::string lower_program(const uint32_t* code, size_t n_words,
   em.value(b.root, 0);
   if (!em.err.empty()) return em.err;
   out->vm_depth = em.maxdepth;
-  if ((int)out->vm.size() > kMaxVmInstr) return "lowered program too long";
+  if (opt.fuse) fold_sum(out);
+  out->w_exec = out->sum_mode == SUM_SQUARE ? 1.0 : 0.0;
+  for (const VmInstr& ins : out->vm) out->w_exec += vm_weight(vm_op(ins));
+  if ((int)(out->vm.size() + out->sum_tail.size()) > kMaxVmInstr)
+    return "lowered program too long";
   return "";
 }
 
@@ -620,7 +715,13 @@ std::string disassemble(const Lowered& L) {
     s << (op < VM_NUM_OPS ? names[op] : "?") << " slot=" << vm_slot(i)
       << " a=" << vm_a(i) << " b=" << vm_b(i) << "\n";
   }
-  s << "hoist:";
+  static const char* modes[] = {"value", "square", "affine"};
+  s << "sum: " << (L.sum_mode < 3 ? modes[L.sum_mode] : "?") << ", tail";
+  for (const VmInstr& i : L.sum_tail) {
+    uint32_t op = vm_op(i);
+    s << " " << (op < VM_NUM_OPS ? names[op] : "?") << " " << vm_a(i);
+  }
+  s << "\nhoist:";
   for (size_t k = 0; k < L.hoist.size(); k += 2)
     s << " (" << L.hoist[k] << "," << L.hoist[k + 1] << ")";
   s << "\nn_uniform=" << L.n_uniform << " depth=" << L.vm_depth

@@ -14,7 +14,8 @@ struct LowerOptions {
   // stream them as extra FP32 columns instead of recomputing them per
   // iteration.
   bool precompute_data = true;
-  // Fuse multiply-add patterns and reciprocal/negation of hoisted values.
+  // Fuse multiply-add patterns and reciprocal/negation of hoisted values,
+  // and fold the program's tail into the summation (SumMode in vm.h).
   bool fuse = true;
 };
 
@@ -27,6 +28,10 @@ struct PrecomputedColumn {
 struct Lowered {
   std::vector<VmInstr> vm;  // per-observation register program
   int vm_depth = 0;         // peak values live (T + spill slots)
+  // How the kernel sums the program over a chunk (SumMode), and the
+  // ADD_U / SUB_U / MUL_U operations applied to a data pass's sum.
+  uint32_t sum_mode = SUM_VALUE;
+  std::vector<VmInstr> sum_tail;
   // FP64 program run once per proposal: reference opcodes over the
   // parameters plus OP_STORE_U* ops writing the uniform table.
   std::vector<uint32_t> hoist;
@@ -37,6 +42,9 @@ struct Lowered {
   std::vector<PrecomputedColumn> extra_cols;
   int n_cols_in = 0;
   // VALU weights per observation evaluation (nominal slot counts):
+  // w_exec counts the operations the kernel issues per observation besides
+  // the summation's add (a summation fused with an operation counts as
+  // that operation).
   double w_exec = 0.0;      // of the VM program the kernel executes
   double w_bytecode = 0.0;  // of the program as compiled by the DSL
   int ref_depth = 0;        // stack depth of the reference program

@@ -75,8 +75,10 @@ struct DevBuf {
   }
 };
 
+// Kernel variant: spill slots, observations per lane per chunk, chains per
+// wave.
 struct Variant {
-  int ns, opt;
+  int ns, opt, cw;
 };
 
 Variant pick_variant(int vm_depth);
@@ -109,8 +111,8 @@ struct gpum_hip_model {
   gpum::rt::DevBuf<float> data;
   uint32_t data_ld = 0;
   uint32_t data_chunk = 0;  // layout granularity the data was padded to
-  // Per-model compiled evaluator (empty: the interpreter runs).
-  std::string jit_eval;
+  // Launches run per-model compiled kernels (else the interpreter).
+  bool jit = false;
 };
 
 namespace gpum {
@@ -123,20 +125,34 @@ struct TilingForce {
 };
 
 // Tiling of a launch over n_chains chains (or, with `points`, of the points
-// kernel over n_chains slots: one wave per slot, 16 slots per workgroup).
-// With `force`: the given variant, always streamed, room for extra static
-// LDS.
+// kernel over n_chains slots: one wave per `cw` slots, 16 waves per
+// workgroup).  With `force`: the given variant, always streamed, room for
+// extra static LDS.  `cw_req`: chains per wave asked for (0: automatic;
+// ext->chains_per_wave takes its place when set).
 bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
                  const gpum_run_ext* ext, Tiling* t, bool points = false,
-                 const TilingForce* force = nullptr);
+                 const TilingForce* force = nullptr, int cw_req = 0);
 // Lay the model's columns out on the device with column stride n_pad.
 bool ensure_data(gpum_hip_model* m, uint32_t n_pad);
 
-// The model's compiled kernel of `kind` when it has one and the launch can
-// use it (module kernels get at most 64 KB of dynamic LDS), else nullptr;
-// sets *failed (and the error) when compilation fails.
+// The summation fields of a launch's hot values (the tail is uploaded
+// after the program's padding instruction).
+inline void set_sum_fields(const gpum_hip_model* m, ChainHot* h) {
+  h->sum_mode = m->L.empty ? (uint32_t)SUM_VALUE : m->L.sum_mode;
+  h->n_sum_tail = m->L.empty ? 0u : (uint32_t)m->L.sum_tail.size();
+}
+
+// The model's compiled kernel of `kind` for variant v when the model runs
+// compiled kernels and the launch can use one (module kernels get at most
+// 64 KB of dynamic LDS), else nullptr; sets *failed (and the error) when
+// compilation fails.
 hipFunction_t model_jit_kernel(const gpum_hip_model* m, int kind,
-                               size_t lds_bytes, bool* failed);
+                               const Variant& v, size_t lds_bytes,
+                               bool* failed);
+// Waves of a workgroup of tiling t.
+inline uint32_t tiling_waves(const Tiling& t) {
+  return t.cpb / (uint32_t)t.v.cw * t.wpc;
+}
 hipError_t launch_module(hipFunction_t f, dim3 grid, dim3 block,
                          size_t lds_bytes, hipStream_t s, void** args);
 

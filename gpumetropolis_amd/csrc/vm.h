@@ -85,6 +85,20 @@ enum VmOp : uint32_t {
   VM_NUM_OPS
 };
 
+// How a chunk of observations is summed.  The kernels need the sum of the
+// program's value over the observations, not the values, so with fusion
+// the lowering moves work into that sum (lower.cpp, fold_sum):
+//   SUM_VALUE   s += T
+//   SUM_SQUARE  s = fma(T, T, s): the program's trailing SQR
+//   SUM_AFFINE  the program is c + sum_i U[a_i] * D[b_i]: no T at all,
+//               s = fma(U[a_i], D[b_i], s) for every FMA_UD of the
+//               program (its first instruction loads a zero) and c joins
+//               the tail below.
+// Trailing ADD_U / SUB_U / MUL_U / FMA_UU leave the per-observation program
+// and are applied once to the data pass's sum (the "sum tail"), in FP64:
+//   sum(T + u) = sum(T) + n u,   sum(u T) = u sum(T).
+enum SumMode : uint32_t { SUM_VALUE = 0, SUM_SQUARE = 1, SUM_AFFINE = 2 };
+
 // One VM instruction: two u32 words.
 //   w0 = op | slot << 8
 //   w1 = a | b << 16

@@ -101,6 +101,9 @@ typedef struct gpum_run_ext {
   double* final_sd;       /* out: n_chains * n_params adapted proposal sd */
   double* final_scale;    /* out: n_chains RM scales */
   double* trace_ll;       /* out: n_iter * n_chains data term of each proposal */
+  /* Chains evaluated together by one wave on every observation it reads
+   * (0 = auto; 2 needs a program without a stack: vm_depth 1). */
+  int32_t chains_per_wave;
 } gpum_run_ext;
 
 /* Static description of a lowered model. */
@@ -113,6 +116,12 @@ typedef struct gpum_model_info {
   uint32_t n_hoist_words;  /* FP64 hoist program length (u32 words) */
   double w_exec;           /* nominal VALU slots per observation evaluation */
   double w_bytecode;       /* same weights on the program as compiled */
+  /* How the kernel sums the program over observations: 0 its value, 1 the
+   * square of its value before a trailing square, 2 an affine program
+   * accumulated directly; n_sum_tail trailing uniform operations are
+   * applied once to each data sum instead of per observation. */
+  uint32_t sum_mode;
+  uint32_t n_sum_tail;
 } gpum_model_info;
 
 typedef struct gpum_hip_model gpum_hip_model;
@@ -271,6 +280,7 @@ typedef struct gpum_da_ext {
   double exact_evals;       /* out: proposals evaluated exactly */
   double* trace_prop;       /* out: n_iter * n_chains * n_params proposals */
   double* trace_ll;         /* out: n_iter * n_chains data term (NaN: not run) */
+  uint32_t chains_per_wave; /* exact-stage slots per wave (0 auto, 1 or 2) */
 } gpum_da_ext;
 
 /* Random-walk Metropolis (proposal_mode 0 or 2) run one iteration per
@@ -325,11 +335,15 @@ typedef struct gpum_batch_opts {
                                  batch absorbs the remainder */
   uint32_t warmup_batches;    /* leading batches that adapt */
   int32_t adapt;
-  int32_t chains_per_block;   /* 0 = 16; chains_per_block * waves_per_chain = 16 */
+  int32_t chains_per_block;   /* 0 = 16 * chains_per_wave;
+                                 chains_per_block / chains_per_wave *
+                                 waves_per_chain = 16 */
   int32_t waves_per_chain;
   int32_t use_graph;          /* capture all batches in one HIP graph */
   double kernel_ms;           /* out: device time of the run */
   uint32_t n_workgroups;      /* out: workgroups per batch launch */
+  int32_t chains_per_wave;    /* 0 or 1: one chain per wave; 2: two (every
+                                 model of vm_depth 1) */
 } gpum_batch_opts;
 
 int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,

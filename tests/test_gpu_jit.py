@@ -87,3 +87,60 @@ def test_compiled_batch_equals_interpreter(hip):
                              jit=True, **kw)
     for fa, fb in zip(a.fits, b.fits):
         assert np.array_equal(fa.draws, fb.draws)
+
+
+def _pair_cases():
+    rng = np.random.default_rng(11)
+    out = []
+    g = workloads.gaussian_mean(n=3000)                     # resident, square
+    out.append(("gaussian", g, rng.normal(3.4, 0.05, (7, 1)), 0.03))
+    ga = workloads.gamma(n=40000)                           # streamed, affine
+    out.append(("gamma", ga, np.log([2.5, 0.7]) + rng.normal(0, 0.01, (7, 2)), 0.005))
+    rg = workloads.regressions(n_models=1, seed=5, n_min=30011, n_max=30011)[0]
+    out.append(("regression", rg, rg.meta["true"] + 0.01 * rng.standard_normal((7, 10)), 0.003))
+    return out
+
+
+@pytest.mark.parametrize("jit", [False, True], ids=["vm", "jit"])
+@pytest.mark.parametrize("case", _pair_cases(), ids=lambda c: c[0])
+def test_two_chains_per_wave_equal_one(hip, case, jit):
+    """A wave evaluating two chains on each observation it reads does each
+    chain's arithmetic exactly as a wave of one chain: the same waves per
+    chain give identical data terms, decisions and draws (7 chains: the
+    last pair has an idle member)."""
+    name, wl, init, step = case
+    print(f"[pairs] {name} jit={jit}", flush=True)
+    m = wl.model
+    dm = hip.DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
+                         m.prior.consts, m.n_params, wl.data, wl.n_cols,
+                         wl.n_obs, jit=jit)
+    C = init.shape[0]
+    sd = np.full_like(init, step)
+    for wpc in (2, 4):
+        a = dm.run(init, sd, np.ones(C), 40, 3, trace=True, tiling=(16 // wpc, wpc, 1))
+        b = dm.run(init, sd, np.ones(C), 40, 3, trace=True, tiling=(32 // wpc, wpc, 2))
+        assert np.array_equal(a.trace_plp, b.trace_plp, equal_nan=True), wpc
+        assert np.array_equal(a.draws, b.draws), wpc
+    # the points kernel (one slice, so the sums are ordered alike)
+    c = dm.run_da(None, init, np.ones(C), 20, 7, proposal_sd=sd, trace=True,
+                  work_items=1, chains_per_wave=1)
+    d = dm.run_da(None, init, np.ones(C), 20, 7, proposal_sd=sd, trace=True,
+                  work_items=1, chains_per_wave=2)
+    assert np.array_equal(c.trace_plp, d.trace_plp, equal_nan=True)
+    assert np.array_equal(c.draws, d.draws)
+    dm.close()
+
+
+def test_two_chains_per_wave_batch(hip):
+    print("[pairs] batch", flush=True)
+    wls = workloads.regressions(n_models=3, seed=32, n_min=2000, n_max=40000)
+    rng = np.random.default_rng(4)
+    inits = [w.meta["true"] + 0.01 * rng.standard_normal((32, 10)) for w in wls]
+    kw = dict(n_chains=32, n_iter=80, warmup=40, de_every=40, init=inits,
+              proposal_sd=0.003, seed=2)
+    ms, ds = [w.model for w in wls], [w.data for w in wls]
+    a = gpu_metropolis_batch(ms, ds, tiling=(8, 2), **kw)
+    for jit in (False, True):
+        b = gpu_metropolis_batch(ms, ds, tiling=(16, 2, 2), jit=jit, **kw)
+        for fa, fb in zip(a.fits, b.fits):
+            assert np.array_equal(fa.draws, fb.draws)

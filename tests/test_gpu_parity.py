@@ -63,6 +63,51 @@ def test_lowering_switches_agree(hip, orc, precompute, fuse):
     assert (np.abs(got - ref) / np.abs(ref)).max() <= REL_TOL
 
 
+# Every summation form of the lowering (affine programs accumulated
+# directly, trailing squares, sum tails of uniform operations) against the
+# FP64 evaluation, with and without the fold, on sizes that end in a
+# partial chunk.
+FOLD_FORMS = [
+    ("~ y * mu - 3 * x + 2 - s", [0.7, 1.3]),         # affine, constants in the tail
+    ("~ mu * x - y + s", [1.1, -0.4]),                 # affine, SUB_D term
+    ("~ 2 * (x - mu) - s", [0.2, 0.5]),                # affine, tail MUL then SUB
+    ("~ -log(s) * 2 + 0 * y", [0.0, 1.7]),             # constant term only
+    ("~ (y - mu)^2 * s + 4 - s", [0.3, 0.8]),          # square, tail MUL ADD SUB
+    ("~ (exp(y - mu) + x) * s", [0.1, 0.9]),           # value, tail MUL
+    ("~ log(1 + exp(y * mu)) - x * s", [0.4, 0.2]),    # no fold
+]
+
+
+@pytest.mark.parametrize("n_obs", [1025, 70001])
+@pytest.mark.parametrize("form", FOLD_FORMS, ids=lambda f: f[0])
+def test_summation_forms_match_fp64(hip, orc, form, n_obs):
+    formula, centre = form
+    m = dsl.gpum_model(formula, ["mu", "s"], ["y", "x"])
+    rng = np.random.default_rng(n_obs)
+    data = np.column_stack([rng.normal(0.5, 1.0, n_obs), rng.normal(-0.2, 1.5, n_obs)])
+    data = data.astype(np.float32).astype(np.float64)
+    pts = _points(rng, centre, 0.1, 8)
+    pts[:, 1] = np.abs(pts[:, 1])
+    ref = orc.loglik_batch(m.loglik.code, m.loglik.consts, 2, data, 2, n_obs, pts)
+    print(f"[fold] {formula} n={n_obs}", flush=True)
+    scale = np.abs(ref) + 1e-7 * n_obs
+    for fuse in (1, 0):
+        dm = hip.DeviceModel(m.loglik.code, m.loglik.consts, [], [], 2, data, 2,
+                             n_obs, fuse=fuse)
+        got = dm.loglik(pts)
+        assert np.max(np.abs(got - ref) / scale) <= REL_TOL, (fuse, got, ref)
+        if fuse and dm.info()["vm_depth"] <= 1:
+            # two chains per wave: the proposals' data terms
+            r = dm.run(pts, np.full_like(pts, 0.01), np.ones(8), 3, 1,
+                       trace=True, tiling=(16, 1, 2))
+            props = r.trace_prop.reshape(-1, 2)
+            ref2 = orc.loglik_batch(m.loglik.code, m.loglik.consts, 2, data, 2,
+                                    n_obs, props)
+            sc2 = np.abs(ref2) + 1e-7 * n_obs
+            assert np.max(np.abs(r.trace_ll.ravel() - ref2) / sc2) <= REL_TOL
+        dm.close()
+
+
 @pytest.mark.parametrize("n_obs", [1, 7, 255, 1023, 1025, 4097, 70001])
 def test_ragged_sizes(hip, orc, n_obs):
     rng = np.random.default_rng(n_obs)
