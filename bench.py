@@ -32,6 +32,16 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 VALU_PEAK = 256 * 64 * 2.4e9  # FP32 lane-ops/s, non-packed (one slot per op)
+
+# HBM bytes per launch of the headline kernel, from two rocprofv3 --pmc
+# passes over this script's default launches (FETCH_SIZE scaled x2.0 and
+# WRITE_SIZE x1.0 by a 1 GiB copy); the summary and commands are in
+# profiles/compiled_chain_kernel_gamma_1e6obs_4096chains.md.  Reported only
+# for the launch shape it was measured on.
+PROFILED_TRAFFIC = {
+    ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 15_734_562 * 1024 * 2.0,
+                                         "write_bytes": 2_425_142 * 1024 * 1.0},
+}
 HBM_PEAK = 8.0e12             # B/s
 
 
@@ -379,6 +389,9 @@ def main():
     cpb = 16
     n_wg = math.ceil(C / cpb)
     bytes_per_launch = (a.iters_per_step * n_wg * N * info["n_cols_total"] * 4)
+    rec = PROFILED_TRAFFIC.get(("compiled" if a.jit else "interpreter", C, N,
+                                a.iters_per_step))
+    traffic = rec["fetch_bytes"] + rec["write_bytes"] if rec else None
     out = {
         "metric": "ESS/s at R-hat<1.01 + obs-loglik evals/s; plain vs "
                   "surrogate-DA, 1 MI355X",
@@ -410,7 +423,10 @@ def main():
             "peak": VALU_PEAK / 1e12,
             "unit": "T VALU-slot/s",
             "frac": achieved_slots / VALU_PEAK,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this "
+                               "launch shape, scaled by a copy probe"
+                               if traffic else None),
             "w_exec": w_exec,
             "w_bytecode": info["w_bytecode"],
             "kernel": ("gpum_jit_kernel (chain_body, compiled gamma program)"

@@ -394,7 +394,15 @@ class DeviceModel:
         self._h = h
         self.jit = False
         self.jit_compile_ms = 0.0
-        if jit:
+        self.jit_error = ""
+        if jit == "auto":
+            # Compiled kernels where run-time compilation works in this
+            # process, else the interpreter (the reason is kept).
+            try:
+                self.set_jit(True)
+            except BackendError as e:
+                self.jit_error = str(e)
+        elif jit:
             self.set_jit(True)
 
     def set_jit(self, enable: bool) -> float:

@@ -92,7 +92,7 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                    de_noise: float = 1e-3, de_every=None, de_sync=False,
                    seed: int = 1, backend: str = "hip",
                    tiling=None, ladder_size=None, surrogate=None,
-                   engine: str = "auto", jit: bool = False) -> Fit:
+                   engine: str = "auto", jit="auto") -> Fit:
     if backend not in ("hip", "auto"):
         raise ValueError(f"backend '{backend}' is not available here; this "
                          "package is the hip backend")
@@ -165,6 +165,9 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
         dm.close()
     fit.kernel_ms = call.kernel_ms
     fit.adaptation["engine"] = engine
+    fit.adaptation["compiled"] = dm.jit
+    if dm.jit_error:
+        fit.adaptation["compile_error"] = dm.jit_error
     return fit
 
 

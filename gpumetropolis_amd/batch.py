@@ -42,7 +42,7 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
                          adapt: bool = True, seed: int = 1, tiling="auto",
                          use_graph: bool = True,
                          tile_cache: Optional[str] = None,
-                         jit: bool = False) -> BatchFit:
+                         jit="auto") -> BatchFit:
     if method not in ("de", "rwm"):
         raise ValueError('batched runs take method "de" or "rwm"')
     if len(models) != len(datas) or not models:
@@ -89,8 +89,9 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
             gammas.append(g)
         mode = 1 if method == "de" else 0
         infos = [dm.info() for dm in dms]
+        compiled = all(d.jit for d in dms)
         key = autotune.tile_key([i["n_vm_instr"] for i in infos],
-                                [d.n_obs for d in dms]) + ("-jit" if jit else "")
+                                [d.n_obs for d in dms]) + ("-jit" if compiled else "")
         timings = {}
         if tiling == "auto":
             def time_candidate(t):
@@ -100,7 +101,7 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
                                    use_graph=False)
                 return inf["kernel_ms"]
             cands = autotune.CANDIDATES
-            if jit and all(i["vm_depth"] <= 1 for i in infos):
+            if compiled and all(i["vm_depth"] <= 1 for i in infos):
                 cands = cands + autotune.PAIR_CANDIDATES
             tiling, timings = autotune.choose(key, time_candidate,
                                               candidates=cands,

@@ -314,30 +314,31 @@ __device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
 
 // The wave's share of one tile for its kCw chains: chunks part, part + wpc,
 // ... of 64*OPT observations; FP32 sum per chunk, FP64 across chunks, added
-// to acc[j].
-template <class E>
-__device__ __forceinline__ void tile_pass(const ChainHot& H,
-                                          const float* __restrict__ tile,
-                                          uint32_t t0, uint32_t part,
-                                          uint32_t wpc, const float* U,
-                                          int lane, double (&acc)[E::kCw]) {
+// to acc[j].  One loop per summation mode, so the interpreter's loop carries
+// only its mode's registers.
+template <class E, uint32_t MODE>
+__device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
+                                               const float* __restrict__ tile,
+                                               uint32_t t0, uint32_t part,
+                                               uint32_t wpc, const float* U,
+                                               int lane,
+                                               double (&acc)[E::kCw]) {
   constexpr int OPT = E::kOpt, CW = E::kCw;
   const uint32_t chunk = 64u * OPT;
   const uint32_t nchunks = H.tile_len / chunk;
-  const uint32_t mode = E::mode(H);
   for (uint32_t ck = part; ck < nchunks; ck += wpc) {
     const uint32_t base = ck * chunk;
     const uint32_t first = t0 + base;
     float s[CW];
     if (first + chunk <= H.n_obs) {
-      if (mode == SUM_AFFINE) {
+      if constexpr (MODE == SUM_AFFINE) {
         E::run_acc(H, U, tile + base, H.tile_len, lane, s);
       } else {
         float T[CW][OPT];
         E::run(H, U, tile + base, H.tile_len, lane, T);
 #pragma unroll
         for (int j = 0; j < CW; ++j) {
-          if (mode == SUM_SQUARE) {
+          if constexpr (MODE == SUM_SQUARE) {
             s[j] = 0.0f;
 #pragma unroll
             for (int k = 0; k < OPT; ++k) s[j] = fmaf(T[j][k], T[j][k], s[j]);
@@ -359,13 +360,31 @@ __device__ __forceinline__ void tile_pass(const ChainHot& H,
         for (int k = 0; k < OPT; ++k) {
           uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
           const float v = obs < H.n_obs ? T[j][k] : 0.0f;
-          s[j] = mode == SUM_SQUARE ? fmaf(v, v, s[j]) : s[j] + v;
+          if constexpr (MODE == SUM_SQUARE)
+            s[j] = fmaf(v, v, s[j]);
+          else
+            s[j] = s[j] + v;
         }
       }
     }
 #pragma unroll
     for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
   }
+}
+
+template <class E>
+__device__ __forceinline__ void tile_pass(const ChainHot& H,
+                                          const float* __restrict__ tile,
+                                          uint32_t t0, uint32_t part,
+                                          uint32_t wpc, const float* U,
+                                          int lane, double (&acc)[E::kCw]) {
+  const uint32_t mode = E::mode(H);
+  if (mode == SUM_AFFINE)
+    tile_pass_mode<E, SUM_AFFINE>(H, tile, t0, part, wpc, U, lane, acc);
+  else if (mode == SUM_SQUARE)
+    tile_pass_mode<E, SUM_SQUARE>(H, tile, t0, part, wpc, U, lane, acc);
+  else
+    tile_pass_mode<E, SUM_VALUE>(H, tile, t0, part, wpc, U, lane, acc);
 }
 
 // One workgroup's chains chain0 .. chain0 + cpb - 1 through the whole run.

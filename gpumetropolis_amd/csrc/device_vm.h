@@ -139,87 +139,83 @@ __device__ __forceinline__ VmIns4 vm_fetch(vm_prog_t prog, uint32_t pc) {
   return r;
 }
 
-// The data column an instruction reads, or -1.
-__device__ __forceinline__ int vm_col(uint32_t op, uint32_t a, uint32_t b) {
-  switch (op) {
-    case VM_LOAD_D: case VM_PUSH_D:
-    case VM_ADD_D: case VM_SUB_D: case VM_MUL_D: case VM_DIV_D: case VM_POW_D:
-    case VM_RSUB_D: case VM_RDIV_D: case VM_RPOW_D:
-    case VM_FMA_DU:
-      return (int)a;
-    case VM_FMA_UD:
-      return (int)b;
-    default:
-      return -1;
-  }
-}
-
-// One VM instruction on one chunk of one chain, its data column (if any)
-// already in D.  The interpreter calls it with the fetched opcode; code
+// One VM instruction on one chunk of CW chains (chain j's uniform table is
+// U + j * nu).  An instruction that reads a data column reads it from LDS
+// once for all CW chains; each chain's arithmetic is the single-chain
+// arithmetic.  The interpreter calls it with the fetched opcode; code
 // generated per model calls it with constants, so the switch folds away and
 // each program becomes straight-line code.
-template <int NS, int OPT>
-__device__ __forceinline__ void vm_apply(uint32_t op, uint32_t slot, uint32_t a,
-                                         uint32_t b, const float* __restrict__ U,
-                                         const float (&D)[OPT], float (&T)[OPT],
-                                         float (&S)[NS > 0 ? NS : 1][OPT]) {
+template <int NS, int OPT, int CW>
+__device__ __forceinline__ void vm_step(uint32_t op, uint32_t slot, uint32_t a,
+                                        uint32_t b, const float* __restrict__ U,
+                                        uint32_t nu,
+                                        const float* __restrict__ tile,
+                                        uint32_t ld, int lane,
+                                        float (&T)[CW][OPT],
+                                        float (&S)[CW][NS > 0 ? NS : 1][OPT],
+                                        float (&D)[OPT]) {
   switch (op) {
-    case VM_LOAD_U: {
-      float u = U[a];
+    case VM_LOAD_U:
 #pragma unroll
-      for (int k = 0; k < OPT; ++k) T[k] = u;
+      for (int j = 0; j < CW; ++j) {
+        const float u = U[j * nu + a];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[j][k] = u;
+      }
       break;
-    }
     case VM_LOAD_D:
+      vm_load_col<OPT>(tile + a * ld, lane, T[0]);
 #pragma unroll
-      for (int k = 0; k < OPT; ++k) T[k] = D[k];
+      for (int j = 1; j < CW; ++j)
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[j][k] = T[0][k];
       break;
     case VM_PUSH_U:
       if constexpr (NS > 0) {
-        vm_spill<0, NS, OPT>(slot, S, T);
-        float u = U[a];
 #pragma unroll
-        for (int k = 0; k < OPT; ++k) T[k] = u;
+        for (int j = 0; j < CW; ++j) {
+          vm_spill<0, NS, OPT>(slot, S[j], T[j]);
+          const float u = U[j * nu + a];
+#pragma unroll
+          for (int k = 0; k < OPT; ++k) T[j][k] = u;
+        }
       }
       break;
     case VM_PUSH_D:
       if constexpr (NS > 0) {
-        vm_spill<0, NS, OPT>(slot, S, T);
 #pragma unroll
-        for (int k = 0; k < OPT; ++k) T[k] = D[k];
+        for (int j = 0; j < CW; ++j) vm_spill<0, NS, OPT>(slot, S[j], T[j]);
+        vm_load_col<OPT>(tile + a * ld, lane, T[0]);
+#pragma unroll
+        for (int j = 1; j < CW; ++j)
+#pragma unroll
+          for (int k = 0; k < OPT; ++k) T[j][k] = T[0][k];
       }
       break;
-    case VM_ADD_S:
-      if constexpr (NS > 0)
-        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s + t; });
-      break;
-    case VM_SUB_S:
-      if constexpr (NS > 0)
-        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s - t; });
-      break;
-    case VM_MUL_S:
-      if constexpr (NS > 0)
-        vm_bin_s<0, NS, OPT>(slot, S, T, [](float s, float t) { return s * t; });
-      break;
-    case VM_DIV_S:
-      if constexpr (NS > 0)
-        vm_bin_s<0, NS, OPT>(slot, S, T,
-                             [](float s, float t) { return s * vm_rcp(t); });
-      break;
-    case VM_POW_S:
-      if constexpr (NS > 0)
-        vm_bin_s<0, NS, OPT>(slot, S, T,
-                             [](float s, float t) { return vm_pow(s, t); });
-      break;
-#define GPUM_UOP(NAME, EXPR)                          \
-case NAME: {                                        \
-  float u = U[a];                                   \
-  _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
-    float t = T[k];                                 \
-    T[k] = (EXPR);                                  \
-  }                                                 \
-  break;                                            \
-}
+#define GPUM_SOP(NAME, EXPR)                                           \
+case NAME:                                                             \
+  if constexpr (NS > 0) {                                              \
+    _Pragma("unroll") for (int j = 0; j < CW; ++j)                     \
+        vm_bin_s<0, NS, OPT>(slot, S[j], T[j],                         \
+                             [](float s, float t) { return (EXPR); }); \
+  }                                                                    \
+  break;
+    GPUM_SOP(VM_ADD_S, s + t)
+    GPUM_SOP(VM_SUB_S, s - t)
+    GPUM_SOP(VM_MUL_S, s * t)
+    GPUM_SOP(VM_DIV_S, s * vm_rcp(t))
+    GPUM_SOP(VM_POW_S, vm_pow(s, t))
+#undef GPUM_SOP
+#define GPUM_UOP(NAME, EXPR)                              \
+case NAME:                                              \
+  _Pragma("unroll") for (int j = 0; j < CW; ++j) {      \
+    const float u = U[j * nu + a];                      \
+    _Pragma("unroll") for (int k = 0; k < OPT; ++k) {   \
+      const float t = T[j][k];                          \
+      T[j][k] = (EXPR);                                 \
+    }                                                   \
+  }                                                     \
+  break;
     GPUM_UOP(VM_ADD_U, t + u)
     GPUM_UOP(VM_SUB_U, t - u)
     GPUM_UOP(VM_MUL_U, t * u)
@@ -228,15 +224,17 @@ case NAME: {                                        \
     GPUM_UOP(VM_RDIV_U, u * vm_rcp(t))
     GPUM_UOP(VM_RPOW_U, vm_pow(u, t))
 #undef GPUM_UOP
-#define GPUM_DOP(NAME, EXPR)                          \
-case NAME: {                                        \
-  _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
-    float t = T[k];                                 \
-    float d = D[k];                                 \
-    T[k] = (EXPR);                                  \
-  }                                                 \
-  break;                                            \
-}
+#define GPUM_DOP(NAME, EXPR)                              \
+case NAME:                                              \
+  vm_load_col<OPT>(tile + a * ld, lane, D);             \
+  _Pragma("unroll") for (int j = 0; j < CW; ++j) {      \
+    _Pragma("unroll") for (int k = 0; k < OPT; ++k) {   \
+      const float t = T[j][k];                          \
+      const float d = D[k];                             \
+      T[j][k] = (EXPR);                                 \
+    }                                                   \
+  }                                                     \
+  break;
     GPUM_DOP(VM_ADD_D, t + d)
     GPUM_DOP(VM_SUB_D, t - d)
     GPUM_DOP(VM_MUL_D, t * d)
@@ -246,14 +244,15 @@ case NAME: {                                        \
     GPUM_DOP(VM_RDIV_D, d * vm_rcp(t))
     GPUM_DOP(VM_RPOW_D, vm_pow(d, t))
 #undef GPUM_DOP
-#define GPUM_UNOP(NAME, EXPR)                         \
-case NAME: {                                        \
-  _Pragma("unroll") for (int k = 0; k < OPT; ++k) { \
-    float t = T[k];                                 \
-    T[k] = (EXPR);                                  \
-  }                                                 \
-  break;                                            \
-}
+#define GPUM_UNOP(NAME, EXPR)                             \
+case NAME:                                              \
+  _Pragma("unroll") for (int j = 0; j < CW; ++j) {      \
+    _Pragma("unroll") for (int k = 0; k < OPT; ++k) {   \
+      const float t = T[j][k];                          \
+      T[j][k] = (EXPR);                                 \
+    }                                                   \
+  }                                                     \
+  break;
     GPUM_UNOP(VM_NEG, -t)
     GPUM_UNOP(VM_EXP, vm_exp(t))
     GPUM_UNOP(VM_LOG, vm_log(t))
@@ -265,46 +264,35 @@ case NAME: {                                        \
     GPUM_UNOP(VM_RCP, vm_rcp(t))
     GPUM_UNOP(VM_SOFTPLUS, vm_softplus(t))
 #undef GPUM_UNOP
-    case VM_FMA_UU: {
-      float ua = U[a], ub = U[b];
+    case VM_FMA_UU:
 #pragma unroll
-      for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], ua, ub);
-      break;
-    }
-    case VM_FMA_UD: {
-      float ua = U[a];
+      for (int j = 0; j < CW; ++j) {
+        const float ua = U[j * nu + a], ub = U[j * nu + b];
 #pragma unroll
-      for (int k = 0; k < OPT; ++k) T[k] = fmaf(ua, D[k], T[k]);
+        for (int k = 0; k < OPT; ++k) T[j][k] = fmaf(T[j][k], ua, ub);
+      }
       break;
-    }
-    case VM_FMA_DU: {
-      float ub = U[b];
+    case VM_FMA_UD:
+      vm_load_col<OPT>(tile + b * ld, lane, D);
 #pragma unroll
-      for (int k = 0; k < OPT; ++k) T[k] = fmaf(T[k], D[k], ub);
+      for (int j = 0; j < CW; ++j) {
+        const float ua = U[j * nu + a];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[j][k] = fmaf(ua, D[k], T[j][k]);
+      }
       break;
-    }
+    case VM_FMA_DU:
+      vm_load_col<OPT>(tile + a * ld, lane, D);
+#pragma unroll
+      for (int j = 0; j < CW; ++j) {
+        const float ub = U[j * nu + b];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[j][k] = fmaf(T[j][k], D[k], ub);
+      }
+      break;
     default:
       break;
   }
-}
-
-// One VM instruction on one chunk of CW chains: the data column is read
-// from LDS once and used by every chain (chain j's uniform table is
-// U + j * nu).  Each chain's arithmetic is the single-chain arithmetic.
-template <int NS, int OPT, int CW>
-__device__ __forceinline__ void vm_step(uint32_t op, uint32_t slot, uint32_t a,
-                                        uint32_t b, const float* __restrict__ U,
-                                        uint32_t nu,
-                                        const float* __restrict__ tile,
-                                        uint32_t ld, int lane,
-                                        float (&T)[CW][OPT],
-                                        float (&S)[CW][NS > 0 ? NS : 1][OPT],
-                                        float (&D)[OPT]) {
-  const int col = vm_col(op, a, b);
-  if (col >= 0) vm_load_col<OPT>(tile + (uint32_t)col * ld, lane, D);
-#pragma unroll
-  for (int j = 0; j < CW; ++j)
-    vm_apply<NS, OPT>(op, slot, a, b, U + j * nu, D, T[j], S[j]);
 }
 
 // SUM_AFFINE: s[j] += U_j[a] * column b over the chunk, one fused operation
