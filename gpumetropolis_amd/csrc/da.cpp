@@ -485,7 +485,7 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
       temps, sd, L;
   DevBuf<int32_t> pos;
   DevBuf<uint32_t> list, count, queue, n_sl;
-  DevBuf<float> draws;
+  DevBuf<float> draws, draws_it;
   DevBuf<double> tr_s, tr_plp, tr_prop, tr_ll;
   DevBuf<uint8_t> tr_stage;
   DevBuf<DaArgs> d_args;
@@ -499,7 +499,8 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
             temps.upload(temperatures, C, "upload temperatures") &&
             pos.alloc(C, "allocate state") && list.alloc(C, "allocate state") &&
             count.alloc(1, "allocate state") &&
-            draws.alloc(n_draws, "allocate draws");
+            draws.alloc(n_draws, "allocate draws") &&
+            draws_it.alloc(n_draws, "allocate draws");
   if (ok && proposal_mode == 0) ok = sd.upload(proposal_sd, ncp, "upload proposal_sd");
   if (ok && proposal_mode == 2)
     ok = L.upload(proposal_l, ncp * np, "upload proposal_l");
@@ -559,6 +560,7 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
     a.s_raw = S->s.p;
   }
   a.draws = draws.p;
+  a.draws_it = draws_it.p;
   a.trace_s = X.trace_s ? tr_s.p : nullptr;
   a.trace_plp = X.trace_plp ? tr_plp.p : nullptr;
   a.trace_stage = X.trace_stage ? tr_stage.p : nullptr;
@@ -621,6 +623,11 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
       m, t.stream_mode ? JIT_POINTS_STREAM : JIT_POINTS_RESIDENT, t.v,
       t.lds_bytes + kPointsExtraLds, &jit_failed);
   if (jit_failed) return 1;
+  if (!jf && t.v.cw > 2) {
+    set_err("%d chains per wave need a compiled kernel, which this launch "
+            "(%zu B of LDS) cannot use", t.v.cw, (size_t)t.lds_bytes);
+    return 1;
+  }
   auto points = [&](const PointsArgs* p) -> hipError_t {
     if (!jf)
       return launch_points_kernel(p, n_wg, h, t.v.ns, t.v.opt, t.v.cw,
@@ -646,6 +653,7 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
     chk(points(d_pa.p));
     chk(launch_da_stage2(d_args.p, C, it, st));
   }
+  if (e == hipSuccess) e = launch_da_draws(draws_it.p, draws.p, n_iter, C * np, st);
   hipEventRecord(e1, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   float ms = 0.0f;

@@ -45,6 +45,7 @@ struct DaArgs {
   uint16_t* Xb;              // network input rows (bf16, 16 per row)
   const float* s_raw;        // network output per row
   // outputs
+  float* draws_it;       // iteration-major staging: (t * np + j) * C + c
   float* draws;          // reference layout t + n_iter (c + C j)
   double* trace_s;       // n_iter x C surrogate log-posterior of prop
   double* trace_plp;     // n_iter x C exact log-posterior (NaN: not run)
@@ -59,6 +60,9 @@ hipError_t launch_da_init_finish(const DaArgs* A, uint32_t n_chains,
 hipError_t launch_da_propose(const DaArgs* A, uint32_t n_chains, uint32_t t,
                              hipStream_t s);
 hipError_t launch_da_stage1(const DaArgs* A, uint32_t t, hipStream_t s);
+// draws (reference layout) from draws_it, once after the iterations.
+hipError_t launch_da_draws(const float* draws_it, float* draws, uint32_t n_iter,
+                           uint32_t rows, hipStream_t s);
 hipError_t launch_da_stage2(const DaArgs* A, uint32_t n_chains, uint32_t t,
                             hipStream_t s);
 // Network input rows of arbitrary points (surrogate predictions).

@@ -1,9 +1,10 @@
 // da_kernels.hip -- the stepwise sampler's per-iteration kernels.
 //
 // One iteration is a short sequence of launches on one stream:
-//   propose   one thread per chain: Philox normals and the proposal
-//             (cpu_native.rs:726-784, modes 0 and 2), its prior, and with a
-//             surrogate the network's input row;
+//   propose   one wave per chain, a lane per parameter: Philox normals and
+//             the proposal (cpu_native.rs:726-784, modes 0 and 2), then its
+//             prior (lead lane) and with a surrogate the network's input
+//             row (a lane per input);
 //   [network forward over all proposals]
 //   stage1    one workgroup: the surrogate's Metropolis test
 //             log u1 < (s(prop) - s(theta)) / T, and an ordered compaction of
@@ -13,7 +14,9 @@
 //   stage2    one thread per chain: the exact test, corrected by the
 //             surrogate ratio so the target is unchanged (Christen and Fox,
 //             "Markov chain Monte Carlo using an approximation", JCGS 2005):
-//             log u < ((lp' - lp) - (s' - s)) / T; record the draw.
+//             log u < ((lp' - lp) - (s' - s)) / T; record the draw in an
+//             iteration-major staging buffer (coalesced), turned into the
+//             reference's layout once after the run.
 // Without a surrogate every proposal passes stage 1 and stage 2 is the
 // reference's Metropolis test (cpu_native.rs:785-799) with the same
 // uniforms as the fused chain kernel.
@@ -79,37 +82,57 @@ __global__ void da_init_finish_kernel(const DaArgs* __restrict__ A) {
                     : 0.0;
 }
 
-__global__ void da_propose_kernel(const DaArgs* __restrict__ A, uint32_t t) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= A->n_chains) return;
+constexpr int kProposeWaves = 4;  // chains per 256-thread workgroup
+
+__global__ void __launch_bounds__(256) da_propose_kernel(const DaArgs* __restrict__ A,
+                                                         uint32_t t) {
+  __shared__ double prop_sh[kProposeWaves][kMaxParams];
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const uint32_t c = blockIdx.x * kProposeWaves + (uint32_t)w;
+  const bool live = c < A->n_chains;  // uniform across the wave
   const uint32_t np = A->n_params;
-  const double two_pi = 6.283185307179586;
-  StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
-  const double* th = A->theta + (size_t)c * np;
-  double* pr = A->prop + (size_t)c * np;
-  if (A->mode == 2) {
-    double z[kMaxParams];
-    for (uint32_t j = 0; j < np; ++j) {
-      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-      z[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+  const bool mine = live && (uint32_t)lane < np;  // lane j: parameter j
+  if (live) {
+    const double two_pi = 6.283185307179586;
+    double z = 0.0;
+    if (mine) {
+      StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
+      const double u1 = us.get(2 * lane), u2 = us.get(2 * lane + 1);
+      z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
     }
-    const double* L = A->proposal_l + (size_t)c * np * np;
-    for (uint32_t k = 0; k < np; ++k) {
-      double a = th[k];
-      for (uint32_t j = 0; j <= k; ++j) a += L[k * np + j] * z[j];
-      pr[k] = a;
+    const double* th = A->theta + (size_t)c * np;
+    double p = 0.0;
+    if (A->mode == 2) {
+      // pr[k] = th[k] + sum_{j <= k} L[k][j] z[j], j ascending as the
+      // reference sums it.
+      const double* L = A->proposal_l + (size_t)c * np * np;
+      double a = mine ? th[lane] : 0.0;
+      for (uint32_t j = 0; j < np; ++j) {
+        const double zj = __shfl(z, (int)j, 64);
+        if (mine && j <= (uint32_t)lane) a += L[(uint32_t)lane * np + j] * zj;
+      }
+      p = a;
+    } else if (mine) {
+      p = th[lane] + A->proposal_sd[(size_t)c * np + lane] * z;
     }
-  } else {
-    const double* sd = A->proposal_sd + (size_t)c * np;
-    for (uint32_t j = 0; j < np; ++j) {
-      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-      double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-      pr[j] = th[j] + sd[j] * z;
+    if (mine) {
+      A->prop[(size_t)c * np + lane] = p;
+      prop_sh[w][lane] = p;
     }
   }
-  A->prop_prior[c] = eval_params64(A->prior, A->n_prior, A->prior_consts, pr, nullptr);
-  if (A->use_surrogate)
-    write_features(pr, A->n_in, A->norm_mean, A->norm_w, A->Xb + (size_t)c * kMlpIn);
+  __syncthreads();
+  if (!live) return;
+  if (lane == 0)
+    A->prop_prior[c] = eval_params64(A->prior, A->n_prior, A->prior_consts,
+                                     prop_sh[w], nullptr);
+  if (A->use_surrogate && lane < kMlpIn) {
+    const uint32_t n_in = A->n_in;
+    double x = 0.0;
+    if ((uint32_t)lane < n_in)
+      for (uint32_t j = 0; j < n_in; ++j)
+        x += A->norm_w[(uint32_t)lane * n_in + j] * (prop_sh[w][j] - A->norm_mean[j]);
+    A->Xb[(size_t)c * kMlpIn + lane] = f2bf((float)x);
+  }
 }
 
 // One workgroup of 1024 threads walks the chains in order, so the slot of
@@ -196,8 +219,27 @@ __global__ void da_stage2_kernel(const DaArgs* __restrict__ A, uint32_t t) {
     for (uint32_t j = 0; j < np; ++j)
       A->trace_prop[((size_t)t * C + c) * np + j] = A->prop[(size_t)c * np + j];
   if (A->trace_stage) A->trace_stage[(size_t)t * C + c] = p < 0 ? 0 : (acc_ok ? 2 : 1);
-  const size_t T = A->n_iter;
-  for (uint32_t j = 0; j < np; ++j) A->draws[t + T * (c + (size_t)C * j)] = (float)th[j];
+  for (uint32_t j = 0; j < np; ++j)
+    A->draws_it[((size_t)t * np + j) * C + c] = (float)th[j];
+}
+
+// draws[t + n_iter * r] = draws_it[t * rows + r] with r = c + C j: a
+// transpose of the (n_iter x rows) staging matrix through LDS tiles.
+__global__ void __launch_bounds__(256) da_draws_kernel(const float* __restrict__ in,
+                                                       float* __restrict__ out,
+                                                       uint32_t n_iter, uint32_t rows) {
+  __shared__ float tile[32][33];
+  const uint32_t r0 = blockIdx.x * 32, t0 = blockIdx.y * 32;
+  const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (uint32_t k = ty; k < 32; k += 8) {
+    const uint32_t t = t0 + k, r = r0 + tx;
+    if (t < n_iter && r < rows) tile[k][tx] = in[(size_t)t * rows + r];
+  }
+  __syncthreads();
+  for (uint32_t k = ty; k < 32; k += 8) {
+    const uint32_t r = r0 + k, t = t0 + tx;
+    if (t < n_iter && r < rows) out[(size_t)r * n_iter + t] = tile[tx][k];
+  }
 }
 
 static dim3 chain_grid(uint32_t n) { return dim3((n + 255) / 256); }
@@ -212,11 +254,20 @@ hipError_t launch_da_init_finish(const DaArgs* A, uint32_t n, hipStream_t s) {
 }
 hipError_t launch_da_propose(const DaArgs* A, uint32_t n, uint32_t t,
                              hipStream_t s) {
-  hipLaunchKernelGGL(da_propose_kernel, chain_grid(n), dim3(256), 0, s, A, t);
+  hipLaunchKernelGGL(da_propose_kernel,
+                     dim3((n + kProposeWaves - 1) / kProposeWaves), dim3(256), 0,
+                     s, A, t);
   return hipGetLastError();
 }
 hipError_t launch_da_stage1(const DaArgs* A, uint32_t t, hipStream_t s) {
   hipLaunchKernelGGL(da_stage1_kernel, dim3(1), dim3(1024), 0, s, A, t);
+  return hipGetLastError();
+}
+hipError_t launch_da_draws(const float* draws_it, float* draws, uint32_t n_iter,
+                           uint32_t rows, hipStream_t s) {
+  if (n_iter == 0 || rows == 0) return hipSuccess;
+  hipLaunchKernelGGL(da_draws_kernel, dim3((rows + 31) / 32, (n_iter + 31) / 32),
+                     dim3(256), 0, s, draws_it, draws, n_iter, rows);
   return hipGetLastError();
 }
 hipError_t launch_da_stage2(const DaArgs* A, uint32_t n, uint32_t t,

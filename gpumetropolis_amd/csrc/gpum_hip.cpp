@@ -237,21 +237,27 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
       cw_req = ext->chains_per_wave;
     // Automatic: two chains per wave when the program keeps no stack, the
     // chains fill every CU with pairs and the launch runs a compiled kernel
-    // (the interpreter's dispatch holds both chains' registers at every
-    // opcode merge and spills).  A ladder or a pinned workgroup of odd size
-    // keeps one.
+    // (the interpreter's dispatch holds both chains' registers at each
+    // opcode merge and spills).  A ladder or a pinned workgroup must hold
+    // whole pairs.
     if (cw_req == 0) {
       const bool has_tiling =
           ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4;
-      const bool odd_ladder =
-          ext && ext->swap_every > 0 && ext->swap_group > 1 && ext->swap_group % 2u;
-      const bool odd_pin =
-          has_tiling && ext->chains_per_block > 0 && ext->chains_per_block % 2;
-      if (m->jit && pick_variant(m->L.vm_depth).ns == 0 &&
-          n_chains >= 2u * (uint32_t)g_dev.n_cu && !odd_ladder && !odd_pin &&
-          pick_tiling_cw(m, n_chains, ext, t, points, 2, nullptr) &&
-          t->lds_bytes <= kJitMaxDynamicLds)
-        return true;
+      // Four chains per wave (an explicit choice, affine programs) measured
+      // slower for 4096 gamma chains: with 4 waves per group each wave
+      // gets 2 chunks per tile and the per-tile work dominates.
+      for (uint32_t cw : {2u}) {
+        const bool split_ladder = ext && ext->swap_every > 0 &&
+                                  ext->swap_group > 1 && ext->swap_group % cw;
+        const bool split_pin = has_tiling && ext->chains_per_block > 0 &&
+                               (uint32_t)ext->chains_per_block % cw;
+        if (m->jit && pick_variant(m->L.vm_depth).ns == 0 &&
+            n_chains >= cw * (uint32_t)g_dev.n_cu && !split_ladder &&
+            !split_pin &&
+            pick_tiling_cw(m, n_chains, ext, t, points, (int)cw, nullptr) &&
+            t->lds_bytes <= kJitMaxDynamicLds)
+          return true;
+      }
       cw_req = 1;
     }
   }
@@ -277,8 +283,13 @@ static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
   // Chains per wave: two chains share every observation a wave reads from
   // LDS (half the LDS traffic per evaluation).
   if (!force) {
-    if (cw_in != 1 && !(cw_in == 2 && t->v.ns == 0)) {
-      set_err("chains_per_wave must be 1, or 2 for a program of depth 1");
+    if (cw_in != 1 && !((cw_in == 2 || cw_in == 4) && t->v.ns == 0)) {
+      set_err("chains_per_wave must be 1, or 2 or 4 for a program of depth 1");
+      return false;
+    }
+    if (cw_in == 4 && !m->jit) {
+      set_err("four chains per wave run on compiled kernels only "
+              "(gpum_hip_model_set_jit)");
       return false;
     }
     t->v.cw = cw_in;
@@ -730,14 +741,19 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   h.swap_group = a.swap_group;
   set_sum_fields(m, &h);
 
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
   bool jit_failed = false;
   hipFunction_t jf = model_jit_kernel(
       m, t.stream_mode ? JIT_CHAIN_STREAM : JIT_CHAIN_RESIDENT, t.v,
       t.lds_bytes, &jit_failed);
   if (jit_failed) return 1;
+  if (!jf && t.v.cw > 2) {
+    set_err("%d chains per wave need a compiled kernel, which this launch "
+            "(%zu B of LDS) cannot use", t.v.cw, (size_t)t.lds_bytes);
+    return 1;
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
   hipEventRecord(e0, g_dev.stream);
   hipError_t e;
   if (jf) {

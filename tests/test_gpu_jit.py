@@ -104,10 +104,10 @@ def _pair_cases():
 @pytest.mark.parametrize("jit", [False, True], ids=["vm", "jit"])
 @pytest.mark.parametrize("case", _pair_cases(), ids=lambda c: c[0])
 def test_two_chains_per_wave_equal_one(hip, case, jit):
-    """A wave evaluating two chains on each observation it reads does each
-    chain's arithmetic exactly as a wave of one chain: the same waves per
-    chain give identical data terms, decisions and draws (7 chains: the
-    last pair has an idle member)."""
+    """A wave evaluating two (or four) chains on each observation it reads
+    does each chain's arithmetic exactly as a wave of one chain: the same
+    waves per chain give identical data terms, decisions and draws (7
+    chains: the last group has idle members)."""
     name, wl, init, step = case
     print(f"[pairs] {name} jit={jit}", flush=True)
     m = wl.model
@@ -116,18 +116,22 @@ def test_two_chains_per_wave_equal_one(hip, case, jit):
                          wl.n_obs, jit=jit)
     C = init.shape[0]
     sd = np.full_like(init, step)
-    for wpc in (2, 4):
+    # (waves per chain group, chains per wave); four only compiled
+    pairs = [(2, 2), (4, 2)] + ([(4, 4)] if jit else [])
+    for wpc, cw in pairs:
         a = dm.run(init, sd, np.ones(C), 40, 3, trace=True, tiling=(16 // wpc, wpc, 1))
-        b = dm.run(init, sd, np.ones(C), 40, 3, trace=True, tiling=(32 // wpc, wpc, 2))
-        assert np.array_equal(a.trace_plp, b.trace_plp, equal_nan=True), wpc
-        assert np.array_equal(a.draws, b.draws), wpc
+        b = dm.run(init, sd, np.ones(C), 40, 3, trace=True,
+                   tiling=(16 * cw // wpc, wpc, cw))
+        assert np.array_equal(a.trace_plp, b.trace_plp, equal_nan=True), (wpc, cw)
+        assert np.array_equal(a.draws, b.draws), (wpc, cw)
     # the points kernel (one slice, so the sums are ordered alike)
     c = dm.run_da(None, init, np.ones(C), 20, 7, proposal_sd=sd, trace=True,
                   work_items=1, chains_per_wave=1)
-    d = dm.run_da(None, init, np.ones(C), 20, 7, proposal_sd=sd, trace=True,
-                  work_items=1, chains_per_wave=2)
-    assert np.array_equal(c.trace_plp, d.trace_plp, equal_nan=True)
-    assert np.array_equal(c.draws, d.draws)
+    for cw in (2, 4) if jit else (2,):
+        d = dm.run_da(None, init, np.ones(C), 20, 7, proposal_sd=sd, trace=True,
+                      work_items=1, chains_per_wave=cw)
+        assert np.array_equal(c.trace_plp, d.trace_plp, equal_nan=True), cw
+        assert np.array_equal(c.draws, d.draws), cw
     dm.close()
 
 
