@@ -73,6 +73,7 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     return 1;
   }
   int max_depth = 0;
+  bool all_affine = true;
   for (uint32_t k = 0; k < n_models; ++k) {
     const gpum_batch_model& b = models[k];
     if (!b.model || !b.init || !b.proposal_sd || b.n_chains == 0) {
@@ -85,9 +86,10 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
       return 1;
     }
     max_depth = std::max(max_depth, b.model->L.vm_depth);
+    all_affine = all_affine && model_affine(b.model);
   }
   TilingForce force;
-  force.v = pick_variant(max_depth);
+  force.v = pick_variant(max_depth, all_affine);
   if (cw == 2 && force.v.ns != 0) {
     set_err("chains_per_wave 2 needs every model's program at depth 1");
     return 1;

@@ -124,6 +124,20 @@ __device__ __forceinline__ void vm_bin_s(uint32_t slot, const float (&S)[NS][OPT
   }
 }
 
+template <int I, int NS, int OPT>
+__device__ __forceinline__ void vm_unspill(uint32_t slot,
+                                           const float (&S)[NS][OPT],
+                                           float (&T)[OPT]) {
+  if constexpr (I < NS) {
+    if (slot == I) {
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) T[k] = S[I][k];
+    } else {
+      vm_unspill<I + 1, NS, OPT>(slot, S, T);
+    }
+  }
+}
+
 // Evaluate the program on one chunk; `tile` points at the chunk's first
 // observation of column 0 and `ld` is the column stride of the tile (floats).
 // The program lives in the constant address space: its address and the pc
@@ -264,6 +278,27 @@ case NAME:                                              \
     GPUM_UNOP(VM_RCP, vm_rcp(t))
     GPUM_UNOP(VM_SOFTPLUS, vm_softplus(t))
 #undef GPUM_UNOP
+    case VM_SAVE:
+      if constexpr (NS > 0) {
+#pragma unroll
+        for (int j = 0; j < CW; ++j) vm_spill<0, NS, OPT>(slot, S[j], T[j]);
+      }
+      break;
+    case VM_LOAD_S:
+      if constexpr (NS > 0) {
+#pragma unroll
+        for (int j = 0; j < CW; ++j) vm_unspill<0, NS, OPT>(a, S[j], T[j]);
+      }
+      break;
+    case VM_PUSH_S:
+      if constexpr (NS > 0) {
+#pragma unroll
+        for (int j = 0; j < CW; ++j) {
+          vm_spill<0, NS, OPT>(slot, S[j], T[j]);
+          vm_unspill<0, NS, OPT>(a, S[j], T[j]);
+        }
+      }
+      break;
     case VM_FMA_UU:
 #pragma unroll
       for (int j = 0; j < CW; ++j) {

@@ -104,9 +104,14 @@ bool device_init() {
   return true;
 }
 
-Variant pick_variant(int vm_depth) {
+Variant pick_variant(int vm_depth, bool affine) {
   int need = vm_depth > 1 ? vm_depth - 1 : 0;
-  if (need == 0) return {0, 16, 1};
+  // A depth-1 program that keeps a value per observation (not affine)
+  // works on 8 observations per lane, so that two chains per wave hold
+  // their values, their data and their uniforms without spilling; the
+  // choice depends on the program only, so one and two chains per wave
+  // compute alike.
+  if (need == 0) return {0, affine ? 16 : 8, 1};
   if (need <= 1) return {1, 16, 1};
   if (need <= 3) return {3, 16, 1};
   if (need <= 7) return {7, 8, 1};
@@ -251,7 +256,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
                                   ext->swap_group > 1 && ext->swap_group % cw;
         const bool split_pin = has_tiling && ext->chains_per_block > 0 &&
                                (uint32_t)ext->chains_per_block % cw;
-        if (m->jit && pick_variant(m->L.vm_depth).ns == 0 &&
+        if (m->jit && m->L.vm_depth <= 1 &&
             n_chains >= cw * (uint32_t)g_dev.n_cu && !split_ladder &&
             !split_pin &&
             pick_tiling_cw(m, n_chains, ext, t, points, (int)cw, nullptr) &&
@@ -269,7 +274,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
 static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
                            const gpum_run_ext* ext, Tiling* t, bool points,
                            int cw_in, const TilingForce* force) {
-  t->v = force ? force->v : pick_variant(m->L.vm_depth);
+  t->v = force ? force->v : pick_variant(m->L.vm_depth, model_affine(m));
   const size_t static_lds =
       kStreamStaticLds + (force ? force->extra_static
                                 : (points ? kPointsExtraLds : 0));
@@ -536,7 +541,7 @@ int gpum_hip_model_set_jit(gpum_hip_model* m, int32_t enable,
   }
   // Compile the streamed chain kernel of the variant of large launches now
   // (a failure is reported here); other kinds and variants on first use.
-  Variant v = pick_variant(m->L.vm_depth);
+  Variant v = pick_variant(m->L.vm_depth, model_affine(m));
   if (v.ns == 0) v.cw = 2;
   std::string src = jit_evaluator_source(m->L, v.ns, v.opt, v.cw);
   std::string err;

@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <map>
+#include <set>
 #include <sstream>
 
 #include "math64.h"
@@ -32,6 +33,7 @@ VmInstr make_instr(uint32_t op, uint32_t slot = 0, uint32_t a = 0,
 double vm_weight(uint32_t op) {
   switch (op) {
     case VM_LOAD_U: case VM_LOAD_D: case VM_PUSH_U: case VM_PUSH_D:
+    case VM_SAVE: case VM_LOAD_S: case VM_PUSH_S:
       return 0.0;
     case VM_DIV_S: case VM_DIV_D: case VM_RDIV_U: case VM_RDIV_D:
     case VM_RCP:
@@ -296,6 +298,12 @@ struct Emitter {
   std::map<std::string, int> dcol;
   int maxdepth = 0;
   std::string err;
+  // Common subexpressions: repeated data-dependent subtrees evaluated once
+  // and kept in a spill slot (numbered from 0 here, placed above the
+  // program's stack by finish_cse).
+  std::set<std::string> cse_keys;
+  std::map<std::string, int> cse_slot;
+  int n_cse = 0;
 
   Emitter(const Builder& bb, const LowerOptions& o, Lowered* L)
       : b(bb), opt(o), out(L) {}
@@ -410,8 +418,60 @@ struct Emitter {
     if (depth > maxdepth) maxdepth = depth;
   }
 
+  // Repeated subtrees worth keeping: data-dependent, not an operand, at
+  // least two VALU slots, marked at their outermost repetition (a repeat
+  // inside one is evaluated once with it).
+  void plan_cse(int root) {
+    std::map<std::string, int> count;
+    for (const Node& n : b.nodes) ++count[n.key];
+    mark_cse(root, count);
+  }
+  void mark_cse(int i, const std::map<std::string, int>& count) {
+    if (i < 0 || is_operand(i)) return;
+    const Node& n = N(i);
+    if (is_u(i)) return;
+    if (count.at(n.key) >= 2 && subtree_weight(i) >= 2.0) {
+      cse_keys.insert(n.key);
+      return;
+    }
+    mark_cse(n.k0, count);
+    mark_cse(n.k1, count);
+  }
+  // Place the CSE slots above the deepest spill slot the program uses.
+  void finish_cse() {
+    if (n_cse == 0) return;
+    const uint32_t base = (uint32_t)std::max(0, maxdepth - 1);
+    for (VmInstr& ins : out->vm) {
+      const uint32_t op = vm_op(ins);
+      if (op == VM_SAVE)
+        ins = make_instr(op, base + vm_slot(ins), vm_a(ins), vm_b(ins));
+      else if (op == VM_LOAD_S || op == VM_PUSH_S)
+        ins = make_instr(op, vm_slot(ins), base + vm_a(ins), vm_b(ins));
+    }
+    maxdepth = (int)base + 1 + n_cse;
+  }
+
   // Leave the value of node i in T; `d` values are on the stack below.
   void value(int i, int d) {
+    const std::string& key = N(i).key;
+    auto it = cse_slot.find(key);
+    if (it != cse_slot.end()) {
+      if (d == 0)
+        put(VM_LOAD_S, 0, (uint32_t)it->second);
+      else
+        put(VM_PUSH_S, (uint32_t)(d - 1), (uint32_t)it->second);
+      bump(d + 1);
+      return;
+    }
+    value_raw(i, d);
+    if (cse_keys.count(key)) {
+      const int k = n_cse++;
+      put(VM_SAVE, (uint32_t)k);
+      cse_slot[key] = k;
+    }
+  }
+
+  void value_raw(int i, int d) {
     if (is_operand(i)) {
       Operand o = operand(i);
       if (d == 0)
@@ -690,9 +750,12 @@ std::string lower_program(const uint32_t* code, size_t n_words,
   if (!e.empty()) return e;
   out->w_bytecode = bytecode_weight(b);
   Emitter em(b, opt, out);
+  if (opt.fuse) em.plan_cse(b.root);
   em.value(b.root, 0);
   if (!em.err.empty()) return em.err;
+  em.finish_cse();
   out->vm_depth = em.maxdepth;
+  if (out->vm_depth > kMaxStack) return "lowered program too deep";
   if (opt.fuse) fold_sum(out);
   out->w_exec = out->sum_mode == SUM_SQUARE ? 1.0 : 0.0;
   for (const VmInstr& ins : out->vm) out->w_exec += vm_weight(vm_op(ins));
@@ -708,7 +771,7 @@ std::string disassemble(const Lowered& L) {
       "RDIV_U", "RPOW_U", "ADD_D", "SUB_D", "MUL_D", "DIV_D", "POW_D",
       "RSUB_D", "RDIV_D", "RPOW_D", "NEG", "EXP", "LOG", "SQRT", "LGAMMA",
       "SQR", "CUBE", "POW4", "RCP", "FMA_UU", "FMA_UD", "FMA_DU",
-      "SOFTPLUS"};
+      "SOFTPLUS", "SAVE", "LOAD_S", "PUSH_S"};
   std::ostringstream s;
   for (const VmInstr& i : L.vm) {
     uint32_t op = vm_op(i);

@@ -81,7 +81,7 @@ struct Variant {
   int ns, opt, cw;
 };
 
-Variant pick_variant(int vm_depth);
+Variant pick_variant(int vm_depth, bool affine);
 uint32_t round_up(uint32_t x, uint32_t m);
 
 struct Tiling {
@@ -137,6 +137,10 @@ bool ensure_data(gpum_hip_model* m, uint32_t n_pad);
 
 // The summation fields of a launch's hot values (the tail is uploaded
 // after the program's padding instruction).
+inline bool model_affine(const gpum_hip_model* m) {
+  return !m->L.empty && m->L.sum_mode == SUM_AFFINE;
+}
+
 inline void set_sum_fields(const gpum_hip_model* m, ChainHot* h) {
   h->sum_mode = m->L.empty ? (uint32_t)SUM_VALUE : m->L.sum_mode;
   h->n_sum_tail = m->L.empty ? 0u : (uint32_t)m->L.sum_tail.size();

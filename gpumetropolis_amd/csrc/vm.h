@@ -82,6 +82,10 @@ enum VmOp : uint32_t {
   VM_FMA_UD,  // T <- T + U[a] * D[b]
   VM_FMA_DU,  // T <- T * D[a] + U[b]
   VM_SOFTPLUS,  // T <- log(1 + exp(T)), overflow-safe
+  // common subexpressions kept in spill slots above the program's stack
+  VM_SAVE,    // S[slot] <- T
+  VM_LOAD_S,  // T <- S[a] (stack was empty)
+  VM_PUSH_S,  // S[slot] <- T ; T <- S[a]
   VM_NUM_OPS
 };
 
