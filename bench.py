@@ -72,6 +72,9 @@ def parse():
                    help="iterations per chunk of the R-hat-gated runs")
     p.add_argument("--da-max-iters", type=int, default=16000)
     p.add_argument("--da-train-steps", type=int, default=4000)
+    p.add_argument("--no-pt", action="store_true",
+                   help="skip the parallel-tempering workload")
+    p.add_argument("--pt-iters", type=int, default=4000)
     p.add_argument("--no-batch", action="store_true",
                    help="skip the many-model (512 regressions) section")
     p.add_argument("--batch-models", type=int, default=512)
@@ -263,6 +266,47 @@ def da_section(a):
     return out
 
 
+def pt_section(a):
+    """Parallel tempering on the bimodal mixture: 200,000 FP32 rows, 16
+    geometric temperatures (beta 1 .. 0.01) x 256 replicas, swaps every 10
+    iterations; batches, per-chain adaptation and swap sweeps run inside one
+    launch."""
+    from gpumetropolis_amd import workloads
+    from gpumetropolis_amd.api import gpu_metropolis
+    from gpumetropolis_amd.diagnostics import summarize
+
+    wl = workloads.mixture(n_iter=a.pt_iters)
+    G, R = wl.meta["n_temps"], wl.meta["n_replicas"]
+    C = G * R
+    rng = np.random.default_rng(a.seed)
+    init = rng.normal(0.0, 4.0, (C, 2))
+    t0 = time.perf_counter()
+    fit = gpu_metropolis(wl.model, wl.data, init=init, proposal_sd=0.3,
+                         n_iter=a.pt_iters, method="pt",
+                         temperatures=wl.meta["temperatures"],
+                         swap_every=wl.meta["swap_every"], ladder_size=G,
+                         seed=a.seed, jit=a.jit)
+    wall = time.perf_counter() - t0
+    evals = C * wl.n_obs * a.pt_iters
+    cold = fit.draws[:, ::G, :]                 # beta = 1 member of each ladder
+    s = summarize(cold)
+    # the posterior's two modes are the label swaps (mu1, mu2) = (-3, 3), (3, -3)
+    upper = float(np.mean(cold[:, :, 0] > cold[:, :, 1]))
+    sh = fit.adaptation["swap_history"]
+    return {"workload": f"bimodal mixture, {wl.n_obs} FP32 rows, {G} temperatures "
+                        f"x {R} replicas, {a.pt_iters} iterations, swaps every "
+                        f"{wl.meta['swap_every']}",
+            "obs_evals_per_s": evals / (fit.kernel_ms / 1e3),
+            "kernel_ms": fit.kernel_ms, "wall_s": wall,
+            "swap_accept": float(np.nanmean(sh)),
+            "cold_draws_in_upper_mode": upper,
+            "cold_ess_min": s["min_ess"],
+            "cold_ess_per_s": s["min_ess"] / (fit.kernel_ms / 1e3),
+            "cold_rhat_max": s["max_rhat"],
+            "engine": "compiled per model" if fit.adaptation.get("compiled")
+                      else "interpreter"}
+
+
 def batch_section(a):
     """512 independent linear regressions of log-uniform size in [1e3, 1e6]
     rows (x1..x8, FP32), DE-MC with 64 chains each, every batch of every
@@ -300,7 +344,7 @@ def batch_section(a):
             "graph_ms": bf.kernel_ms, "wall_s_incl_setup": wall,
             "data_gen_s": gen_s,
             "tiling": list(bf.tiling), "tile_key": bf.tile_key,
-            "tile_timings_ms": {f"{k[0]}x{k[1]}": v
+            "tile_timings_ms": {"x".join(map(str, k)): v
                                 for k, v in bf.tile_timings.items()},
             "workgroups_per_launch": bf.n_workgroups,
             "launches": 2 * info["n_batches"],
@@ -455,6 +499,8 @@ def main():
     dm.close()
     if not a.no_da:
         out["surrogate_da"] = da_section(a)
+    if not a.no_pt:
+        out["parallel_tempering"] = pt_section(a)
     if not a.no_batch:
         out["batched_models"] = batch_section(a)
     if not a.no_cpu_baseline:

@@ -254,13 +254,11 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
   // compiled program with the launch's variant.
   hipFunction_t jf = nullptr;
   {
-    const std::string src0 = jit_evaluator_source(
-        models[0].model->L, force.v.ns, force.v.opt, force.v.cw);
+    const std::string src0 = model_jit_source(models[0].model, force.v);
     bool same = true;
     for (uint32_t k = 0; k < n_models && same; ++k)
       same = models[k].model->jit &&
-             (k == 0 || jit_evaluator_source(models[k].model->L, force.v.ns,
-                                             force.v.opt, force.v.cw) == src0);
+             (k == 0 || model_jit_source(models[k].model, force.v) == src0);
     if (same) {
       bool failed = false;
       jf = model_jit_kernel(models[0].model, JIT_BATCH, force.v,

@@ -11,7 +11,11 @@
 //                   SUM_AFFINE: the chunk's sums s[kCw], accumulated
 //                   directly;
 //   E::finish(H, U, s, with_const)
-//                   apply the sum tail to one chain's data-pass sum.
+//                   apply the sum tail to one chain's data-pass sum;
+//   E::Params::hoist(code, n, consts, p, U), E::Params::prior(code, n,
+//                   consts, p)
+//                   the FP64 parameter programs (hoisted values into the
+//                   uniform table; the prior).
 // The interpreter (VmEval) and code generated per model share this file.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -193,9 +197,21 @@ __device__ __noinline__ void pt_sweep(const ChainArgs* __restrict__ A,
   }
 }
 
+// The proposal's parameter programs: the hoisted program into the uniform
+// table and the prior (the evaluator's own FP64 code).
+template <class E>
+__device__ __forceinline__ void step_params(const ChainArgs* __restrict__ A,
+                                            double* st, float* U) {
+  Rec r = make_rec(st, (int)A->n_params);
+  E::hoist(A->hoist, A->n_hoist, A->consts, r.prop, U);
+  r.prior_lp() = E::prior(A->prior, A->n_prior, A->prior_consts, r.prop);
+}
+
 // Prepare iteration it: Philox uniforms, Box-Muller, the proposal
-// (cpu_native.rs:726-784), the hoisted parameter program into the uniform
-// table and the prior of the proposal.
+// (cpu_native.rs:726-784), then its parameter programs.  One call per
+// iteration from the kernel body (each call saves the body's live
+// registers).
+template <class E>
 __device__ __noinline__ void chain_prepare_iter(const ChainArgs* __restrict__ A,
                                                 uint32_t it, uint32_t chain,
                                                 double* st, float* U) {
@@ -239,10 +255,10 @@ __device__ __noinline__ void chain_prepare_iter(const ChainArgs* __restrict__ A,
       r.prop[j] = r.state[j] + g * diff + A->de_noise * sd[j] * z;
     }
   }
-  eval_params64(A->hoist, A->n_hoist, A->consts, r.prop, U);
-  r.prior_lp() = eval_params64(A->prior, A->n_prior, A->prior_consts, r.prop, nullptr);
+  step_params<E>(A, st, U);
 }
 
+template <class E>
 __device__ __noinline__ void chain_init(const ChainArgs* __restrict__ A,
                                         uint32_t chain, double* st, float* U) {
   const int np = (int)A->n_params;
@@ -259,8 +275,7 @@ __device__ __noinline__ void chain_init(const ChainArgs* __restrict__ A,
   r.wn() = 0.0;
   r.scale() = A->init_scale ? A->init_scale[chain] : 1.0;
   r.batch_acc() = 0.0;
-  eval_params64(A->hoist, A->n_hoist, A->consts, r.prop, U);
-  r.prior_lp() = eval_params64(A->prior, A->n_prior, A->prior_consts, r.prop, nullptr);
+  step_params<E>(A, st, U);
 }
 
 __device__ __noinline__ void chain_finish(const ChainArgs* __restrict__ A,
@@ -422,7 +437,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
 
   for (int i = tid; i < cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
   __syncthreads();
-  if (lead) chain_init(Ap, chain, st, U);
+  if (lead) chain_init<typename E::Params>(Ap, chain, st, U);
 
   const uint32_t tile_len = H.tile_len;
   const uint32_t n4 = tile_len / 4;      // float4s per column of a tile
@@ -538,7 +553,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
         __syncthreads();
       }
     }
-    if (lead && it < H.n_iter) chain_prepare_iter(Ap, it, chain, st, U);
+    if (lead && it < H.n_iter)
+      chain_prepare_iter<typename E::Params>(Ap, it, chain, st, U);
     __syncthreads();
   }
   if (lead) chain_finish(Ap, chain, st);
@@ -576,11 +592,12 @@ __device__ __forceinline__ void batch_body(const BatchWork* __restrict__ work,
 // that about `target_items` items exist whatever the count, and published
 // in *n_slices_out.  The last workgroup to finish resets the queue.
 // The uniform table of slot k's point (lead lane of the slot's wave).
+template <class E>
 __device__ __noinline__ void points_hoist(const PointsArgs* __restrict__ P,
                                           uint32_t k, float* U) {
   const uint32_t pi = P->index ? P->index[k] : k;
-  eval_params64(P->hoist, P->n_hoist, P->consts,
-                P->points + (size_t)pi * P->n_params, U);
+  E::hoist(P->hoist, P->n_hoist, P->consts,
+           P->points + (size_t)pi * P->n_params, U);
 }
 
 // Item count and slicing of one launch (identical in every workgroup).
@@ -637,7 +654,8 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
     const uint32_t kl = k0 + (uint32_t)(lane < CW ? lane : 0);  // lane's slot
     for (int i = tid; i < (int)H.cpb * NU; i += blockDim.x) Uall[i] = H.u_init[i % NU];
     __syncthreads();
-    if (lane < CW && kl < plan_sh.count) points_hoist(P, kl, Uw + lane * NU);
+    if (lane < CW && kl < plan_sh.count)
+      points_hoist<typename E::Params>(P, kl, Uw + lane * NU);
     const uint32_t ti0 = min(sl * tps, n_tiles);
     const uint32_t ti1 = min(ti0 + tps, n_tiles);
     const uint32_t n4 = tile_len / 4;
@@ -721,11 +739,27 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
   }
 }
 
+// The interpreter's parameter programs (one instance for every variant).
+struct VmParams {
+  static __device__ __forceinline__ void hoist(const uint32_t* code, uint32_t n,
+                                               const double* consts,
+                                               const double* p, float* U) {
+    eval_params64(code, n, consts, p, U);
+  }
+  static __device__ __forceinline__ double prior(const uint32_t* code,
+                                                 uint32_t n,
+                                                 const double* consts,
+                                                 const double* p) {
+    return eval_params64(code, n, consts, p, nullptr);
+  }
+};
+
 // The interpreter as an evaluator.
 template <int NS, int OPT, int CW>
 struct VmEval {
   static constexpr int kOpt = OPT;
   static constexpr int kCw = CW;
+  using Params = VmParams;
   static __device__ __forceinline__ void run(const ChainHot& H, const float* U,
                                              const float* tile, uint32_t ld,
                                              int lane, float (&T)[CW][OPT]) {

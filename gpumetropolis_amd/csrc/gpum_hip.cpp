@@ -212,6 +212,7 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
   std::vector<float> u0(m->n_uniform_pad, 0.0f);
   for (auto& kv : m->L.const_uniforms) u0[kv.first] = kv.second;
   m->prior_code.assign(prior_code, prior_code + prior_len);
+  m->prior_consts_h.assign(prior_consts, prior_consts + n_prior_consts);
   if (!m->vm.upload(vm4.data(), vm4.size(), "upload program")) return false;
   if (!m->hoist.upload(m->L.hoist.data(), m->L.hoist.size(), "upload program"))
     return false;
@@ -408,13 +409,18 @@ static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
   return true;
 }
 
+std::string model_jit_source(const gpum_hip_model* m, const Variant& v) {
+  return jit_evaluator_source(m->L, m->prior_code.data(), m->prior_code.size(),
+                              m->prior_consts_h.data(), v.ns, v.opt, v.cw);
+}
+
 hipFunction_t model_jit_kernel(const gpum_hip_model* m, int kind,
                                const Variant& v, size_t lds_bytes,
                                bool* failed) {
   *failed = false;
   if (!m->jit || lds_bytes > kJitMaxDynamicLds) return nullptr;
   std::string err;
-  hipFunction_t f = jit_kernel(jit_evaluator_source(m->L, v.ns, v.opt, v.cw),
+  hipFunction_t f = jit_kernel(model_jit_source(m, v),
                                (JitKind)kind, &err);
   if (!f) {
     set_err("%s", err.c_str());
@@ -543,7 +549,7 @@ int gpum_hip_model_set_jit(gpum_hip_model* m, int32_t enable,
   // (a failure is reported here); other kinds and variants on first use.
   Variant v = pick_variant(m->L.vm_depth, model_affine(m));
   if (v.ns == 0) v.cw = 2;
-  std::string src = jit_evaluator_source(m->L, v.ns, v.opt, v.cw);
+  std::string src = model_jit_source(m, v);
   std::string err;
   double ms = 0.0;
   if (!jit_kernel(src, JIT_CHAIN_STREAM, &err, &ms)) {

@@ -5,6 +5,8 @@
 #include <hip/hip_version.h>
 #include <hip/hiprtc.h>
 
+#include <string.h>
+
 #include <chrono>
 #include <map>
 #include <mutex>
@@ -139,9 +141,71 @@ bool runtime_matches(std::string* why) {
   return true;
 }
 
+// An FP64 constant written exactly (any value, -0.0 and NaN included).
+std::string f64_literal(double v) {
+  uint64_t bits;
+  memcpy(&bits, &v, sizeof bits);
+  std::ostringstream o;
+  o << "__builtin_bit_cast(double, 0x" << std::hex << bits << "ull)";
+  return o.str();
+}
+
+// A reference program over the parameters (hoist or prior) as straight-line
+// FP64 statements: the operations of eval_params64 (step_common.h), one per
+// statement, in its order, so the results are its results.  Returns the
+// expression of the value left at the bottom of the stack ("0.0" if none).
+std::string fp64_program(const uint32_t* code, size_t n_words,
+                         const double* consts, std::ostringstream& o) {
+  std::vector<std::string> st;
+  int n = 0;
+  auto fresh = [&]() { return "v" + std::to_string(n++); };
+  for (size_t pc = 0; pc + 1 < n_words; pc += 2) {
+    const uint32_t op = code[pc], arg = code[pc + 1];
+    if (op == OP_PUSH_CONST) {
+      st.push_back(f64_literal(consts[arg]));
+    } else if (op == OP_PUSH_PARAM) {
+      st.push_back("p[" + std::to_string(arg) + "]");
+    } else if (op == OP_STORE_U || op == OP_STORE_U_RCP || op == OP_STORE_U_NEG) {
+      const std::string a = st.back();
+      st.pop_back();
+      o << "    U[" << arg << "] = (float)("
+        << (op == OP_STORE_U ? a : op == OP_STORE_U_RCP ? "1.0 / " + a : "-" + a)
+        << ");\n";
+    } else if (op == OP_NEG || op == OP_EXP || op == OP_LOG || op == OP_SQRT ||
+               op == OP_LGAMMA) {
+      const std::string a = st.back();
+      st.pop_back();
+      const std::string v = fresh();
+      const char* f = op == OP_NEG ? "-" : op == OP_EXP ? "exp" : op == OP_LOG ? "log"
+                      : op == OP_SQRT ? "sqrt" : "lgamma64";
+      o << "    const double " << v << " = " << f << "(" << a << ");\n";
+      st.push_back(v);
+    } else {
+      const std::string b = st.back();
+      st.pop_back();
+      const std::string a = st.back();
+      st.pop_back();
+      const std::string v = fresh();
+      o << "    const double " << v << " = ";
+      switch (op) {
+        case OP_ADD: o << a << " + " << b; break;
+        case OP_SUB: o << a << " - " << b; break;
+        case OP_MUL: o << a << " * " << b; break;
+        case OP_DIV: o << a << " / " << b; break;
+        default: o << "pow(" << a << ", opaque64(" << b << "))"; break;
+      }
+      o << ";\n";
+      st.push_back(v);
+    }
+  }
+  return st.empty() ? "0.0" : st.front();
+}
+
 }  // namespace
 
-std::string jit_evaluator_source(const Lowered& L, int ns, int opt, int cw) {
+std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
+                                 size_t n_prior, const double* prior_consts,
+                                 int ns, int opt, int cw) {
   std::ostringstream o;
   const int nsd = ns > 0 ? ns : 1;
   o << "#include \"chain_body.h\"\n"
@@ -184,7 +248,21 @@ std::string jit_evaluator_source(const Lowered& L, int ns, int opt, int cw) {
         << "], (double)U[" << vm_b(t) << "], alpha, beta);\n";
     o << "    return vm_tail_apply(alpha, beta, s, H.n_obs, with_const);\n";
   }
-  o << "  }\n};\n}  // namespace gpum\n";
+  // The callers' program lengths still decide: a launch that passes an
+  // empty program (the data term alone, without the prior) gets what the
+  // interpreter returns for one.
+  o << "  }\n"
+    << "  using Params = JitEval;\n"
+    << "  static __device__ __forceinline__ void hoist(const uint32_t*, uint32_t n,\n"
+    << "      const double*, const double* __restrict__ p, float* U) {\n"
+    << "    if (n < 2) return;\n";
+  fp64_program(L.hoist.data(), L.hoist.size(), L.consts.data(), o);
+  o << "  }\n"
+    << "  static __device__ __forceinline__ double prior(const uint32_t*, uint32_t n,\n"
+    << "      const double*, const double* __restrict__ p) {\n"
+    << "    if (n < 2) return 0.0;\n";
+  const std::string r = fp64_program(prior, n_prior, prior_consts, o);
+  o << "    return " << r << ";\n  }\n};\n}  // namespace gpum\n";
   return o.str();
 }
 

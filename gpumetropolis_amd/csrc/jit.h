@@ -21,8 +21,11 @@ enum JitKind { JIT_CHAIN_STREAM = 0, JIT_CHAIN_RESIDENT, JIT_POINTS_STREAM,
 
 // Source of the evaluator for a lowered program run with variant
 // (ns, opt, cw): ns spill slots, opt observations per lane per chunk, cw
-// chains per wave.
-std::string jit_evaluator_source(const Lowered& L, int ns, int opt, int cw);
+// chains per wave; with the model's FP64 parameter programs (the lowered
+// hoist program and the prior, reference bytecode over the parameters).
+std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
+                                 size_t n_prior, const double* prior_consts,
+                                 int ns, int opt, int cw);
 
 // The kernel of `kind` for an evaluator source (compiled on first use).
 // On failure returns nullptr and fills *err.

@@ -107,6 +107,7 @@ struct gpum_hip_model {
   gpum::rt::DevBuf<float> u_init;
   uint32_t n_uniform_pad = 1;
   std::vector<uint32_t> prior_code;
+  std::vector<double> prior_consts_h;
   // device data, laid out for `data_ld`
   gpum::rt::DevBuf<float> data;
   uint32_t data_ld = 0;
@@ -153,6 +154,8 @@ inline void set_sum_fields(const gpum_hip_model* m, ChainHot* h) {
 hipFunction_t model_jit_kernel(const gpum_hip_model* m, int kind,
                                const Variant& v, size_t lds_bytes,
                                bool* failed);
+// Source of the model's compiled evaluator for variant v.
+std::string model_jit_source(const gpum_hip_model* m, const Variant& v);
 // Waves of a workgroup of tiling t.
 inline uint32_t tiling_waves(const Tiling& t) {
   return t.cpb / (uint32_t)t.v.cw * t.wpc;

@@ -10,8 +10,18 @@
 
 namespace gpum {
 
+// Hides a compile-time constant from the optimiser: code generated per
+// model passes its constant exponents through it, so pow(x, 2.0) there
+// calls the same library routine the interpreter calls, not a rewrite.
+__device__ __forceinline__ double opaque64(double x) {
+  __asm__ volatile("" : "+v"(x));
+  return x;
+}
+
 // FP64 stack evaluation of a reference program over the parameters (no
-// data), with the hoist extensions writing the uniform table.
+// data), with the hoist extensions writing the uniform table.  Code
+// generated per model (jit.cpp) performs the same operations in the same
+// order.
 static __device__ __noinline__ double eval_params64(const uint32_t* __restrict__ code,
                                 uint32_t n_words,
                                 const double* __restrict__ consts,
