@@ -211,14 +211,17 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
   m->n_uniform_pad = (uint32_t)std::max(1, m->L.n_uniform);
   std::vector<float> u0(m->n_uniform_pad, 0.0f);
   for (auto& kv : m->L.const_uniforms) u0[kv.first] = kv.second;
-  m->prior_code.assign(prior_code, prior_code + prior_len);
+  // The device's copy of the prior squares with OP_SQUARE (square_pow2).
+  m->prior_code = square_pow2(prior_code, prior_len, prior_consts);
   m->prior_consts_h.assign(prior_consts, prior_consts + n_prior_consts);
   if (!m->vm.upload(vm4.data(), vm4.size(), "upload program")) return false;
   if (!m->hoist.upload(m->L.hoist.data(), m->L.hoist.size(), "upload program"))
     return false;
   if (!m->consts.upload(m->L.consts.data(), m->L.consts.size(), "upload consts"))
     return false;
-  if (!m->prior.upload(prior_code, prior_len, "upload prior")) return false;
+  if (!m->prior.upload(m->prior_code.data(), m->prior_code.size(),
+                       "upload prior"))
+    return false;
   if (!m->prior_consts.upload(prior_consts, n_prior_consts, "upload prior"))
     return false;
   if (!m->u_init.upload(u0.data(), u0.size(), "upload uniforms")) return false;

@@ -171,6 +171,12 @@ std::string fp64_program(const uint32_t* code, size_t n_words,
       o << "    U[" << arg << "] = (float)("
         << (op == OP_STORE_U ? a : op == OP_STORE_U_RCP ? "1.0 / " + a : "-" + a)
         << ");\n";
+    } else if (op == OP_SQUARE) {
+      const std::string a = st.back();
+      st.pop_back();
+      const std::string v = fresh();
+      o << "    const double " << v << " = " << a << " * " << a << ";\n";
+      st.push_back(v);
     } else if (op == OP_NEG || op == OP_EXP || op == OP_LOG || op == OP_SQRT ||
                op == OP_LGAMMA) {
       const std::string a = st.back();

@@ -754,6 +754,8 @@ std::string lower_program(const uint32_t* code, size_t n_words,
   em.value(b.root, 0);
   if (!em.err.empty()) return em.err;
   em.finish_cse();
+  out->hoist = square_pow2(out->hoist.data(), out->hoist.size(),
+                           out->consts.data());
   out->vm_depth = em.maxdepth;
   if (out->vm_depth > kMaxStack) return "lowered program too deep";
   if (opt.fuse) fold_sum(out);
@@ -762,6 +764,24 @@ std::string lower_program(const uint32_t* code, size_t n_words,
   if ((int)(out->vm.size() + out->sum_tail.size()) > kMaxVmInstr)
     return "lowered program too long";
   return "";
+}
+
+std::vector<uint32_t> square_pow2(const uint32_t* code, size_t n_words,
+                                  const double* consts) {
+  std::vector<uint32_t> out;
+  for (size_t pc = 0; pc + 1 < n_words; pc += 2) {
+    const uint32_t op = code[pc], arg = code[pc + 1];
+    if (op == OP_PUSH_CONST && pc + 3 < n_words && code[pc + 2] == OP_POW &&
+        consts[arg] == 2.0) {
+      out.push_back(OP_SQUARE);
+      out.push_back(0);
+      pc += 2;
+      continue;
+    }
+    out.push_back(op);
+    out.push_back(arg);
+  }
+  return out;
 }
 
 std::string disassemble(const Lowered& L) {

@@ -68,6 +68,12 @@ std::string lower_program(const uint32_t* code, size_t n_words,
 double eval_ref64(const uint32_t* code, size_t n_words, const double* consts,
                   const double* params, const double* row);
 
+// A parameter program (hoist or prior, reference opcodes) with every
+// `PUSH_CONST c; POW` whose constant c is 2 replaced by OP_SQUARE: x * x is
+// the correctly rounded value of pow(x, 2), without the general routine.
+std::vector<uint32_t> square_pow2(const uint32_t* code, size_t n_words,
+                                  const double* consts);
+
 // Human-readable listing of a lowered program (debugging aid).
 std::string disassemble(const Lowered& L);
 

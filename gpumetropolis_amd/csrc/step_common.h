@@ -47,6 +47,7 @@ static __device__ __noinline__ double eval_params64(const uint32_t* __restrict__
       case OP_STORE_U: U[arg] = (float)st[--sp]; break;
       case OP_STORE_U_RCP: U[arg] = (float)(1.0 / st[--sp]); break;
       case OP_STORE_U_NEG: U[arg] = (float)(-st[--sp]); break;
+      case OP_SQUARE: st[sp - 1] = st[sp - 1] * st[sp - 1]; break;
       default: break;
     }
   }

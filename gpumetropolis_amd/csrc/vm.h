@@ -140,6 +140,8 @@ enum RefOp : uint32_t {
   OP_STORE_U = 100,      // pop -> U[arg] = (float)v
   OP_STORE_U_RCP = 101,  // pop -> U[arg] = (float)(1 / v)
   OP_STORE_U_NEG = 102,  // pop -> U[arg] = (float)(-v)
+  // x^2 with a constant exponent 2, rewritten by square_pow2 (lower.cpp):
+  OP_SQUARE = 103,       // top -> top * top (what a correctly rounded pow gives)
 };
 
 }  // namespace gpum
