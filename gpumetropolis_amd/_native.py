@@ -223,9 +223,12 @@ def _p(a: Optional[np.ndarray], t):
 @dataclass
 class RunOutput:
     """The reference's ChainResult (chain_kernel.rs:360-369) as numpy."""
-    draws: np.ndarray          # (n_iter, n_chains, n_params) float32
-    accept_rate: np.ndarray    # (n_chains,) float32
-    last_log_post: np.ndarray  # (n_chains,) float64
+    draws: Optional[np.ndarray]  # (n_iter, n_chains, n_params) float32
+    # With raw_draws: the library's layout, (n_params, n_chains, n_iter),
+    # and draws is None (the caller rearranges when it needs to).
+    draws_pct: Optional[np.ndarray] = None
+    accept_rate: np.ndarray = None    # (n_chains,) float32
+    last_log_post: np.ndarray = None  # (n_chains,) float64
     final_state: Optional[np.ndarray] = None
     trace_plp: Optional[np.ndarray] = None
     trace_prop: Optional[np.ndarray] = None
@@ -244,14 +247,17 @@ class RunOutput:
     trace_stage: Optional[np.ndarray] = None  # (n_iter, n_chains) uint8
 
 
-def _collect(res: ChainResult) -> RunOutput:
+def _collect(res: ChainResult, raw: bool = False) -> RunOutput:
     T, Cc, P = res.n_iter, res.n_chains, res.n_params
     n = T * Cc * P
     flat = np.ctypeslib.as_array(res.draws, shape=(max(n, 1),))[:n].copy()
-    draws = flat.reshape((P, Cc, T)).transpose(2, 1, 0).copy()
     acc = np.ctypeslib.as_array(res.accept_rate, shape=(Cc,)).copy()
     lp = np.ctypeslib.as_array(res.last_log_post, shape=(Cc,)).copy()
     lib().gpum_hip_result_free(C.byref(res))
+    if raw:
+        return RunOutput(draws=None, draws_pct=flat.reshape((P, Cc, T)),
+                         accept_rate=acc, last_log_post=lp)
+    draws = flat.reshape((P, Cc, T)).transpose(2, 1, 0).copy()
     return RunOutput(draws=draws, accept_rate=acc, last_log_post=lp)
 
 
@@ -423,7 +429,7 @@ class DeviceModel:
     def run(self, init, proposal_sd, temperatures, n_iter, seed,
             proposal_mode=0, gamma=0.0, de_noise=0.0, proposal_l=None,
             trace=False, final_state=False, stream_id=0,
-            tiling=None, batches=None) -> RunOutput:
+            tiling=None, batches=None, raw_draws=False) -> RunOutput:
         np_ = self.n_params
         init, sd, temps = _f64(init), _f64(proposal_sd), _f64(temperatures)
         pl = None if proposal_l is None else _f64(proposal_l)
@@ -436,7 +442,8 @@ class DeviceModel:
             _p(temps, C.c_double), n_iter, seed & 0xFFFFFFFF, proposal_mode,
             gamma, de_noise, _p(pl, C.c_double), C.byref(ext), C.byref(res))
         _check(rc)
-        return _finish(_collect(res), ext, keep, n_iter, n_chains, np_)
+        return _finish(_collect(res, raw_draws), ext, keep, n_iter, n_chains,
+                       np_)
 
     def run_da(self, surrogate, init, temperatures, n_iter, seed,
                proposal_sd=None, proposal_l=None, trace=False,

@@ -7,9 +7,9 @@
 //   E::run(H, U, tile, ld, lane, T)
 //                   evaluate the program on one chunk into T[kCw][kOpt];
 //   E::mode(H)      how a chunk is summed (SumMode);
-//   E::run_acc(H, U, tile, ld, lane, s)
-//                   SUM_AFFINE: the chunk's sums s[kCw], accumulated
-//                   directly;
+//   E::acc_state(H, U), E::run_acc(H, state, U, tile, ld, lane, s)
+//                   SUM_AFFINE: what a tile pass keeps (e.g. the uniforms),
+//                   and the chunk's sums s[kCw], accumulated directly;
 //   E::finish(H, U, s, with_const)
 //                   apply the sum tail to one chain's data-pass sum;
 //   E::Params::hoist(code, n, consts, p, U), E::Params::prior(code, n,
@@ -341,13 +341,16 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
   constexpr int OPT = E::kOpt, CW = E::kCw;
   const uint32_t chunk = 64u * OPT;
   const uint32_t nchunks = H.tile_len / chunk;
+  // The affine form's uniforms, read once per tile pass.
+  const typename E::AccState ust = E::acc_state(H, U);
+#pragma unroll 2
   for (uint32_t ck = part; ck < nchunks; ck += wpc) {
     const uint32_t base = ck * chunk;
     const uint32_t first = t0 + base;
     float s[CW];
     if (first + chunk <= H.n_obs) {
       if constexpr (MODE == SUM_AFFINE) {
-        E::run_acc(H, U, tile + base, H.tile_len, lane, s);
+        E::run_acc(H, ust, U, tile + base, H.tile_len, lane, s);
       } else {
         float T[CW][OPT];
         E::run(H, U, tile + base, H.tile_len, lane, T);
@@ -769,7 +772,14 @@ struct VmEval {
   static __device__ __forceinline__ uint32_t mode(const ChainHot& H) {
     return H.sum_mode;
   }
+  // The interpreter reads the uniforms as it goes.
+  struct AccState {};
+  static __device__ __forceinline__ AccState acc_state(const ChainHot&,
+                                                       const float*) {
+    return AccState{};
+  }
   static __device__ __forceinline__ void run_acc(const ChainHot& H,
+                                                 const AccState&,
                                                  const float* U,
                                                  const float* tile,
                                                  uint32_t ld, int lane,

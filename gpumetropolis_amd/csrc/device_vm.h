@@ -348,6 +348,21 @@ __device__ __forceinline__ void vm_acc_fma(const float* __restrict__ U,
   }
 }
 
+// vm_acc_fma with the chains' uniforms already in registers.
+template <int OPT, int CW>
+__device__ __forceinline__ void vm_acc_fma_u(const float (&ua)[CW],
+                                             const float* __restrict__ tile,
+                                             uint32_t ld, int lane, uint32_t b,
+                                             float (&s)[CW]) {
+  float D[OPT];
+  vm_load_col<OPT>(tile + b * ld, lane, D);
+#pragma unroll
+  for (int j = 0; j < CW; ++j) {
+#pragma unroll
+    for (int k = 0; k < OPT; ++k) s[j] = fmaf(ua[j], D[k], s[j]);
+  }
+}
+
 // The chunk sums of an affine program: its FMA_UD instructions (pc >= 1;
 // instruction 0 loads the zero the per-observation form starts from).
 template <int OPT, int CW>

@@ -7,6 +7,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <map>
 #include <mutex>
@@ -232,16 +233,28 @@ std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
   o << "  }\n"
     << "  static __device__ __forceinline__ uint32_t mode(const ChainHot&) {\n"
     << "    return " << L.sum_mode << "u;\n  }\n"
+    << "  struct AccState { float u[" << std::max<size_t>(1, L.vm.size())
+    << "][" << cw << "]; };\n"
+    << "  static __device__ __forceinline__ AccState acc_state(const ChainHot& H,\n"
+    << "      const float* __restrict__ U) {\n"
+    << "    AccState st;\n"
+    << "    const uint32_t nu = H.n_uniform_pad;\n";
+  // The uniforms of the affine form's products, read once per tile pass.
+  if (L.sum_mode == SUM_AFFINE)
+    for (size_t pc = 1; pc < L.vm.size(); ++pc)
+      o << "    for (int j = 0; j < " << cw << "; ++j) st.u[" << pc
+        << "][j] = U[j * nu + " << vm_a(L.vm[pc]) << "u];\n";
+  o << "    return st;\n  }\n"
     << "  static __device__ __forceinline__ void run_acc(const ChainHot& H,\n"
-    << "      const float* __restrict__ U, const float* __restrict__ tile,\n"
-    << "      uint32_t ld, int lane, float (&s)[" << cw << "]) {\n"
-    << "    const uint32_t nu = H.n_uniform_pad;\n"
+    << "      const AccState& st, const float* __restrict__ U,\n"
+    << "      const float* __restrict__ tile, uint32_t ld, int lane,\n"
+    << "      float (&s)[" << cw << "]) {\n"
     << "    for (int j = 0; j < " << cw << "; ++j) s[j] = 0.0f;\n";
   // The interpreter's vm_run_acc order: instructions 1.. of the program.
   if (L.sum_mode == SUM_AFFINE)
     for (size_t pc = 1; pc < L.vm.size(); ++pc)
-      o << "    vm_acc_fma<" << opt << ", " << cw << ">(U, nu, tile, ld, lane, "
-        << vm_a(L.vm[pc]) << "u, " << vm_b(L.vm[pc]) << "u, s);\n";
+      o << "    vm_acc_fma_u<" << opt << ", " << cw << ">(st.u[" << pc
+        << "], tile, ld, lane, " << vm_b(L.vm[pc]) << "u, s);\n";
   o << "  }\n"
     << "  static __device__ __forceinline__ double finish(const ChainHot& H,\n"
     << "      const float* U, double s, bool with_const) {\n";

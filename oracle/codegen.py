@@ -1,6 +1,6 @@
 """Per-model compiled log-likelihood for the CPU baseline.
 
-TEST / BASELINE INFRASTRUCTURE ONLY.  Mirrors the reference's Cranelift JIT
+TEST AND CPU-BASELINE INFRASTRUCTURE ONLY.  Mirrors the reference's Cranelift JIT
 (src/rust/src/jit.rs:441-631): the bytecode becomes straight-line f64 code
 inside the observation loop, accumulated in observation order, calling libm
 exp/log/pow and the Lanczos log-gamma.  The C source is compiled with the
