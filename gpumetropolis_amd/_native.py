@@ -144,6 +144,10 @@ def lib():
     L.gpum_hip_model_create.restype = C.c_void_p
     L.gpum_hip_model_info.argtypes = [C.c_void_p, C.POINTER(ModelInfo)]
     L.gpum_hip_model_info.restype = C.c_int
+    L.gpum_hip_lower_info.argtypes = [_u32p, C.c_size_t, _f64p, C.c_size_t,
+                                      C.c_uint32, C.c_uint32, C.c_int32,
+                                      C.c_int32, C.POINTER(ModelInfo)]
+    L.gpum_hip_lower_info.restype = C.c_int
     L.gpum_hip_model_run.argtypes = [C.c_void_p, _f64p, C.c_uint32, _f64p,
                                      _f64p, C.c_uint32, C.c_uint32,
                                      C.c_uint32, C.c_double, C.c_double,
@@ -358,6 +362,20 @@ def run_chains(code, consts, prior_code, prior_consts, n_params, data, n_cols,
         C.byref(res))
     _check(rc)
     return _finish(_collect(res), ext, keep, n_iter, n_chains, n_params)
+
+
+def lower_info(code, consts, n_params, n_cols, precompute_data=None,
+               fuse=None) -> dict:
+    """What the kernels would run for a log-likelihood program (host-only
+    lowering, no device)."""
+    L = lib()
+    code, consts = _u32(code), _f64(consts)
+    inf = ModelInfo()
+    _check(L.gpum_hip_lower_info(
+        _p(code, C.c_uint32), code.size, _p(consts, C.c_double), consts.size,
+        n_params, n_cols, -1 if precompute_data is None else int(precompute_data),
+        -1 if fuse is None else int(fuse), C.byref(inf)))
+    return {k: getattr(inf, k) for k, _ in ModelInfo._fields_}
 
 
 def loglik_batch(code, consts, n_params, data, n_cols, n_obs, points):

@@ -569,21 +569,57 @@ void gpum_hip_model_free(gpum_hip_model* m) {
   delete m;
 }
 
+static void fill_info(const Lowered& L, uint32_t n_cols_total,
+                      gpum_model_info* info) {
+  info->n_vm_instr = (uint32_t)L.vm.size();
+  info->vm_depth = (uint32_t)L.vm_depth;
+  info->ref_depth = (uint32_t)L.ref_depth;
+  info->n_uniform = (uint32_t)L.n_uniform;
+  info->n_cols_total = n_cols_total;
+  info->n_hoist_words = (uint32_t)L.hoist.size();
+  info->w_exec = L.w_exec;
+  info->w_bytecode = L.w_bytecode;
+  info->sum_mode = L.empty ? 0u : L.sum_mode;
+  info->n_sum_tail = L.empty ? 0u : (uint32_t)L.sum_tail.size();
+}
+
 int gpum_hip_model_info(const gpum_hip_model* m, gpum_model_info* info) {
   if (!m || !info) {
     set_err("null model or info");
     return 1;
   }
-  info->n_vm_instr = (uint32_t)m->L.vm.size();
-  info->vm_depth = (uint32_t)m->L.vm_depth;
-  info->ref_depth = (uint32_t)m->L.ref_depth;
-  info->n_uniform = (uint32_t)m->L.n_uniform;
-  info->n_cols_total = m->n_cols_total;
-  info->n_hoist_words = (uint32_t)m->L.hoist.size();
-  info->w_exec = m->L.w_exec;
-  info->w_bytecode = m->L.w_bytecode;
-  info->sum_mode = m->L.empty ? 0u : m->L.sum_mode;
-  info->n_sum_tail = m->L.empty ? 0u : (uint32_t)m->L.sum_tail.size();
+  fill_info(m->L, m->n_cols_total, info);
+  return 0;
+}
+
+int gpum_hip_lower_info(const uint32_t* code, size_t code_len,
+                        const double* consts, size_t n_consts,
+                        uint32_t n_params, uint32_t n_cols,
+                        int32_t precompute_data, int32_t fuse,
+                        gpum_model_info* info) {
+  if (!info || (code_len && !code) || (n_consts && !consts)) {
+    set_err("null program, constants or info");
+    return 1;
+  }
+  if (n_params > (uint32_t)kMaxParams) {
+    set_err("at most %d parameters", kMaxParams);
+    return 1;
+  }
+  LowerOptions opt;
+  opt.precompute_data = precompute_data != 0;
+  opt.fuse = fuse != 0;
+  Lowered L;
+  std::string e = lower_program(code, code_len, consts, n_consts, n_params,
+                                n_cols, opt, &L);
+  if (!e.empty()) {
+    set_err("log-likelihood: %s", e.c_str());
+    return 1;
+  }
+  if (L.n_uniform > kMaxUniform) {
+    set_err("too many uniform operands");
+    return 1;
+  }
+  fill_info(L, L.empty ? 0u : n_cols + (uint32_t)L.extra_cols.size(), info);
   return 0;
 }
 

@@ -183,6 +183,17 @@ gpum_hip_model* gpum_hip_model_create(
 
 int gpum_hip_model_info(const gpum_hip_model* m, gpum_model_info* info);
 
+/* What the kernels would run for a log-likelihood program, without a device:
+ * the lowering of gpum_hip_model_create on the host only (validation,
+ * hoisting, precomputed columns, fusion, summation fold, common
+ * subexpressions).  No counterpart in the reference; for inspecting a model
+ * before it is uploaded.  fuse / precompute_data as gpum_hip_model_create. */
+int gpum_hip_lower_info(const uint32_t* code, size_t code_len,
+                        const double* consts, size_t n_consts,
+                        uint32_t n_params, uint32_t n_cols,
+                        int32_t precompute_data, int32_t fuse,
+                        gpum_model_info* info);
+
 /* One batch on a resident model; arguments as gpum_hip_run_chains_ex. */
 int gpum_hip_model_run(gpum_hip_model* m, const double* init,
                        uint32_t n_chains, const double* proposal_sd,
