@@ -75,6 +75,8 @@ def parse():
     p.add_argument("--no-pt", action="store_true",
                    help="skip the parallel-tempering workload")
     p.add_argument("--pt-iters", type=int, default=4000)
+    p.add_argument("--no-eval", action="store_true",
+                   help="skip the FP64 pointwise / gradient section")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the many-model (512 regressions) section")
     p.add_argument("--batch-models", type=int, default=512)
@@ -353,6 +355,68 @@ def batch_section(a):
                                           for f in bf.fits]))}
 
 
+def eval_section(a):
+    """FP64 batch evaluation at many parameter points: the per-observation
+    log-likelihood matrix (64 posterior draws x the 1e6 gamma rows, 512 MB
+    of output in two device chunks) and the gradient of the summed data term
+    at a Cramer-Rao stencil (33 points = centre +- h e_k, logistic-16, 1e6
+    rows).  Kernel time from HIP events; the oracle times a sample of the
+    same calls on one host thread."""
+    from gpumetropolis_amd import _native, workloads
+    from oracle import oracle
+
+    out = {}
+    wg = workloads.gamma(n=a.n_obs)
+    mg = wg.model
+    rng = np.random.default_rng(a.seed)
+    pts = np.array([np.log(2.5), np.log(0.7)]) + 0.002 * rng.standard_normal((64, 2))
+    args = (mg.loglik.code, mg.loglik.consts, 2, wg.data, 1, wg.n_obs)
+    _native.loglik_pointwise(*args, pts[:2])  # warm the kernel
+    t0 = time.perf_counter()
+    _native.loglik_pointwise(*args, pts)
+    wall = time.perf_counter() - t0
+    ms = _native.last_eval_ms()
+    evals = 64 * wg.n_obs
+    t0 = time.perf_counter()
+    oracle.loglik_pointwise(*args, pts[:2])
+    cpu_rate = 2 * wg.n_obs / (time.perf_counter() - t0)
+    # Algorithmic bytes per evaluation: the FP64 output written and the FP64
+    # data read (one column).
+    bpe = 8 + 8 * wg.n_cols
+    gbps = bpe * evals / (ms / 1e3) / 1e9
+    out["pointwise"] = {
+        "workload": f"gamma, {wg.n_obs} rows x 64 points, FP64",
+        "kernel_ms": ms, "evals_per_s": evals / (ms / 1e3),
+        "wall_s_incl_pcie": wall,
+        "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": gbps * 1e9 / HBM_PEAK,
+                     "bytes_per_eval": bpe},
+        "cpu_oracle_evals_per_s": cpu_rate, "cpu_cores": 1}
+    wl = workloads.logistic(n=1_000_000)
+    ml = wl.model
+    h = 1e-3
+    centre = np.zeros(16)
+    stencil = np.vstack([centre] + [centre + s * h * np.eye(16)[k]
+                                    for k in range(16) for s in (1, -1)])
+    args = (ml.loglik.code, ml.loglik.consts, 16, wl.data, wl.n_cols, wl.n_obs)
+    _native.grad_batch(*args, stencil[:1])
+    t0 = time.perf_counter()
+    _native.grad_batch(*args, stencil)
+    wall = time.perf_counter() - t0
+    ms = _native.last_eval_ms()
+    evals = len(stencil) * wl.n_obs
+    t0 = time.perf_counter()
+    oracle.grad_batch(*args, stencil[:1])
+    cpu_rate = wl.n_obs / (time.perf_counter() - t0)
+    out["gradient"] = {
+        "workload": f"logistic-16, {wl.n_obs} rows x {len(stencil)} points "
+                    "(Cramer-Rao stencil), FP64 reverse mode",
+        "kernel_ms": ms, "obs_grads_per_s": evals / (ms / 1e3),
+        "wall_s_incl_upload": wall,
+        "cpu_oracle_obs_grads_per_s": cpu_rate, "cpu_cores": 1}
+    return out
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -507,6 +571,8 @@ def main():
         out["parallel_tempering"] = pt_section(a)
     if not a.no_batch:
         out["batched_models"] = batch_section(a)
+    if not a.no_eval:
+        out["batch_evaluation"] = eval_section(a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -137,6 +137,16 @@ def lib():
                                         _f64p, C.c_uint32, C.c_uint32, _f64p,
                                         C.c_uint32, _f64p]
     L.gpum_hip_loglik_batch.restype = C.c_int
+    for fn in (L.gpum_hip_loglik_pointwise, L.gpum_hip_grad_batch):
+        fn.argtypes = [_u32p, sz, _f64p, sz, C.c_uint32, _f64p, C.c_uint32,
+                       C.c_uint32, _f64p, C.c_uint32, _f64p]
+        fn.restype = C.c_int
+    L.gpum_hip_last_eval_ms.argtypes = []
+    L.gpum_hip_last_eval_ms.restype = C.c_double
+    L.gpum_hip_jit_check.argtypes = [_u32p, sz, _f64p, sz, _u32p, sz, _f64p,
+                                     sz, C.c_uint32, C.c_uint32, C.c_int32,
+                                     C.c_uint32, C.POINTER(C.c_double)]
+    L.gpum_hip_jit_check.restype = C.c_int
     L.gpum_hip_model_create.argtypes = [_u32p, sz, _f64p, sz, _u32p, sz,
                                         _f64p, sz, C.c_uint32, _f64p,
                                         C.c_uint32, C.c_uint32, C.c_int32,
@@ -391,6 +401,63 @@ def loglik_batch(code, consts, n_params, data, n_cols, n_obs, points):
                                  _p(out, C.c_double))
     _check(rc)
     return out[:n_points]
+
+
+def loglik_pointwise(code, consts, n_params, data, n_cols, n_obs, points):
+    """rust_loglik_pointwise (lib.rs:187-207) on the HIP backend: the FP64
+    log-likelihood of each observation at each point, (n_points, n_obs)."""
+    L = lib()
+    code, consts, data, points = _u32(code), _f64(consts), _f64(data), _f64(points)
+    n_points = points.size // n_params if n_params else 0
+    out = np.zeros(max(n_points * n_obs, 1), np.float64)
+    rc = L.gpum_hip_loglik_pointwise(
+        _p(code, C.c_uint32), code.size, _p(consts, C.c_double), consts.size,
+        n_params, _p(data, C.c_double), n_cols, n_obs,
+        _p(points, C.c_double), n_points, _p(out, C.c_double))
+    _check(rc)
+    return out[:n_points * n_obs].reshape(n_points, n_obs)
+
+
+def grad_batch(code, consts, n_params, data, n_cols, n_obs, points):
+    """rust_grad_batch (lib.rs:158-179) on the HIP backend: the FP64 gradient
+    of the summed data term at each point, (n_points, n_params)."""
+    L = lib()
+    code, consts, data, points = _u32(code), _f64(consts), _f64(data), _f64(points)
+    n_points = points.size // n_params if n_params else 0
+    out = np.zeros(max(n_points * n_params, 1), np.float64)
+    rc = L.gpum_hip_grad_batch(
+        _p(code, C.c_uint32), code.size, _p(consts, C.c_double), consts.size,
+        n_params, _p(data, C.c_double), n_cols, n_obs,
+        _p(points, C.c_double), n_points, _p(out, C.c_double))
+    _check(rc)
+    return out[:n_points * n_params].reshape(n_points, n_params)
+
+
+JIT_KINDS = ("chain_stream", "chain_resident", "points_stream",
+             "points_resident", "batch")
+
+
+def jit_check(code, consts, prior_code, prior_consts, n_params, n_cols,
+              chains_per_wave=1, kinds=JIT_KINDS) -> float:
+    """Compile a model's per-model kernels with hiprtc without a device
+    (gpum_hip_jit_check); returns the compile time in ms, raises on a
+    compile error."""
+    L = lib()
+    code, prior_code = _u32(code), _u32(prior_code)
+    consts, prior_consts = _f64(consts), _f64(prior_consts)
+    mask = sum(1 << JIT_KINDS.index(k) for k in kinds)
+    ms = C.c_double(0.0)
+    _check(L.gpum_hip_jit_check(
+        _p(code, C.c_uint32), code.size, _p(consts, C.c_double), consts.size,
+        _p(prior_code, C.c_uint32), prior_code.size,
+        _p(prior_consts, C.c_double), prior_consts.size, n_params, n_cols,
+        chains_per_wave, mask, C.byref(ms)))
+    return ms.value
+
+
+def last_eval_ms() -> float:
+    """Kernel time of this thread's last loglik_pointwise / grad_batch."""
+    return float(lib().gpum_hip_last_eval_ms())
 
 
 class DeviceModel:

@@ -21,10 +21,12 @@ OBJDIR = os.path.join(os.path.dirname(HERE), "_build")
 SOURCES = ["chain_kernel_inst0.hip", "chain_kernel_inst1.hip",
            "chain_kernel_inst2.hip", "chain_kernel_inst3.hip",
            "da_kernels.hip", "batch_kernels.hip", "mlp.hip", "chain_kernel.hip",
-           "gpum_hip.cpp", "da.cpp", "batch.cpp", "jit.cpp", "lower.cpp"]
+           "eval64.hip", "gpum_hip.cpp", "da.cpp", "batch.cpp", "jit.cpp",
+           "lower.cpp", "eval64.cpp"]
 HEADERS = ["vm.h", "lower.h", "chain_kernel.h", "chain_kernels.h",
            "device_vm.h", "math64.h", "philox.h", "runtime.h", "step_common.h",
-           "bf16.h", "mlp.h", "da.h", "batch.h", "chain_body.h", "jit.h"]
+           "bf16.h", "mlp.h", "da.h", "batch.h", "chain_body.h", "jit.h",
+           "eval64.h"]
 # Device headers embedded into the library for the per-model kernels
 # compiled at run time (csrc/jit.cpp).
 JIT_HEADERS = ["chain_body.h", "chain_kernel.h", "device_vm.h", "math64.h",
@@ -85,7 +87,8 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
     write_jit_sources()
-    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    # Objects named after the whole source name (x.hip and x.cpp may pair).
+    objs = [os.path.join(OBJDIR, s + ".o") for s in SOURCES]
     cmds = [[hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, s), "-o", o]
             for s, o in zip(SOURCES, objs)]
     jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))

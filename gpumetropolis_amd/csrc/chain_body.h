@@ -341,48 +341,58 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
   constexpr int OPT = E::kOpt, CW = E::kCw;
   const uint32_t chunk = 64u * OPT;
   const uint32_t nchunks = H.tile_len / chunk;
+  // Chunks wholly inside the data, and chunks holding any observation.  The
+  // counts are computed once, so the loops below carry no per-chunk test
+  // and their counters stay scalar.
+  const uint32_t left = H.n_obs > t0 ? H.n_obs - t0 : 0u;
+  const uint32_t nfull = min(nchunks, left / chunk);
+  const uint32_t nlive = min(nchunks, (left + chunk - 1) / chunk);
   // The affine form's uniforms, read once per tile pass.
   const typename E::AccState ust = E::acc_state(H, U);
+  uint32_t ck = part;
 #pragma unroll 2
-  for (uint32_t ck = part; ck < nchunks; ck += wpc) {
+  for (; ck < nfull; ck += wpc) {
     const uint32_t base = ck * chunk;
-    const uint32_t first = t0 + base;
     float s[CW];
-    if (first + chunk <= H.n_obs) {
-      if constexpr (MODE == SUM_AFFINE) {
-        E::run_acc(H, ust, U, tile + base, H.tile_len, lane, s);
-      } else {
-        float T[CW][OPT];
-        E::run(H, U, tile + base, H.tile_len, lane, T);
-#pragma unroll
-        for (int j = 0; j < CW; ++j) {
-          if constexpr (MODE == SUM_SQUARE) {
-            s[j] = 0.0f;
-#pragma unroll
-            for (int k = 0; k < OPT; ++k) s[j] = fmaf(T[j][k], T[j][k], s[j]);
-          } else {
-            s[j] = T[j][0];
-#pragma unroll
-            for (int k = 1; k < OPT; ++k) s[j] += T[j][k];
-          }
-        }
-      }
+    if constexpr (MODE == SUM_AFFINE) {
+      E::run_acc(H, ust, U, tile + base, H.tile_len, lane, s);
     } else {
-      // The last chunk of the data: observations past n_obs count zero.
       float T[CW][OPT];
       E::run(H, U, tile + base, H.tile_len, lane, T);
 #pragma unroll
       for (int j = 0; j < CW; ++j) {
-        s[j] = 0.0f;
+        if constexpr (MODE == SUM_SQUARE) {
+          s[j] = 0.0f;
 #pragma unroll
-        for (int k = 0; k < OPT; ++k) {
-          uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
-          const float v = obs < H.n_obs ? T[j][k] : 0.0f;
-          if constexpr (MODE == SUM_SQUARE)
-            s[j] = fmaf(v, v, s[j]);
-          else
-            s[j] = s[j] + v;
+          for (int k = 0; k < OPT; ++k) s[j] = fmaf(T[j][k], T[j][k], s[j]);
+        } else {
+          s[j] = T[j][0];
+#pragma unroll
+          for (int k = 1; k < OPT; ++k) s[j] += T[j][k];
         }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
+  }
+  for (; ck < nlive; ck += wpc) {
+    const uint32_t base = ck * chunk;
+    const uint32_t first = t0 + base;
+    // The last chunk of the data: observations past n_obs count zero.
+    float s[CW];
+    float T[CW][OPT];
+    E::run(H, U, tile + base, H.tile_len, lane, T);
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+      s[j] = 0.0f;
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) {
+        uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
+        const float v = obs < H.n_obs ? T[j][k] : 0.0f;
+        if constexpr (MODE == SUM_SQUARE)
+          s[j] = fmaf(v, v, s[j]);
+        else
+          s[j] = s[j] + v;
       }
     }
 #pragma unroll

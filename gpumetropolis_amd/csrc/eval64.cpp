@@ -1,0 +1,288 @@
+// eval64.cpp -- FP64 evaluation of a log-likelihood program at many
+// parameter points on the device: the per-observation matrix behind WAIC /
+// PSIS-LOO and the gradient behind the Cramer-Rao diagnostic
+// (cpu_native.rs:306-329, 371-391; their FFI lib.rs:158-207).
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "eval64.h"
+#include "runtime.h"
+
+namespace gpum {
+namespace {
+
+using rt::DevBuf;
+using rt::g_dev;
+using rt::set_err;
+using rt::set_hip_err;
+
+thread_local double g_last_eval_ms = 0.0;
+
+// Device buffers bounded per launch: a pointwise chunk's output and a
+// gradient chunk's partial sums.
+constexpr size_t kPointwiseChunkBytes = size_t(256) << 20;
+constexpr size_t kGradPartialBytes = size_t(64) << 20;
+
+bool is_leaf(uint32_t op) { return op <= OP_PUSH_DATA; }
+bool is_unary_op(uint32_t op) {
+  return op == OP_NEG || op == OP_EXP || op == OP_LOG || op == OP_SQRT ||
+         op == OP_LGAMMA;
+}
+
+// The program with each instruction's operand positions on the tape: a
+// stack of the instructions whose values are live, as the reference's
+// forward pass keeps (cpu_native.rs:156-211).
+std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops) {
+  std::vector<TapeOp> tape(n_ops);
+  std::vector<uint32_t> st;
+  for (size_t pc = 0; pc < n_ops; ++pc) {
+    TapeOp o{code[2 * pc], code[2 * pc + 1], 0u, 0u};
+    if (!is_leaf(o.op)) {
+      if (is_unary_op(o.op)) {
+        o.l = st.back();
+        st.pop_back();
+      } else {
+        o.r = st.back();
+        st.pop_back();
+        o.l = st.back();
+        st.pop_back();
+      }
+    }
+    st.push_back((uint32_t)pc);
+    tape[pc] = o;
+  }
+  return tape;
+}
+
+// Everything one call uploads.
+struct EvalSetup {
+  DevBuf<TapeOp> prog;
+  DevBuf<double> consts, data, points;
+  Eval64Args a{};
+};
+
+// The program's checks (validate_program) plus its peak stack depth.
+bool check_program(const uint32_t* code, size_t code_len, size_t n_consts,
+                   uint32_t n_params, uint32_t n_cols, int* depth) {
+  if (code_len > 0 && !code) {
+    set_err("code must not be NULL");
+    return false;
+  }
+  std::string e = validate_program(code, code_len, n_consts, n_params, n_cols,
+                                   true, depth);
+  if (!e.empty()) {
+    set_err("%s", e.c_str());
+    return false;
+  }
+  return true;
+}
+
+bool setup(const uint32_t* code, size_t code_len, const double* consts,
+           size_t n_consts, uint32_t n_params, const double* data,
+           uint32_t n_cols, uint32_t n_obs, const double* points,
+           uint32_t n_points, int depth, EvalSetup* S) {
+  if (n_consts > 0 && !consts) {
+    set_err("consts must not be NULL");
+    return false;
+  }
+  if ((size_t)n_obs * n_cols > 0 && !data) {
+    set_err("data must not be NULL");
+    return false;
+  }
+  const size_t n_ops = code_len / 2;
+  std::vector<TapeOp> tape = make_tape(code, n_ops);
+  // Row-major (n_obs x n_cols) in, column-major on the device.
+  std::vector<double> cols((size_t)n_obs * n_cols);
+  for (uint32_t i = 0; i < n_obs; ++i)
+    for (uint32_t c = 0; c < n_cols; ++c)
+      cols[(size_t)c * n_obs + i] = data[(size_t)i * n_cols + c];
+  if (!S->prog.upload(tape.data(), tape.size(), "uploading the program") ||
+      !S->consts.upload(consts, n_consts, "uploading constants") ||
+      !S->data.upload(cols.data(), cols.size(), "uploading data") ||
+      !S->points.upload(points, (size_t)n_points * n_params,
+                        "uploading points"))
+    return false;
+  Eval64Args& a = S->a;
+  a.prog = S->prog.p;
+  a.n_ops = (uint32_t)n_ops;
+  a.n_params = n_params;
+  a.consts = S->consts.p;
+  a.data = S->data.p;
+  a.n_obs = n_obs;
+  a.n_points = n_points;
+  a.points = S->points.p;
+  a.depth = (uint32_t)depth;
+  return true;
+}
+
+uint32_t grid_for(uint64_t items) {
+  const uint64_t cap = (uint64_t)g_dev.n_cu * 8u;
+  return (uint32_t)std::max<uint64_t>(1, std::min(items, cap));
+}
+
+// Kernel time between two events already recorded on the stream.
+double elapsed_ms(hipEvent_t e0, hipEvent_t e1) {
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, e0, e1);
+  return (double)ms;
+}
+
+struct Events {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  Events() {
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+  }
+  ~Events() {
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+};
+
+}  // namespace
+}  // namespace gpum
+
+using namespace gpum;
+
+extern "C" {
+
+int gpum_hip_loglik_pointwise(const uint32_t* code, size_t code_len,
+                              const double* consts, size_t n_consts,
+                              uint32_t n_params, const double* data,
+                              uint32_t n_cols, uint32_t n_obs,
+                              const double* points, uint32_t n_points,
+                              double* out) {
+  std::lock_guard<std::mutex> lk(rt::g_mu);
+  g_last_eval_ms = 0.0;
+  if (!rt::device_init()) return 1;
+  hipSetDevice(g_dev.id);
+  int depth = 0;
+  if (!check_program(code, code_len, n_consts, n_params, n_cols, &depth))
+    return 1;
+  // As the reference: the point count is points.len() / n_params.
+  if (n_params == 0) n_points = 0;
+  if (n_points == 0 || n_obs == 0) return 0;
+  if (!points || !out) {
+    set_err("points and out must not be NULL");
+    return 1;
+  }
+  if (code_len == 0) {
+    // An empty program evaluates to 0 (cpu_native.rs:82-84).
+    memset(out, 0, (size_t)n_points * n_obs * sizeof(double));
+    return 0;
+  }
+  EvalSetup S;
+  if (!setup(code, code_len, consts, n_consts, n_params, data, n_cols, n_obs,
+             points, n_points, depth, &S))
+    return 1;
+  const uint32_t chunk = (uint32_t)std::max<size_t>(
+      1, std::min<size_t>(n_points,
+                          kPointwiseChunkBytes / ((size_t)n_obs * 8)));
+  DevBuf<double> d_out;
+  if (!d_out.alloc((size_t)chunk * n_obs, "allocating the output")) return 1;
+  hipStream_t st = g_dev.stream;
+  Events ev;
+  double ms = 0.0;
+  for (uint32_t p0 = 0; p0 < n_points; p0 += chunk) {
+    const uint32_t np = std::min(chunk, n_points - p0);
+    Eval64Args a = S.a;
+    a.points = S.points.p + (size_t)p0 * n_params;
+    a.n_points = np;
+    a.out = d_out.p;
+    const uint64_t blocks = (uint64_t)((n_obs + 255u) / 256u) * np;
+    hipError_t e = hipEventRecord(ev.e0, st);
+    if (e == hipSuccess) e = launch_pointwise64(a, grid_for(blocks), st);
+    if (e == hipSuccess) e = hipEventRecord(ev.e1, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out + (size_t)p0 * n_obs, d_out.p,
+                         (size_t)np * n_obs * sizeof(double),
+                         hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      set_hip_err("pointwise log-likelihood", e);
+      return 1;
+    }
+    ms += elapsed_ms(ev.e0, ev.e1);
+  }
+  g_last_eval_ms = ms;
+  return 0;
+}
+
+int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
+                        const double* consts, size_t n_consts,
+                        uint32_t n_params, const double* data, uint32_t n_cols,
+                        uint32_t n_obs, const double* points, uint32_t n_points,
+                        double* out) {
+  std::lock_guard<std::mutex> lk(rt::g_mu);
+  g_last_eval_ms = 0.0;
+  if (!rt::device_init()) return 1;
+  hipSetDevice(g_dev.id);
+  int depth = 0;
+  if (!check_program(code, code_len, n_consts, n_params, n_cols, &depth))
+    return 1;
+  if (n_params == 0) n_points = 0;
+  if (n_points == 0) return 0;
+  if (!points || !out) {
+    set_err("points and out must not be NULL");
+    return 1;
+  }
+  if (n_params > (uint32_t)kMaxParams) {
+    set_err("the gradient supports at most %d parameters", kMaxParams);
+    return 1;
+  }
+  if (code_len / 2 > kMaxTapeOps) {
+    set_err("the gradient supports programs of at most %u instructions",
+            kMaxTapeOps);
+    return 1;
+  }
+  if (code_len == 0 || n_obs == 0) {
+    // No term depends on the parameters: the gradient is zero.
+    memset(out, 0, (size_t)n_points * n_params * sizeof(double));
+    return 0;
+  }
+  EvalSetup S;
+  if (!setup(code, code_len, consts, n_consts, n_params, data, n_cols, n_obs,
+             points, n_points, depth, &S))
+    return 1;
+  const uint32_t ns = (n_obs + kGradSliceObs - 1) / kGradSliceObs;
+  const uint32_t chunk = (uint32_t)std::max<size_t>(
+      1, std::min<size_t>(n_points,
+                          kGradPartialBytes / ((size_t)ns * n_params * 8)));
+  DevBuf<double> d_part, d_out;
+  if (!d_part.alloc((size_t)chunk * ns * n_params, "allocating partial sums") ||
+      !d_out.alloc((size_t)n_points * n_params, "allocating the output"))
+    return 1;
+  hipStream_t st = g_dev.stream;
+  Events ev;
+  hipError_t e = hipEventRecord(ev.e0, st);
+  for (uint32_t p0 = 0; e == hipSuccess && p0 < n_points; p0 += chunk) {
+    const uint32_t np = std::min(chunk, n_points - p0);
+    Eval64Args a = S.a;
+    a.points = S.points.p + (size_t)p0 * n_params;
+    a.n_points = np;
+    a.out = d_part.p;
+    a.n_slices = ns;
+    e = launch_grad64(a, grid_for((uint64_t)ns * np), st);
+    if (e == hipSuccess)
+      e = launch_grad64_finish(d_part.p, np, n_params, ns,
+                               d_out.p + (size_t)p0 * n_params, st);
+  }
+  if (e == hipSuccess) e = hipEventRecord(ev.e1, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, d_out.p, (size_t)n_points * n_params * sizeof(double),
+                       hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    set_hip_err("gradient batch", e);
+    return 1;
+  }
+  g_last_eval_ms = elapsed_ms(ev.e0, ev.e1);
+  return 0;
+}
+
+double gpum_hip_last_eval_ms(void) { return g_last_eval_ms; }
+
+}  // extern "C"

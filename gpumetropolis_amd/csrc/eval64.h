@@ -1,0 +1,54 @@
+// eval64.h -- FP64 evaluation of reference programs at many parameter points:
+// the per-observation log-likelihood matrix (cpu_native.rs:371-391) and the
+// gradient of the summed data term by reverse-mode differentiation
+// (cpu_native.rs:148-279, 306-329).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpum {
+
+// One reference instruction with the tape positions of its operands: the
+// index of the instruction that produced each (every instruction produces
+// one value, so the instruction index is the tape index).  l / r are unused
+// by leaves, r by unary operations.
+struct TapeOp {
+  uint32_t op;
+  uint32_t arg;
+  uint32_t l;
+  uint32_t r;
+};
+
+// Observations per (point, slice) work item of the gradient kernel: 256
+// threads, 16 observations each.  It depends on nothing but the kernel, so
+// a point's gradient has the same summation order in any batch.
+constexpr uint32_t kGradSliceObs = 4096;
+// Longest program the gradient kernel's tape holds.
+constexpr uint32_t kMaxTapeOps = 1024;
+
+struct Eval64Args {
+  const TapeOp* prog;
+  uint32_t n_ops;
+  uint32_t n_params;
+  const double* consts;
+  const double* data;    // column-major: column c at data[c * n_obs]
+  uint32_t n_obs;
+  uint32_t n_points;
+  const double* points;  // n_points x n_params, point-major
+  double* out;           // pointwise: n_points x n_obs; gradient: partials
+  uint32_t n_slices;     // gradient: observation slices per point
+  uint32_t depth;        // peak stack depth of the program
+};
+
+// out[p * n_obs + i] = program at point p, observation i.
+hipError_t launch_pointwise64(const Eval64Args& a, uint32_t grid,
+                              hipStream_t st);
+// partials[(p * n_slices + s) * n_params + k]: d/d param k of the sum over
+// slice s's observations at point p.
+hipError_t launch_grad64(const Eval64Args& a, uint32_t grid, hipStream_t st);
+// out[p * n_params + k] = sum over slices, in slice order.
+hipError_t launch_grad64_finish(const double* partials, uint32_t n_points,
+                                uint32_t n_params, uint32_t n_slices,
+                                double* out, hipStream_t st);
+
+}  // namespace gpum

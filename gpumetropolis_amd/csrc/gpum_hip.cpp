@@ -623,6 +623,61 @@ int gpum_hip_lower_info(const uint32_t* code, size_t code_len,
   return 0;
 }
 
+int gpum_hip_jit_check(const uint32_t* code, size_t code_len,
+                       const double* consts, size_t n_consts,
+                       const uint32_t* prior_code, size_t prior_len,
+                       const double* prior_consts, size_t n_prior_consts,
+                       uint32_t n_params, uint32_t n_cols,
+                       int32_t chains_per_wave, uint32_t kinds,
+                       double* compile_ms) {
+  if ((code_len && !code) || (n_consts && !consts) ||
+      (prior_len && !prior_code) || (n_prior_consts && !prior_consts)) {
+    set_err("null program or constants");
+    return 1;
+  }
+  if (n_params > (uint32_t)kMaxParams) {
+    set_err("at most %d parameters", kMaxParams);
+    return 1;
+  }
+  std::string e = validate_program(prior_code, prior_len, n_prior_consts,
+                                   n_params, 0, false, nullptr);
+  if (!e.empty()) {
+    set_err("prior: %s", e.c_str());
+    return 1;
+  }
+  LowerOptions opt;
+  Lowered L;
+  e = lower_program(code, code_len, consts, n_consts, n_params, n_cols, opt, &L);
+  if (!e.empty()) {
+    set_err("log-likelihood: %s", e.c_str());
+    return 1;
+  }
+  Variant v = pick_variant(L.vm_depth, !L.empty && L.sum_mode == SUM_AFFINE);
+  const int cw = chains_per_wave > 0 ? chains_per_wave : 1;
+  if (cw != 1 && !((cw == 2 || cw == 4) && v.ns == 0)) {
+    set_err("chains_per_wave must be 1, or 2 or 4 for a program of depth 1");
+    return 1;
+  }
+  v.cw = cw;
+  const std::vector<uint32_t> prior = square_pow2(prior_code, prior_len,
+                                                  prior_consts);
+  const std::string src = jit_evaluator_source(L, prior.data(), prior.size(),
+                                               prior_consts, v.ns, v.opt, v.cw);
+  double total = 0.0;
+  for (int k = 0; k < JIT_NUM_KINDS; ++k) {
+    if (!(kinds & (1u << k))) continue;
+    std::string err;
+    double ms = 0.0;
+    if (!jit_compile_only(src, (JitKind)k, &err, &ms)) {
+      set_err("%s", err.c_str());
+      return 1;
+    }
+    total += ms;
+  }
+  if (compile_ms) *compile_ms = total;
+  return 0;
+}
+
 static int model_run_locked(gpum_hip_model* m, const double* init,
                             uint32_t n_chains, const double* proposal_sd,
                             const double* temperatures, uint32_t n_iter,

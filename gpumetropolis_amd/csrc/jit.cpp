@@ -66,9 +66,9 @@ struct Entry {
 std::mutex g_jit_mu;
 std::map<std::string, Entry> g_cache;  // by full source text
 
-bool compile(const std::string& src, Entry* out, std::string* err,
-             double* ms) {
-  auto t0 = std::chrono::steady_clock::now();
+// hiprtc's code object for a kernel source (no device involved).
+bool compile_code(const std::string& src, std::vector<char>* code,
+                  std::string* err) {
   std::vector<const char*> names, texts;
   names.push_back("hip/hip_runtime.h");
   texts.push_back(kStubRuntime);
@@ -103,9 +103,17 @@ bool compile(const std::string& src, Entry* out, std::string* err,
   }
   size_t n = 0;
   hiprtcGetCodeSize(prog, &n);
-  std::vector<char> code(n);
-  hiprtcGetCode(prog, code.data());
+  code->resize(n);
+  hiprtcGetCode(prog, code->data());
   hiprtcDestroyProgram(&prog);
+  return true;
+}
+
+bool compile(const std::string& src, Entry* out, std::string* err,
+             double* ms) {
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<char> code;
+  if (!compile_code(src, &code, err)) return false;
   hipError_t e = hipModuleLoadData(&out->mod, code.data());
   if (e == hipSuccess) e = hipModuleGetFunction(&out->fn, out->mod, "gpum_jit_kernel");
   if (e != hipSuccess) {
@@ -299,6 +307,17 @@ hipFunction_t jit_kernel(const std::string& evaluator, JitKind kind,
   if (!compile(src, &e, err, compile_ms)) return nullptr;
   g_cache[src] = e;
   return e.fn;
+}
+
+bool jit_compile_only(const std::string& evaluator, JitKind kind,
+                      std::string* err, double* compile_ms) {
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<char> code;
+  if (!compile_code(evaluator + kKernels[kind], &code, err)) return false;
+  if (compile_ms)
+    *compile_ms = std::chrono::duration<double, std::milli>(
+                      std::chrono::steady_clock::now() - t0).count();
+  return true;
 }
 
 }  // namespace gpum

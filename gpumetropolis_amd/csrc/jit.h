@@ -32,6 +32,11 @@ std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
 hipFunction_t jit_kernel(const std::string& evaluator, JitKind kind,
                          std::string* err, double* compile_ms = nullptr);
 
+// Compile the kernel of `kind` with hiprtc without loading it: needs no
+// device, only the compiler.  On failure fills *err.
+bool jit_compile_only(const std::string& evaluator, JitKind kind,
+                      std::string* err, double* compile_ms = nullptr);
+
 // Dynamic LDS a module kernel may use without an attribute call.
 constexpr size_t kJitMaxDynamicLds = 65536;
 

@@ -36,4 +36,31 @@ __host__ __device__ inline double lgamma64(double x) {
   return lgamma64_main(x);
 }
 
+// Digamma: the recurrence psi(x) = psi(x + 1) - 1/x moves the argument to
+// x >= 6, then the asymptotic series ln x - 1/(2x) - 1/(12 x^2)
+// + 1/(120 x^4) - 1/(252 x^6) (Abramowitz & Stegun 6.3.5, 6.3.18).  The
+// reference's CPU path takes the same steps (src/rust/src/cpu_native.rs:66-77)
+// with no bound on the recurrence; here arguments below -64 first use the
+// reflection psi(x) = psi(1 - x) - pi / tan(pi x), so every call ends within
+// 70 steps, and -inf gives NaN.
+__host__ __device__ inline double digamma64(double x) {
+  // Rounded operation by operation, as the reference's (no fused a*b+c).
+#pragma clang fp contract(off)
+  double result = 0.0;
+  if (x < -64.0) {
+    const double pi = 3.14159265358979323846;
+    if (x == -__builtin_inf()) return __builtin_nan("");
+    result = -pi / tan(pi * x);
+    x = 1.0 - x;
+  }
+  while (x < 6.0) {
+    result -= 1.0 / x;
+    x += 1.0;
+  }
+  const double f = 1.0 / (x * x);
+  const double inner = 1.0 / 120.0 - f * (1.0 / 252.0);
+  const double series = 1.0 / 12.0 - f * inner;
+  return result + log(x) - 1.0 / (2.0 * x) - f * series;
+}
+
 }  // namespace gpum

@@ -11,6 +11,10 @@
  *   gpum_hip_available     <- rust_available_backends src/rust/src/lib.rs:51-61
  *   gpum_hip_loglik_batch  <- loglik_batch src/rust/src/cpu_native.rs:339-360
  *                             via rust_loglik_batch src/rust/src/lib.rs:131-151
+ *   gpum_hip_loglik_pointwise <- loglik_pointwise cpu_native.rs:371-391
+ *                             via rust_loglik_pointwise lib.rs:187-207
+ *   gpum_hip_grad_batch    <- grad_batch cpu_native.rs:306-329 (tape
+ *                             :148-279) via rust_grad_batch lib.rs:158-179
  *
  * Conventions (as in the reference):
  *   - `code` is the DSL bytecode, (op, arg) u32 pairs (R/dsl.R:13-17);
@@ -170,6 +174,36 @@ int gpum_hip_loglik_batch(const uint32_t* code, size_t code_len,
                           const double* points, uint32_t n_points,
                           double* out);
 
+/* Log-likelihood of each observation at each of n_points parameter vectors,
+ * evaluated in FP64 on the device; the HIP replacement of loglik_pointwise
+ * (cpu_native.rs:371-391, FFI rust_loglik_pointwise lib.rs:187-207), which
+ * backs WAIC and PSIS-LOO.  out: n_points * n_obs doubles, out[p * n_obs + i]
+ * for observation i at point p.  As the reference, n_params = 0 means no
+ * points and an empty program evaluates to 0. */
+int gpum_hip_loglik_pointwise(const uint32_t* code, size_t code_len,
+                              const double* consts, size_t n_consts,
+                              uint32_t n_params, const double* data,
+                              uint32_t n_cols, uint32_t n_obs,
+                              const double* points, uint32_t n_points,
+                              double* out);
+
+/* Gradient of the log-likelihood (data term, summed over the observations)
+ * at each of n_points parameter vectors, by reverse-mode differentiation of
+ * the bytecode in FP64 on the device; the HIP replacement of grad_batch
+ * (cpu_native.rs:148-279, 306-329, FFI rust_grad_batch lib.rs:158-179),
+ * which backs the Cramer-Rao diagnostic.  out: n_points * n_params doubles,
+ * point-major.  At most 16 parameters and 1024 instructions. */
+int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
+                        const double* consts, size_t n_consts,
+                        uint32_t n_params, const double* data, uint32_t n_cols,
+                        uint32_t n_obs, const double* points, uint32_t n_points,
+                        double* out);
+
+/* Device time (ms) of the kernels of this thread's last
+ * gpum_hip_loglik_pointwise or gpum_hip_grad_batch call (HIP events on the
+ * library's stream; copies excluded).  No counterpart in the reference. */
+double gpum_hip_last_eval_ms(void);
+
 /* A model with its lowered program and data resident on the device, for
  * repeated batches without re-uploading the data (the reference uploads
  * every call, chain_kernel.rs:393-413).  precompute_data / fuse: -1
@@ -193,6 +227,21 @@ int gpum_hip_lower_info(const uint32_t* code, size_t code_len,
                         uint32_t n_params, uint32_t n_cols,
                         int32_t precompute_data, int32_t fuse,
                         gpum_model_info* info);
+
+/* Compile, with hiprtc and without a device, the kernels
+ * gpum_hip_model_set_jit would compile for this program and prior with
+ * chains_per_wave chains per wave (0 or 1: one).  kinds: bit k selects
+ * kernel kind k (0 chain streaming, 1 chain resident, 2 points streaming,
+ * 3 points resident, 4 batched).  compile_ms (may be NULL): total compile
+ * time.  No counterpart in the reference; it checks that a model's compiled
+ * kernels build on a host without a GPU. */
+int gpum_hip_jit_check(const uint32_t* code, size_t code_len,
+                       const double* consts, size_t n_consts,
+                       const uint32_t* prior_code, size_t prior_len,
+                       const double* prior_consts, size_t n_prior_consts,
+                       uint32_t n_params, uint32_t n_cols,
+                       int32_t chains_per_wave, uint32_t kinds,
+                       double* compile_ms);
 
 /* One batch on a resident model; arguments as gpum_hip_run_chains_ex. */
 int gpum_hip_model_run(gpum_hip_model* m, const double* init,

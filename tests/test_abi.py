@@ -23,7 +23,8 @@ def test_header_declares_entry_points():
     names = declared()
     for want in ("gpum_hip_run_chains", "gpum_hip_available",
                  "gpum_hip_last_error", "gpum_hip_result_free",
-                 "gpum_hip_loglik_batch"):
+                 "gpum_hip_loglik_batch", "gpum_hip_loglik_pointwise",
+                 "gpum_hip_grad_batch"):
         assert want in names
 
 
