@@ -76,7 +76,8 @@ def parse():
                    help="skip the parallel-tempering workload")
     p.add_argument("--pt-iters", type=int, default=4000)
     p.add_argument("--no-eval", action="store_true",
-                   help="skip the FP64 pointwise / gradient section")
+                   help="skip the FP64 pointwise / gradient and the "
+                        "synchronous DE sections")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the many-model (512 regressions) section")
     p.add_argument("--batch-models", type=int, default=512)
@@ -417,6 +418,34 @@ def eval_section(a):
     return out
 
 
+def de_sync_section(a):
+    """Synchronous Differential Evolution (path B) on the gamma model's
+    1e6 FP32 rows: 1024 chains, 200 generations, one launch sequence per
+    generation (proposal, exact stage over every CU, accept)."""
+    from gpumetropolis_amd import _native, workloads
+
+    w = workloads.gamma(n=a.n_obs)
+    m = w.model
+    C, T = 1024, 200
+    rng = np.random.default_rng(a.seed)
+    init = np.array([math.log(2.5), math.log(0.7)]) + 0.002 * rng.standard_normal((C, 2))
+    dm = _native.DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
+                             m.prior.consts, 2, w.data, 1, w.n_obs, jit=a.jit)
+    g = 2.38 / math.sqrt(4.0)
+    sd = np.full((C, 2), 1e-4)
+    dm.run_da(None, init, np.ones(C), 5, a.seed, proposal_sd=sd,
+              de_sync=(g, 1e-3))
+    r = dm.run_da(None, init, np.ones(C), T, a.seed, proposal_sd=sd,
+                  de_sync=(g, 1e-3))
+    dm.close()
+    evals = C * w.n_obs * T
+    return {"workload": f"DE path B, gamma, {w.n_obs} FP32 rows, {C} chains "
+                        f"x {T} generations",
+            "obs_evals_per_s": evals / (r.kernel_ms / 1e3),
+            "ms_per_generation": r.kernel_ms / T,
+            "mean_accept": float(np.mean(r.accept_rate))}
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -573,6 +602,7 @@ def main():
         out["batched_models"] = batch_section(a)
     if not a.no_eval:
         out["batch_evaluation"] = eval_section(a)
+        out["de_sync"] = de_sync_section(a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -63,7 +63,8 @@ class DaExt(C.Structure):
                 ("kernel_ms", C.c_double), ("exact_evals", C.c_double),
                 ("trace_prop", C.POINTER(C.c_double)),
                 ("trace_ll", C.POINTER(C.c_double)),
-                ("chains_per_wave", C.c_uint32)]
+                ("chains_per_wave", C.c_uint32),
+                ("gamma", C.c_double), ("de_noise", C.c_double)]
 
 
 class BatchModel(C.Structure):
@@ -194,6 +195,12 @@ def lib():
     L.gpum_hip_surrogate_predict.argtypes = [C.c_void_p, _f64p, C.c_uint32,
                                              _f64p]
     L.gpum_hip_surrogate_predict.restype = C.c_int
+    L.gpum_hip_run_de_sync.argtypes = [_u32p, sz, _f64p, sz, _u32p, sz, _f64p,
+                                       sz, C.c_uint32, _f64p, C.c_uint32,
+                                       C.c_uint32, _f64p, C.c_uint32, _f64p,
+                                       C.c_uint32, C.c_uint32, C.c_double,
+                                       C.c_double, C.POINTER(ChainResult)]
+    L.gpum_hip_run_de_sync.restype = C.c_int
     L.gpum_hip_model_run_da.argtypes = [C.c_void_p, C.c_void_p, _f64p,
                                         C.c_uint32, _f64p, _f64p, C.c_uint32,
                                         C.c_uint32, C.c_uint32, _f64p,
@@ -374,6 +381,27 @@ def run_chains(code, consts, prior_code, prior_consts, n_params, data, n_cols,
     return _finish(_collect(res), ext, keep, n_iter, n_chains, n_params)
 
 
+def run_de_sync(code, consts, prior_code, prior_consts, n_params, data,
+                n_cols, n_obs, init, proposal_sd, n_iter, seed, gamma,
+                de_noise) -> RunOutput:
+    """rust_gpu_metropolis_de_sync (lib.rs:215-260): synchronous
+    Differential Evolution (path B) on the HIP backend."""
+    L = lib()
+    code, prior_code = _u32(code), _u32(prior_code)
+    consts, prior_consts = _f64(consts), _f64(prior_consts)
+    data, init, sd = _f64(data), _f64(init), _f64(proposal_sd)
+    n_chains = init.size // n_params
+    res = ChainResult()
+    _check(L.gpum_hip_run_de_sync(
+        _p(code, C.c_uint32), code.size, _p(consts, C.c_double), consts.size,
+        _p(prior_code, C.c_uint32), prior_code.size,
+        _p(prior_consts, C.c_double), prior_consts.size, n_params,
+        _p(data, C.c_double), n_cols, n_obs, _p(init, C.c_double), n_chains,
+        _p(sd, C.c_double), n_iter, seed & 0xFFFFFFFF, gamma, de_noise,
+        C.byref(res)))
+    return _collect(res)
+
+
 def lower_info(code, consts, n_params, n_cols, precompute_data=None,
                fuse=None) -> dict:
     """What the kernels would run for a log-likelihood program (host-only
@@ -533,14 +561,17 @@ class DeviceModel:
     def run_da(self, surrogate, init, temperatures, n_iter, seed,
                proposal_sd=None, proposal_l=None, trace=False,
                final_state=False, stream_id=0, iter_offset=0,
-               work_items=0, work_groups=0, chains_per_wave=0) -> RunOutput:
+               work_items=0, work_groups=0, chains_per_wave=0,
+               de_sync=None) -> RunOutput:
         """Stepwise random-walk Metropolis; with `surrogate` (a Surrogate)
         two-stage delayed acceptance, without it plain Metropolis with the
-        uniforms of `run`.  Mode 2 when proposal_l is given, else mode 0."""
+        uniforms of `run`.  Mode 2 when proposal_l is given, else mode 0;
+        `de_sync=(gamma, de_noise)` runs synchronous Differential Evolution
+        (mode 1) with proposal_sd as the jitter scale."""
         np_ = self.n_params
         init, temps = _f64(init), _f64(temperatures)
         n_chains = init.size // np_
-        mode = 2 if proposal_l is not None else 0
+        mode = 2 if proposal_l is not None else (1 if de_sync else 0)
         sd = None if proposal_sd is None else _f64(proposal_sd)
         pl = None if proposal_l is None else _f64(proposal_l)
         ext = DaExt()
@@ -550,6 +581,8 @@ class DeviceModel:
         ext.work_items = work_items
         ext.work_groups = work_groups
         ext.chains_per_wave = chains_per_wave
+        if de_sync:
+            ext.gamma, ext.de_noise = float(de_sync[0]), float(de_sync[1])
         keep = {"rate": np.zeros(n_chains, np.float64)}
         ext.stage1_rate = _p(keep["rate"], C.c_double)
         if final_state:

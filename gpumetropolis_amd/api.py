@@ -20,9 +20,11 @@ at a time exactly where the reference calls ``rust_gpu_metropolis``:
 The model's data are uploaded once (a device-resident model) instead of
 once per batch.  Initial values and PT swap uniforms come from the Philox
 stream keyed on ``seed`` (the reference draws them from R's generator), so
-runs are reproducible from ``seed`` alone.  ``method="mala"`` and
-``de_sync=True`` are refused on this backend, as the reference refuses them
-on its GPU backends (R/api.R:373-377, :468-472).
+runs are reproducible from ``seed`` alone.  ``method="mala"`` is refused on
+this backend, as the reference refuses it on its GPU backends
+(R/api.R:373-377).  ``de_sync=True`` (synchronous Differential Evolution,
+which the reference runs on its CPU path only, R/api.R:464-472) runs on the
+stepwise sampler, one launch sequence per generation.
 """
 from __future__ import annotations
 
@@ -144,10 +146,11 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
             if n_chains < 4:
                 raise ValueError('`method = "de"` needs at least 4 chains.')
             if de_sync:
-                raise ValueError('`de_sync = TRUE` runs on `backend = "cpu"` '
-                                 "in this release.")
-            fit = _de(call, model, init_mat, sd_mat, gamma, de_noise, n_iter,
-                      warmup, de_every, adapt, seed, n_chains)
+                fit = _de_sync(dm, model, init_mat, sd_mat, gamma, de_noise,
+                               n_iter, warmup, seed, n_chains)
+            else:
+                fit = _de(call, model, init_mat, sd_mat, gamma, de_noise,
+                          n_iter, warmup, de_every, adapt, seed, n_chains)
         elif adapt and warmup > 0:
             fit = _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup,
                                 seed, warmup_auto, n_chains,
@@ -488,17 +491,43 @@ def _pt_batched(call, np_, n_chains, init_mat, sd_mat, temps, sizes, wb,
     return np.concatenate(draws), acc_hist, swap_hist, cur_sd, scales
 
 
-def _de(call, model, init_mat, sd_mat, gamma, de_noise, n_iter, warmup,
-        de_every, adapt, seed, n_chains) -> Fit:
-    """R/api.R:442-527 and R/adaptive.R:427-524 (path A)."""
-    np_ = model.n_params
+def _de_scales(gamma, de_noise, np_):
+    """ter Braak's default scale and the checks of R/api.R:448-460."""
     if gamma is None:
         gamma = 2.38 / math.sqrt(2 * np_)
     gamma = float(gamma)
     if not math.isfinite(gamma) or gamma <= 0:
         raise ValueError("`gamma` must be positive and finite.")
+    de_noise = float(de_noise)
     if not math.isfinite(de_noise) or de_noise < 0:
         raise ValueError("`de_noise` must be non-negative and finite.")
+    return gamma, de_noise
+
+
+def _de_sync(dm, model, init_mat, sd_mat, gamma, de_noise, n_iter, warmup,
+             seed, n_chains) -> Fit:
+    """R/api.R:464-498 (path B): one call runs every generation on the
+    device (gpum_hip_run_de_sync semantics on the resident model); the
+    warmup rows are trimmed, with no adaptation, as the reference does."""
+    np_ = model.n_params
+    gamma, de_noise = _de_scales(gamma, de_noise, np_)
+    r = dm.run_da(None, init_mat, np.ones(n_chains), n_iter, seed,
+                  proposal_sd=sd_mat, de_sync=(gamma, de_noise))
+    draws = r.draws.astype(np.float64)[warmup:]
+    return Fit(draws=draws, accept_rate=r.accept_rate, method="de",
+               n_iter=n_iter - warmup, n_iter_total=n_iter, warmup=warmup,
+               n_chains=n_chains, n_params=np_, params=list(model.params),
+               seed=seed, kernel_ms=r.kernel_ms,
+               adaptation={"de_sync": True, "gamma": gamma,
+                           "de_noise": de_noise,
+                           "compiled": bool(dm.jit)})
+
+
+def _de(call, model, init_mat, sd_mat, gamma, de_noise, n_iter, warmup,
+        de_every, adapt, seed, n_chains) -> Fit:
+    """R/api.R:442-527 and R/adaptive.R:427-524 (path A)."""
+    np_ = model.n_params
+    gamma, de_noise = _de_scales(gamma, de_noise, np_)
     if de_every is None:
         de_every = max(n_chains, 10)
     sizes, nb = ad.split_batches(n_iter, int(de_every))

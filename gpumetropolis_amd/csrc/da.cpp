@@ -421,14 +421,14 @@ int gpum_hip_surrogate_predict(gpum_hip_surrogate* S, const double* theta,
   return 0;
 }
 
-int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
-                          const double* init, uint32_t n_chains,
-                          const double* proposal_sd,
-                          const double* temperatures, uint32_t n_iter,
-                          uint32_t seed, uint32_t proposal_mode,
-                          const double* proposal_l, gpum_da_ext* ext_in,
-                          gpum_chain_result* out) {
-  std::lock_guard<std::mutex> lk(g_mu);
+// The stepwise sampler with the library lock held.
+static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
+                         const double* init, uint32_t n_chains,
+                         const double* proposal_sd,
+                         const double* temperatures, uint32_t n_iter,
+                         uint32_t seed, uint32_t proposal_mode,
+                         const double* proposal_l, gpum_da_ext* ext_in,
+                         gpum_chain_result* out) {
   if (!m || !out) {
     set_err("null model or result");
     return 1;
@@ -444,14 +444,32 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
     set_err("n_chains must be positive; init and temperatures non-NULL");
     return 1;
   }
-  if (proposal_mode != 0 && proposal_mode != 2) {
-    set_err("the stepwise sampler runs proposal_mode 0 or 2 (got %u)", proposal_mode);
+  if (proposal_mode > 2) {
+    set_err("the stepwise sampler runs proposal_mode 0, 1 or 2 (got %u)",
+            proposal_mode);
     return 1;
   }
-  if ((proposal_mode == 0 && !proposal_sd) || (proposal_mode == 2 && !proposal_l)) {
+  if ((proposal_mode != 2 && !proposal_sd) || (proposal_mode == 2 && !proposal_l)) {
     set_err("proposal_mode %u needs %s", proposal_mode,
-            proposal_mode == 0 ? "proposal_sd" : "proposal_l");
+            proposal_mode == 2 ? "proposal_l" : "proposal_sd");
     return 1;
+  }
+  if (proposal_mode == 1) {
+    // Differential Evolution, path B: the pair of other chains must exist.
+    if (C < 3) {
+      set_err("synchronous Differential Evolution needs at least 3 chains");
+      return 1;
+    }
+    if (!(X.gamma > 0.0) || !isfinite(X.gamma) || !(X.de_noise >= 0.0) ||
+        !isfinite(X.de_noise)) {
+      set_err("gamma must be positive and finite, de_noise non-negative and "
+              "finite");
+      return 1;
+    }
+    if (S) {
+      set_err("Differential Evolution runs without a surrogate");
+      return 1;
+    }
   }
   for (uint32_t c = 0; c < C; ++c) {
     if (!(temperatures[c] > 0.0) || !isfinite(temperatures[c])) {
@@ -501,7 +519,7 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
             count.alloc(1, "allocate state") &&
             draws.alloc(n_draws, "allocate draws") &&
             draws_it.alloc(n_draws, "allocate draws");
-  if (ok && proposal_mode == 0) ok = sd.upload(proposal_sd, ncp, "upload proposal_sd");
+  if (ok && proposal_mode != 2) ok = sd.upload(proposal_sd, ncp, "upload proposal_sd");
   if (ok && proposal_mode == 2)
     ok = L.upload(proposal_l, ncp * np, "upload proposal_l");
   if (ok && X.trace_s) ok = tr_s.alloc((size_t)n_iter * C, "allocate trace");
@@ -530,6 +548,8 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
   a.stream_id = X.stream_id;
   a.iter_offset = X.iter_offset;
   a.mode = proposal_mode;
+  a.gamma = X.gamma;
+  a.de_noise = X.de_noise;
   a.temperatures = temps.p;
   a.proposal_sd = sd.p;
   a.proposal_l = L.p;
@@ -721,6 +741,46 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
       o->exact_evals = evals;
   }
   return 0;
+}
+
+int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
+                          const double* init, uint32_t n_chains,
+                          const double* proposal_sd,
+                          const double* temperatures, uint32_t n_iter,
+                          uint32_t seed, uint32_t proposal_mode,
+                          const double* proposal_l, gpum_da_ext* ext_in,
+                          gpum_chain_result* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return run_da_locked(m, S, init, n_chains, proposal_sd, temperatures, n_iter,
+                       seed, proposal_mode, proposal_l, ext_in, out);
+}
+
+int gpum_hip_run_de_sync(const uint32_t* code, size_t code_len,
+                         const double* consts, size_t n_consts,
+                         const uint32_t* prior_code, size_t prior_len,
+                         const double* prior_consts, size_t n_prior_consts,
+                         uint32_t n_params, const double* data, uint32_t n_cols,
+                         uint32_t n_obs, const double* init, uint32_t n_chains,
+                         const double* proposal_sd, uint32_t n_iter,
+                         uint32_t seed, double gamma, double de_noise,
+                         gpum_chain_result* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!device_init()) return 1;
+  hipSetDevice(g_dev.id);
+  gpum_hip_model m;
+  if (!model_build(&m, code, code_len, consts, n_consts, prior_code, prior_len,
+                   prior_consts, n_prior_consts, n_params, data, n_cols, n_obs,
+                   -1, -1))
+    return 1;
+  // Path B has no temperatures: log u < plp - cur (cpu_native.rs:482).
+  std::vector<double> temps(n_chains, 1.0);
+  gpum_da_ext X;
+  memset(&X, 0, sizeof X);
+  X.struct_size = sizeof X;
+  X.gamma = gamma;
+  X.de_noise = de_noise;
+  return run_da_locked(&m, nullptr, init, n_chains, proposal_sd, temps.data(),
+                       n_iter, seed, 1, nullptr, &X, out);
 }
 
 }  // extern "C"

@@ -14,7 +14,11 @@ struct DaArgs {
   uint32_t seed;
   uint32_t stream_id;
   uint32_t iter_offset;
-  uint32_t mode;  // 0: diagonal sd, 2: Cholesky factor per chain
+  // 0: diagonal sd; 1: Differential Evolution against the current
+  // population, the pair excluding the proposing chain (the synchronous
+  // path B, cpu_native.rs:400-506); 2: Cholesky factor per chain.
+  uint32_t mode;
+  double gamma, de_noise;  // mode 1
   const double* temperatures;
   const double* proposal_sd;
   const double* proposal_l;

@@ -94,15 +94,41 @@ __global__ void __launch_bounds__(256) da_propose_kernel(const DaArgs* __restric
   const bool mine = live && (uint32_t)lane < np;  // lane j: parameter j
   if (live) {
     const double two_pi = 6.283185307179586;
+    StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
+    // Slots of the normal deviates: after a, b and the scale choice in
+    // mode 1 (cpu_native.rs:455-471).
+    const uint32_t zo = A->mode == 1 ? 3u : 0u;
     double z = 0.0;
     if (mine) {
-      StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
-      const double u1 = us.get(2 * lane), u2 = us.get(2 * lane + 1);
+      const double u1 = us.get(zo + 2 * lane), u2 = us.get(zo + 2 * lane + 1);
       z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
     }
     const double* th = A->theta + (size_t)c * np;
     double p = 0.0;
-    if (A->mode == 2) {
+    if (A->mode == 1) {
+      // Rounded operation by operation as the reference (no fused
+      // multiply-add): each generation's states are sums of the previous
+      // generation's differences, so a last-place difference would grow
+      // from generation to generation.
+#pragma clang fp contract(off)
+      // The pair (a, b) from the current generation, a != c, b not in
+      // {c, a} (the host requires at least 3 chains); with probability 0.1
+      // the scale is 1.  Every lane of the wave draws the same pair.
+      const uint32_t C = A->n_chains;
+      uint32_t ia = (uint32_t)(us.get(0) * (double)C);
+      if (ia >= C) ia = C - 1;
+      if (ia == c) ia = (ia + 1) % C;
+      uint32_t ib = (uint32_t)(us.get(1) * (double)C);
+      if (ib >= C) ib = C - 1;
+      while (ib == c || ib == ia) ib = (ib + 1) % C;
+      const double g = us.get(2) < 0.1 ? 1.0 : A->gamma;
+      if (mine) {
+        const double diff = A->theta[(size_t)ia * np + lane] -
+                            A->theta[(size_t)ib * np + lane];
+        p = th[lane] + g * diff +
+            A->de_noise * A->proposal_sd[(size_t)c * np + lane] * z;
+      }
+    } else if (A->mode == 2) {
       // pr[k] = th[k] + sum_{j <= k} L[k][j] z[j], j ascending as the
       // reference sums it.
       const double* L = A->proposal_l + (size_t)c * np * np;
@@ -202,7 +228,8 @@ __global__ void da_stage2_kernel(const DaArgs* __restrict__ A, uint32_t t) {
     double la = lpp - A->cur[c];
     if (A->use_surrogate) la -= A->prop_s[c] - A->cur_s[c];
     StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
-    const double u = us.get(2 * np);
+    // The last slot of the iteration: 2 np + 1 draws, 2 np + 4 in mode 1.
+    const double u = us.get(A->mode == 1 ? 2 * np + 3 : 2 * np);
     acc_ok = log(u) < la / A->temperatures[c];
   }
   double* th = A->theta + (size_t)c * np;

@@ -124,7 +124,8 @@ uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
 }  // namespace rt
 }  // namespace gpum
 
-namespace {
+namespace gpum {
+namespace rt {
 
 bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
                  const double* consts, size_t n_consts,
@@ -228,7 +229,8 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
   return true;
 }
 
-}  // namespace
+}  // namespace rt
+}  // namespace gpum
 
 namespace gpum {
 namespace rt {

@@ -135,6 +135,13 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
                  const TilingForce* force = nullptr, int cw_req = 0);
 // Lay the model's columns out on the device with column stride n_pad.
 bool ensure_data(gpum_hip_model* m, uint32_t n_pad);
+// Validate, lower and upload a model (precompute / fuse: -1 default, 0, 1).
+bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
+                 const double* consts, size_t n_consts,
+                 const uint32_t* prior_code, size_t prior_len,
+                 const double* prior_consts, size_t n_prior_consts,
+                 uint32_t n_params, const double* data, uint32_t n_cols,
+                 uint32_t n_obs, int32_t precompute, int32_t fuse);
 
 // The summation fields of a launch's hot values (the tail is uploaded
 // after the program's padding instruction).

@@ -341,10 +341,16 @@ typedef struct gpum_da_ext {
   double* trace_prop;       /* out: n_iter * n_chains * n_params proposals */
   double* trace_ll;         /* out: n_iter * n_chains data term (NaN: not run) */
   uint32_t chains_per_wave; /* exact-stage slots per wave (0 auto, 1 or 2) */
+  double gamma;             /* proposal_mode 1: difference scale (> 0) */
+  double de_noise;          /* proposal_mode 1: jitter scale on proposal_sd */
 } gpum_da_ext;
 
-/* Random-walk Metropolis (proposal_mode 0 or 2) run one iteration per
- * launch sequence.  With a surrogate (s non-NULL, n_in == n_params), each
+/* Random-walk Metropolis (proposal_mode 0 or 2) or synchronous
+ * Differential Evolution (proposal_mode 1, ext->gamma / ext->de_noise, at
+ * least 3 chains, no surrogate: path B of cpu_native.rs:400-506, every
+ * chain proposing against the previous generation's whole population) run
+ * one iteration per launch sequence; the launch boundary is the
+ * generation barrier.  With a surrogate (s non-NULL, n_in == n_params), each
  * proposal first faces the surrogate's Metropolis test and only survivors
  * are evaluated exactly; the exact test divides out the surrogate ratio,
  * so the target is unchanged (two-stage delayed acceptance).  With
@@ -357,6 +363,25 @@ int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* s,
                           uint32_t seed, uint32_t proposal_mode,
                           const double* proposal_l, gpum_da_ext* ext,
                           gpum_chain_result* out);
+
+/* Synchronous Differential Evolution MCMC (path B), the HIP replacement of
+ * run_de_sync (cpu_native.rs:400-506, FFI rust_gpu_metropolis_de_sync
+ * lib.rs:215-260), which the reference runs on its CPU path only
+ * (R/api.R:464-472).  Each generation every chain proposes
+ * x + g (x_a - x_b) + de_noise * proposal_sd * z against the previous
+ * generation, a != c and b not in {c, a}, g = 1 with probability 0.1 else
+ * gamma, and accepts when log u < plp - cur.  Uniforms: the Philox stream
+ * of gpum_hip_run_chains, 2 n_params + 4 slots per iteration in the
+ * reference's order.  n_chains >= 3.  Result layout as gpum_hip_run_chains. */
+int gpum_hip_run_de_sync(const uint32_t* code, size_t code_len,
+                         const double* consts, size_t n_consts,
+                         const uint32_t* prior_code, size_t prior_len,
+                         const double* prior_consts, size_t n_prior_consts,
+                         uint32_t n_params, const double* data, uint32_t n_cols,
+                         uint32_t n_obs, const double* init, uint32_t n_chains,
+                         const double* proposal_sd, uint32_t n_iter,
+                         uint32_t seed, double gamma, double de_noise,
+                         gpum_chain_result* out);
 
 /* ---- Many models in one run -------------------------------------------
  * No single-call counterpart in the reference, whose host runs one model

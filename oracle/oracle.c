@@ -649,7 +649,13 @@ int orc_run_de_sync(const orc_run_args* a, orc_run_out* o) {
       double plp = log_post(a, prop);
       double u = uniform_at(a, c, t, s++, spi);
       double* nx = next_pop + (size_t)c * np;
-      if (log(u) < plp - cur_lp[c]) {
+      const int acc = log(u) < plp - cur_lp[c];
+      if (o->trace_plp) o->trace_plp[(size_t)t * C + c] = plp;
+      if (o->trace_prop)
+        for (uint32_t j = 0; j < np; ++j)
+          o->trace_prop[((size_t)t * C + c) * np + j] = prop[j];
+      if (o->trace_accept) o->trace_accept[(size_t)t * C + c] = (uint8_t)acc;
+      if (acc) {
         memcpy(nx, prop, np * sizeof(double));
         cur_lp[c] = plp;
         ++accepts[c];

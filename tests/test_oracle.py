@@ -265,6 +265,28 @@ def test_nonfinite_rejected_oracle(orc):
     assert np.all(np.isfinite(d)) and d.min() > 0
 
 
+def test_de_sync_pair_is_the_two_other_chains(orc):
+    """Path B with 3 chains and no jitter (cpu_native.rs:455-471): a != c
+    and b not in {c, a} leave exactly the two other chains, so each
+    proposal step is +-g (x_o1 - x_o2) of the previous generation, with g
+    1 or gamma.  A pair drawn with self, or path A's rule, breaks it."""
+    m, y = _gm(n=200)
+    C, T, gam = 3, 40, 0.7
+    init = np.array([[2.0], [3.1], [4.5]])
+    r = orc.run(m.loglik.code, m.loglik.consts, [], [], 1, y, 1, y.size, init,
+                np.full((C, 1), 0.1), np.ones(C), T, 9, proposal_mode=1,
+                gamma=gam, de_noise=0.0, de_sync=True, trace=True)
+    prev = init[:, 0].copy()
+    for t in range(T):
+        for c in range(C):
+            o1, o2 = [k for k in range(C) if k != c]
+            step = abs(r.trace_prop[t, c, 0] - prev[c])
+            d = abs(prev[o1] - prev[o2])
+            assert min(abs(step - d), abs(step - gam * d)) <= 1e-12 * (1 + d), (t, c)
+        # the generation's states: accepted proposals, else unchanged
+        prev = np.where(r.trace_accept[t], r.trace_prop[t, :, 0], prev)
+
+
 def test_de_sync_excludes_self(orc):
     m, y = _gm(n=500)
     C = 6
