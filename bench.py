@@ -76,8 +76,8 @@ def parse():
                    help="skip the parallel-tempering workload")
     p.add_argument("--pt-iters", type=int, default=4000)
     p.add_argument("--no-eval", action="store_true",
-                   help="skip the FP64 pointwise / gradient and the "
-                        "synchronous DE sections")
+                   help="skip the FP64 pointwise / gradient, synchronous "
+                        "DE and MALA sections")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the many-model (512 regressions) section")
     p.add_argument("--batch-models", type=int, default=512)
@@ -446,6 +446,43 @@ def de_sync_section(a):
             "mean_accept": float(np.mean(r.accept_rate))}
 
 
+def mala_section(a):
+    """MALA (Langevin proposals with the FP64 value-and-gradient kernel) on
+    the 16-dim logistic regression: 1e5 FP32 rows, 256 chains, 50
+    iterations, Cholesky preconditioner; the oracle times one chain's
+    iterations on one host thread."""
+    from gpumetropolis_amd import _native, workloads
+    from oracle import oracle
+
+    w = workloads.logistic(n=100_000, n_chains=256)
+    m = w.model
+    d, C, T = m.n_params, 256, 50
+    rng = np.random.default_rng(a.seed)
+    init = w.meta["true"] + 0.003 * rng.standard_normal((C, d))
+    sd = np.full((C, d), 0.004)
+    L = np.tile(0.004 * np.eye(d), (C, 1, 1)).reshape(-1)
+    args = (m.loglik.code, m.loglik.consts, m.prior.code, m.prior.consts, d,
+            w.data, w.n_cols, w.n_obs)
+    _native.run_chains(*args, init, sd, np.ones(C), 2, a.seed, proposal_mode=3,
+                       proposal_l=L)
+    dm = _native.DeviceModel(*args)
+    r = dm.run_da(None, init, np.ones(C), T, a.seed, proposal_sd=sd,
+                  proposal_l=L, langevin=True)
+    dm.close()
+    grads = C * w.n_obs * T
+    t0 = time.perf_counter()
+    oracle.run(*args, init[:1], sd[:1], np.ones(1), 2, a.seed, proposal_mode=3,
+               proposal_l=L[:d * d], n_threads=1)
+    cpu = w.n_obs * 3 / (time.perf_counter() - t0)  # 2 iterations + the start
+    return {"workload": f"MALA, logistic-16, {w.n_obs} FP32 rows, {C} chains "
+                        f"x {T} iterations, Cholesky preconditioner, FP64 "
+                        "value and gradient",
+            "obs_grads_per_s": grads / (r.kernel_ms / 1e3),
+            "ms_per_iteration": r.kernel_ms / T,
+            "mean_accept": float(np.mean(r.accept_rate)),
+            "cpu_oracle_obs_grads_per_s": cpu, "cpu_cores": 1}
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -603,6 +640,7 @@ def main():
     if not a.no_eval:
         out["batch_evaluation"] = eval_section(a)
         out["de_sync"] = de_sync_section(a)
+        out["mala"] = mala_section(a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -562,16 +562,20 @@ class DeviceModel:
                proposal_sd=None, proposal_l=None, trace=False,
                final_state=False, stream_id=0, iter_offset=0,
                work_items=0, work_groups=0, chains_per_wave=0,
-               de_sync=None) -> RunOutput:
+               de_sync=None, langevin=False) -> RunOutput:
         """Stepwise random-walk Metropolis; with `surrogate` (a Surrogate)
         two-stage delayed acceptance, without it plain Metropolis with the
         uniforms of `run`.  Mode 2 when proposal_l is given, else mode 0;
         `de_sync=(gamma, de_noise)` runs synchronous Differential Evolution
-        (mode 1) with proposal_sd as the jitter scale."""
+        (mode 1) with proposal_sd as the jitter scale; `langevin` runs MALA
+        (mode 3) preconditioned by proposal_l when given, else proposal_sd."""
         np_ = self.n_params
         init, temps = _f64(init), _f64(temperatures)
         n_chains = init.size // np_
-        mode = 2 if proposal_l is not None else (1 if de_sync else 0)
+        if langevin:
+            mode = 3
+        else:
+            mode = 2 if proposal_l is not None else (1 if de_sync else 0)
         sd = None if proposal_sd is None else _f64(proposal_sd)
         pl = None if proposal_l is None else _f64(proposal_l)
         ext = DaExt()

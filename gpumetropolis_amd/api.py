@@ -20,11 +20,11 @@ at a time exactly where the reference calls ``rust_gpu_metropolis``:
 The model's data are uploaded once (a device-resident model) instead of
 once per batch.  Initial values and PT swap uniforms come from the Philox
 stream keyed on ``seed`` (the reference draws them from R's generator), so
-runs are reproducible from ``seed`` alone.  ``method="mala"`` is refused on
-this backend, as the reference refuses it on its GPU backends
-(R/api.R:373-377).  ``de_sync=True`` (synchronous Differential Evolution,
-which the reference runs on its CPU path only, R/api.R:464-472) runs on the
-stepwise sampler, one launch sequence per generation.
+runs are reproducible from ``seed`` alone.  ``method="mala"`` and
+``de_sync=True`` (synchronous Differential Evolution), which the reference
+runs on its CPU path only (R/api.R:373-377, :464-472), run here on the
+stepwise sampler: MALA with its log-posterior and gradient in FP64, path B
+one launch sequence per generation.
 """
 from __future__ import annotations
 
@@ -100,9 +100,6 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                          "package is the hip backend")
     if method not in ("rwm", "pt", "de", "mala"):
         raise ValueError(f"unknown method '{method}'")
-    if method == "mala":
-        raise ValueError('`method = "mala"` runs on `backend = "cpu"` in this '
-                         "release; the hip backend implements rwm, pt and de.")
     X = _data_matrix(model, data)
     n_obs = X.shape[0]
     np_ = model.n_params
@@ -132,8 +129,8 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
         raise ValueError(f"unknown engine '{engine}'")
     if engine == "auto":
         engine = pick_engine(dm.info(), n_chains, n_obs) if method == "rwm" else "fused"
-    if engine == "stepwise" and method != "rwm":
-        raise ValueError('`engine = "stepwise"` runs method "rwm" only.')
+    if engine == "stepwise" and method not in ("rwm", "mala"):
+        raise ValueError('`engine = "stepwise"` runs methods "rwm" and "mala".')
     call = (_StepCaller(dm, n_chains) if engine == "stepwise"
             else _Caller(dm, n_chains, tiling))
 
@@ -151,18 +148,23 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
             else:
                 fit = _de(call, model, init_mat, sd_mat, gamma, de_noise,
                           n_iter, warmup, de_every, adapt, seed, n_chains)
+        elif surrogate and method == "mala":
+            raise ValueError("`surrogate` runs with the random-walk proposal "
+                             '(method = "rwm").')
         elif adapt and warmup > 0:
             fit = _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup,
                                 seed, warmup_auto, n_chains,
-                                surrogate=surrogate, dm=dm)
+                                surrogate=surrogate, dm=dm,
+                                langevin=method == "mala")
         elif surrogate:
             raise ValueError("`surrogate` needs the adaptive warmup "
                              "(adapt = TRUE and warmup > 0) for its training "
                              "pairs.")
         else:
-            r = call(init_mat, sd_mat, n_iter, seed)
+            r = call(init_mat, sd_mat, n_iter, seed,
+                     mode=3 if method == "mala" else 0)
             d = r.draws[warmup:].astype(np.float64)
-            fit = Fit(d, r.accept_rate.astype(np.float64), "rwm", d.shape[0],
+            fit = Fit(d, r.accept_rate.astype(np.float64), method, d.shape[0],
                       n_iter, warmup, n_chains, np_, model.params, seed)
     finally:
         dm.close()
@@ -220,8 +222,8 @@ class _Caller:
 
 
 class _StepCaller:
-    """The `_Caller` interface on the stepwise sampler (proposal modes 0 and
-    2): one launch sequence per iteration, observations split over every
+    """The `_Caller` interface on the stepwise sampler (proposal modes 0, 2
+    and 3): one launch sequence per iteration, observations split over every
     CU."""
 
     def __init__(self, dm, n_chains):
@@ -230,14 +232,15 @@ class _StepCaller:
 
     def __call__(self, init_m, sd_m, iters, s, temps=None, mode=0, g=0.0,
                  noise=0.0, L=None, iter_offset=0, trace=False):
-        if mode not in (0, 2):
-            raise ValueError("the stepwise engine runs proposal modes 0 and 2")
+        if mode not in (0, 2, 3):
+            raise ValueError("the stepwise engine runs proposal modes 0, 2 "
+                             "and 3")
         t = np.ones(self.n_chains) if temps is None else temps
         r = self.dm.run_da(None, init_m, t, iters, int(s) & 0xFFFFFFFF,
-                           proposal_sd=sd_m if mode == 0 else None,
-                           proposal_l=L if mode == 2 else None,
+                           proposal_sd=sd_m if mode != 2 else None,
+                           proposal_l=L if mode != 0 else None,
                            final_state=True, iter_offset=iter_offset,
-                           trace=trace)
+                           trace=trace, langevin=mode == 3)
         self.kernel_ms += r.kernel_ms
         return r
 
@@ -257,16 +260,19 @@ class WarmupResult:
 
 
 def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
-                    auto_stop=False, collect=False) -> WarmupResult:
+                    auto_stop=False, collect=False,
+                    langevin=False) -> WarmupResult:
     """.am_orchestrate_warmup (R/adaptive.R:101-254): per-chain and pooled
     Welford, Robbins-Monro scale per chain, mid-warmup reset, per-chain
     L = scale * chol(pooled covariance), optional early freeze.  With
     `collect`, the batches after the mid-warmup reset also return every
-    proposal with its exact log-likelihood (surrogate training pairs)."""
+    proposal with its exact log-likelihood (surrogate training pairs).
+    `langevin`: MALA batches (mode 3, the factor as preconditioner), scale
+    1.65 / d^(1/6) and target acceptance 0.574 (R/adaptive.R:128, 147)."""
     np_ = n_params
     nb = len(sizes)
     states = ad.WelfordState.init(n_chains, np_, sd_mat ** 2)
-    scale0 = 2.38 / math.sqrt(np_)
+    scale0 = 1.65 / np_ ** (1.0 / 6.0) if langevin else 2.38 / math.sqrt(np_)
     scales = np.full(n_chains, scale0)
     cur_init = init_mat.copy()
     cur_sd = sd_mat.copy()
@@ -274,7 +280,7 @@ def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
     cur_L = None
     L_pool = None
     pool = ad.PooledWelford.init(np_, sd_mat[0] ** 2)
-    target = ad.target_accept(np_)
+    target = 0.574 if langevin else ad.target_accept(np_)
     reset_batch = nb // 2
     prev_scales = scales.copy()
     streak = 0
@@ -284,9 +290,9 @@ def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
     for b in range(1, nb + 1):
         use_L = np_ > 1 and cur_L is not None
         rec = collect and b > reset_batch
+        pm = 3 if langevin else (2 if use_L else 0)
         r = call(cur_init, cur_sd, sizes[b - 1], seed + b - 1,
-                 mode=2 if use_L else 0, L=cur_L if use_L else None,
-                 trace=rec)
+                 mode=pm, L=cur_L if use_L else None, trace=rec)
         if rec:
             pth.append(r.trace_prop.reshape(-1, np_))
             pll.append(r.trace_ll.reshape(-1))
@@ -330,13 +336,15 @@ def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
 
 
 def _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup, seed,
-                  auto_stop, n_chains, surrogate=None, dm=None) -> Fit:
+                  auto_stop, n_chains, surrogate=None, dm=None,
+                  langevin=False) -> Fit:
     """R/adaptive.R:101-254 then R/api.R:537-571; with `surrogate` the
-    sampling batch is two-stage delayed acceptance."""
+    sampling batch is two-stage delayed acceptance; with `langevin` every
+    batch is MALA (mode 3) with the warmup's factor as preconditioner."""
     np_ = model.n_params
     w = adaptive_warmup(call, np_, n_chains, init_mat, sd_mat,
                         ad.warmup_batches(warmup), seed, auto_stop,
-                        collect=bool(surrogate))
+                        collect=bool(surrogate), langevin=langevin)
     sampling = max(1, n_iter - w.used)
     use_L = w.L is not None
     adaptation = {"final_proposal_sd": w.sd, "final_scales": w.scales,
@@ -375,9 +383,11 @@ def _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup, seed,
         net.close()
     else:
         r = call(w.init, w.sd, sampling, seed + w.batches_run,
-                 mode=2 if use_L else 0, L=w.L if use_L else None)
+                 mode=3 if langevin else (2 if use_L else 0),
+                 L=w.L if use_L else None)
     d = r.draws.astype(np.float64)
-    return Fit(d, r.accept_rate.astype(np.float64), "rwm", d.shape[0],
+    return Fit(d, r.accept_rate.astype(np.float64),
+               "mala" if langevin else "rwm", d.shape[0],
                w.used + d.shape[0], w.used, n_chains, np_, model.params, seed,
                adaptation=adaptation)
 

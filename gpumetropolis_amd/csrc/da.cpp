@@ -421,6 +421,188 @@ int gpum_hip_surrogate_predict(gpum_hip_surrogate* S, const double* theta,
   return 0;
 }
 
+// Langevin steps (proposal_mode 3, cpu_native.rs:573-724), with the
+// library lock held: the log-posterior and its gradient come from the FP64
+// tape of the reference programs over the model's columns; every
+// iteration draws the proposals, evaluates value and gradient at all of
+// them (observations split over every CU) and decides.
+static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
+                           const double* proposal_sd,
+                           const double* temperatures, uint32_t n_iter,
+                           uint32_t seed, const double* proposal_l,
+                           const gpum_da_ext& X, gpum_da_ext* ext_in,
+                           gpum_chain_result* out) {
+  const uint32_t np = m->n_params;
+  if (m->code_ref.size() / 2 > kMaxTapeOps) {
+    set_err("Langevin proposals differentiate programs of at most %u "
+            "instructions", kMaxTapeOps);
+    return 1;
+  }
+  if (m->prior_ref.size() / 2 > kMalaMaxPriorOps) {
+    set_err("Langevin proposals differentiate priors of at most %u "
+            "instructions", kMalaMaxPriorOps);
+    return 1;
+  }
+  if (!ensure_eval64(m)) return 1;
+  const uint32_t n_ops = (uint32_t)(m->code_ref.size() / 2);
+  const uint32_t ns = (m->n_obs + kGradSliceObs - 1) / kGradSliceObs;
+  const size_t ncp = (size_t)C * np;
+  const size_t n_draws = (size_t)n_iter * ncp;
+  DevBuf<double> theta, prop, cur, acc, passes, temps, sd, L, gx, gy, mean_x,
+      ll, part;
+  DevBuf<float> draws, draws_it;
+  DevBuf<double> tr_plp, tr_prop, tr_ll;
+  DevBuf<uint8_t> tr_stage;
+  DevBuf<DaArgs> d_args;
+  bool ok = theta.upload(init, ncp, "upload init") &&
+            prop.alloc(ncp, "allocate state") && cur.alloc(C, "allocate state") &&
+            acc.alloc(C, "allocate state") && passes.alloc(C, "allocate state") &&
+            gx.alloc(ncp, "allocate state") && gy.alloc(ncp, "allocate state") &&
+            mean_x.alloc(ncp, "allocate state") && ll.alloc(C, "allocate state") &&
+            part.alloc(std::max<size_t>(1, (size_t)C * ns * (np + 1)),
+                       "allocate partial sums") &&
+            temps.upload(temperatures, C, "upload temperatures") &&
+            sd.upload(proposal_sd, ncp, "upload proposal_sd") &&
+            draws.alloc(n_draws, "allocate draws") &&
+            draws_it.alloc(n_draws, "allocate draws");
+  if (ok && proposal_l) ok = L.upload(proposal_l, ncp * np, "upload proposal_l");
+  if (ok && X.trace_plp) ok = tr_plp.alloc((size_t)n_iter * C, "allocate trace");
+  if (ok && X.trace_stage) ok = tr_stage.alloc((size_t)n_iter * C, "allocate trace");
+  if (ok && X.trace_prop) ok = tr_prop.alloc(n_draws, "allocate trace");
+  if (ok && X.trace_ll) ok = tr_ll.alloc((size_t)n_iter * C, "allocate trace");
+  if (!ok) return 1;
+  hipStream_t st = g_dev.stream;
+  if (hipMemsetAsync(acc.p, 0, C * sizeof(double), st) != hipSuccess ||
+      hipMemsetAsync(passes.p, 0, C * sizeof(double), st) != hipSuccess) {
+    set_err("clear counters: device error");
+    return 1;
+  }
+  DaArgs a;
+  memset(&a, 0, sizeof a);
+  a.n_chains = C;
+  a.n_params = np;
+  a.n_iter = n_iter;
+  a.seed = seed;
+  a.stream_id = X.stream_id;
+  a.iter_offset = X.iter_offset;
+  a.mode = 3;
+  a.temperatures = temps.p;
+  a.proposal_sd = sd.p;
+  a.proposal_l = proposal_l ? L.p : nullptr;
+  a.prior_consts = m->prior_consts.p;
+  a.theta = theta.p;
+  a.prop = prop.p;
+  a.cur = cur.p;
+  a.accepts = acc.p;
+  a.passes = passes.p;
+  a.gx = gx.p;
+  a.gy = gy.p;
+  a.mean_x = mean_x.p;
+  a.ll_prop = ll.p;
+  a.prior_tape = m->tape_prior.p;
+  a.n_prior_ops = (uint32_t)(m->prior_ref.size() / 2);
+  a.draws = draws.p;
+  a.draws_it = draws_it.p;
+  a.trace_plp = X.trace_plp ? tr_plp.p : nullptr;
+  a.trace_stage = X.trace_stage ? tr_stage.p : nullptr;
+  a.trace_prop = X.trace_prop ? tr_prop.p : nullptr;
+  a.trace_ll = X.trace_ll ? tr_ll.p : nullptr;
+  if (!d_args.upload(&a, 1, "upload launch arguments")) return 1;
+
+  // Value and gradient of the data term at C points (the states or the
+  // proposals) into ll / g.
+  Eval64Args ea{};
+  ea.prog = m->tape_ll.p;
+  ea.n_ops = n_ops;
+  ea.n_params = np;
+  ea.consts = m->consts64.p;
+  ea.data = m->data64.p;
+  ea.n_obs = m->n_obs;
+  ea.n_points = C;
+  ea.out = part.p;
+  ea.n_slices = ns;
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)ns * C, (uint64_t)g_dev.n_cu * 8u));
+  auto data_term = [&](const double* points, double* g) -> hipError_t {
+    if (n_ops == 0) {
+      hipError_t r = hipMemsetAsync(ll.p, 0, C * sizeof(double), st);
+      return r == hipSuccess ? hipMemsetAsync(g, 0, ncp * sizeof(double), st) : r;
+    }
+    Eval64Args e = ea;
+    e.points = points;
+    hipError_t r = launch_grad64(e, grid, st);
+    return r == hipSuccess ? launch_grad64_finish(part.p, C, np, ns, g, ll.p, st)
+                           : r;
+  };
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  chk(data_term(theta.p, gx.p));
+  chk(launch_mala_init(d_args.p, C, st));
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, st);
+  for (uint32_t it = 0; it < n_iter && e == hipSuccess; ++it) {
+    chk(launch_mala_propose(d_args.p, C, it, st));
+    chk(data_term(prop.p, gy.p));
+    chk(launch_mala_accept(d_args.p, C, it, st));
+  }
+  if (e == hipSuccess) e = launch_da_draws(draws_it.p, draws.p, n_iter, C * np, st);
+  hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  float ms = 0.0f;
+  if (e == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (e != hipSuccess) {
+    set_hip_err("Langevin sampler", e);
+    return 1;
+  }
+  std::vector<double> h_acc(C);
+  out->n_iter = n_iter;
+  out->n_chains = C;
+  out->n_params = np;
+  out->draws = (float*)malloc(std::max<size_t>(1, n_draws) * sizeof(float));
+  out->accept_rate = (float*)malloc(C * sizeof(float));
+  out->last_log_post = (double*)malloc(C * sizeof(double));
+  if (!out->draws || !out->accept_rate || !out->last_log_post) {
+    gpum_hip_result_free(out);
+    set_err("host out of memory for results");
+    return 1;
+  }
+  auto d2h = [](void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+  };
+  ok = d2h(out->draws, draws.p, n_draws * sizeof(float)) &&
+       d2h(out->last_log_post, cur.p, C * sizeof(double)) &&
+       d2h(h_acc.data(), acc.p, C * sizeof(double));
+  if (ok && X.final_state) ok = d2h(X.final_state, theta.p, ncp * sizeof(double));
+  if (ok && X.trace_plp)
+    ok = d2h(X.trace_plp, tr_plp.p, (size_t)n_iter * C * sizeof(double));
+  if (ok && X.trace_stage) ok = d2h(X.trace_stage, tr_stage.p, (size_t)n_iter * C);
+  if (ok && X.trace_prop) ok = d2h(X.trace_prop, tr_prop.p, n_draws * sizeof(double));
+  if (ok && X.trace_ll)
+    ok = d2h(X.trace_ll, tr_ll.p, (size_t)n_iter * C * sizeof(double));
+  if (!ok) {
+    gpum_hip_result_free(out);
+    set_err("read results: device error");
+    return 1;
+  }
+  for (uint32_t c = 0; c < C; ++c) {
+    out->accept_rate[c] = n_iter == 0 ? 0.0f : (float)h_acc[c] / (float)n_iter;
+    if (X.stage1_rate) X.stage1_rate[c] = n_iter == 0 ? 0.0 : 1.0;
+  }
+  if (ext_in) {
+    if (ext_in->struct_size >= offsetof(gpum_da_ext, kernel_ms) + sizeof(double))
+      ext_in->kernel_ms = ms;
+    if (ext_in->struct_size >= offsetof(gpum_da_ext, exact_evals) + sizeof(double))
+      ext_in->exact_evals = (double)n_iter * C;
+  }
+  return 0;
+}
+
 // The stepwise sampler with the library lock held.
 static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
                          const double* init, uint32_t n_chains,
@@ -444,8 +626,8 @@ static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
     set_err("n_chains must be positive; init and temperatures non-NULL");
     return 1;
   }
-  if (proposal_mode > 2) {
-    set_err("the stepwise sampler runs proposal_mode 0, 1 or 2 (got %u)",
+  if (proposal_mode > 3) {
+    set_err("the stepwise sampler runs proposal_mode 0, 1, 2 or 3 (got %u)",
             proposal_mode);
     return 1;
   }
@@ -453,6 +635,21 @@ static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
     set_err("proposal_mode %u needs %s", proposal_mode,
             proposal_mode == 2 ? "proposal_l" : "proposal_sd");
     return 1;
+  }
+  if (proposal_mode == 3) {
+    // Langevin: preconditioner proposal_l when given, else proposal_sd.
+    if (S) {
+      set_err("Langevin proposals run without a surrogate");
+      return 1;
+    }
+    for (uint32_t c = 0; c < C; ++c) {
+      if (!(temperatures[c] > 0.0) || !isfinite(temperatures[c])) {
+        set_err("temperatures must be positive and finite");
+        return 1;
+      }
+    }
+    return run_mala_locked(m, init, C, proposal_sd, temperatures, n_iter, seed,
+                           proposal_l, X, ext_in, out);
   }
   if (proposal_mode == 1) {
     // Differential Evolution, path B: the pair of other chains must exist.
@@ -742,6 +939,54 @@ static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
   }
   return 0;
 }
+
+}  // extern "C"
+
+namespace gpum {
+namespace rt {
+
+int run_langevin(gpum_hip_model* m, const double* init, uint32_t n_chains,
+                 const double* proposal_sd, const double* temperatures,
+                 uint32_t n_iter, uint32_t seed, const double* proposal_l,
+                 gpum_run_ext* ext, gpum_chain_result* out) {
+  gpum_da_ext X;
+  memset(&X, 0, sizeof X);
+  X.struct_size = sizeof X;
+  std::vector<uint8_t> stage;
+  auto has = [&](size_t end) { return ext && ext->struct_size >= end; };
+  if (ext) {
+    if (has(offsetof(gpum_run_ext, swap_every) + 4) && ext->swap_every > 0) {
+      set_err("proposal_mode 3 runs without fused batches (swap_every = 0)");
+      return 1;
+    }
+    X.stream_id = ext->stream_id;
+    X.final_state = ext->final_state;
+    X.trace_plp = ext->trace_plp;
+    X.trace_prop = ext->trace_prop;
+    if (has(offsetof(gpum_run_ext, iter_offset) + 4)) X.iter_offset = ext->iter_offset;
+    if (has(offsetof(gpum_run_ext, trace_ll) + sizeof(double*)))
+      X.trace_ll = ext->trace_ll;
+    if (ext->trace_accept) {
+      stage.assign((size_t)n_iter * n_chains, 0);
+      X.trace_stage = stage.data();
+    }
+  }
+  const int rc = run_da_locked(m, nullptr, init, n_chains, proposal_sd,
+                               temperatures, n_iter, seed, 3, proposal_l, &X,
+                               out);
+  if (rc == 0 && ext) {
+    if (has(offsetof(gpum_run_ext, kernel_ms) + sizeof(double)))
+      ext->kernel_ms = X.kernel_ms;
+    for (size_t k = 0; k < stage.size(); ++k)
+      ext->trace_accept[k] = stage[k] == 2 ? 1 : 0;
+  }
+  return rc;
+}
+
+}  // namespace rt
+}  // namespace gpum
+
+extern "C" {
 
 int gpum_hip_model_run_da(gpum_hip_model* m, gpum_hip_surrogate* S,
                           const double* init, uint32_t n_chains,

@@ -1,8 +1,11 @@
-// da.h -- per-iteration kernels of the stepwise sampler: plain Metropolis
-// or two-stage delayed acceptance with a surrogate first stage.
+// da.h -- per-iteration kernels of the stepwise sampler: plain Metropolis,
+// two-stage delayed acceptance with a surrogate first stage, synchronous
+// Differential Evolution and Langevin (MALA) steps.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "eval64.h"
 
 namespace gpum {
 
@@ -48,6 +51,15 @@ struct DaArgs {
   double t_mean, t_sd;       // loglik = t_mean + t_sd * network output
   uint16_t* Xb;              // network input rows (bf16, 16 per row)
   const float* s_raw;        // network output per row
+  // Langevin proposals (mode 3, all FP64): gradients of the log-posterior
+  // at the states and at the proposals, the proposals' means, the
+  // proposals' data terms, and the prior as a tape.
+  double* gx;
+  double* gy;
+  double* mean_x;
+  double* ll_prop;
+  const TapeOp* prior_tape;
+  uint32_t n_prior_ops;
   // outputs
   float* draws_it;       // iteration-major staging: (t * np + j) * C + c
   float* draws;          // reference layout t + n_iter (c + C j)
@@ -57,6 +69,17 @@ struct DaArgs {
   double* trace_prop;    // n_iter x C x np proposals
   double* trace_ll;      // n_iter x C data term of the proposal (NaN: not run)
 };
+
+// Langevin steps (mode 3), one thread per chain: the log-posterior and
+// gradient at the states from the data terms in ll_prop / gx; the
+// proposals; the decisions from the data terms in ll_prop / gy.
+hipError_t launch_mala_init(const DaArgs* A, uint32_t n_chains, hipStream_t s);
+hipError_t launch_mala_propose(const DaArgs* A, uint32_t n_chains, uint32_t t,
+                               hipStream_t s);
+hipError_t launch_mala_accept(const DaArgs* A, uint32_t n_chains, uint32_t t,
+                              hipStream_t s);
+// Longest prior the Langevin kernels differentiate.
+constexpr uint32_t kMalaMaxPriorOps = 256;
 
 hipError_t launch_da_init(const DaArgs* A, uint32_t n_chains, hipStream_t s);
 hipError_t launch_da_init_finish(const DaArgs* A, uint32_t n_chains,

@@ -19,13 +19,17 @@
 //             reference's layout once after the run.
 // Without a surrogate every proposal passes stage 1 and stage 2 is the
 // reference's Metropolis test (cpu_native.rs:785-799) with the same
-// uniforms as the fused chain kernel.
+// uniforms as the fused chain kernel.  Mode 1 (synchronous Differential
+// Evolution) changes only the proposal.  Mode 3 (Langevin) runs its own
+// propose and accept kernels around the FP64 value-and-gradient kernel of
+// eval64.hip.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 
 #include "bf16.h"
 #include "da.h"
+#include "eval64_device.h"
 #include "mlp.h"
 #include "philox.h"
 #include "step_common.h"
@@ -269,7 +273,179 @@ __global__ void __launch_bounds__(256) da_draws_kernel(const float* __restrict__
   }
 }
 
+// ---- Langevin steps (mode 3), cpu_native.rs:573-724 -------------------
+// One thread per chain; n_params <= 16, so each chain's vectors are short.
+// Every expression is rounded as the reference's (no fused multiply-add).
+
+// The preconditioned, truncated drift f * 0.5 * M g with M = Ls Ls' (Ls
+// the chain's Cholesky factor, or the diagonal proposal_sd), the whitened
+// drift norm capped at 2 sqrt(d) (cpu_native.rs:618-651).
+__device__ void mala_drift(const DaArgs* __restrict__ A, uint32_t c,
+                           const double* g, double* out) {
+#pragma clang fp contract(off)
+  const uint32_t np = A->n_params;
+  const double dmax = 2.0 * sqrt((double)np);
+  const double* sd = A->proposal_sd + (size_t)c * np;
+  const double* L = A->proposal_l ? A->proposal_l + (size_t)c * np * np : nullptr;
+  double tmp[kMaxParams];
+  for (uint32_t j = 0; j < np; ++j) {
+    if (!L) {
+      tmp[j] = 0.5 * sd[j] * g[j];
+    } else {
+      double acc = 0.0;
+      for (uint32_t k = j; k < np; ++k) acc += L[k * np + j] * g[k];
+      tmp[j] = 0.5 * acc;
+    }
+  }
+  double nrm = 0.0;
+  for (uint32_t j = 0; j < np; ++j) nrm += tmp[j] * tmp[j];
+  nrm = sqrt(nrm);
+  const double f = nrm > dmax ? dmax / nrm : 1.0;
+  for (uint32_t k = 0; k < np; ++k) {
+    if (!L) {
+      out[k] = f * sd[k] * tmp[k];
+    } else {
+      double acc = 0.0;
+      for (uint32_t j = 0; j <= k; ++j) acc += L[k * np + j] * tmp[j];
+      out[k] = f * acc;
+    }
+  }
+}
+
+// -0.5 |Ls^{-1} (v - m)|^2 by forward substitution (cpu_native.rs:655-675).
+__device__ double mala_log_q(const DaArgs* __restrict__ A, uint32_t c,
+                             const double* v, const double* m) {
+#pragma clang fp contract(off)
+  const uint32_t np = A->n_params;
+  const double* sd = A->proposal_sd + (size_t)c * np;
+  const double* L = A->proposal_l ? A->proposal_l + (size_t)c * np * np : nullptr;
+  double tmp[kMaxParams];
+  double q = 0.0;
+  for (uint32_t k = 0; k < np; ++k) {
+    if (!L) {
+      const double r = (v[k] - m[k]) / sd[k];
+      q -= 0.5 * r * r;
+    } else {
+      double acc = v[k] - m[k];
+      for (uint32_t j = 0; j < k; ++j) acc -= L[k * np + j] * tmp[j];
+      tmp[k] = acc / L[k * np + k];
+      q -= 0.5 * tmp[k] * tmp[k];
+    }
+  }
+  return q;
+}
+
+// The prior's value at p, its gradient added into g (after the data
+// term's, as the reference adds them, cpu_native.rs:285-298).
+__device__ double mala_prior(const DaArgs* __restrict__ A, const double* p,
+                             double (&g)[kMaxParams]) {
+  if (A->n_prior_ops == 0) return 0.0;
+  Eval64Args pa{};
+  pa.prog = A->prior_tape;
+  pa.n_ops = A->n_prior_ops;
+  pa.n_params = A->n_params;
+  pa.consts = A->prior_consts;
+  return grad_obs64<kMalaMaxPriorOps>(pa, p, 0u, g);
+}
+
+__global__ void mala_init_kernel(const DaArgs* __restrict__ A) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A->n_chains) return;
+  const uint32_t np = A->n_params;
+  double g[kMaxParams];
+  for (int k = 0; k < kMaxParams; ++k)
+    g[k] = (uint32_t)k < np ? A->gx[(size_t)c * np + k] : 0.0;
+  const double pv = mala_prior(A, A->theta + (size_t)c * np, g);
+  A->cur[c] = A->ll_prop[c] + pv;
+  for (uint32_t k = 0; k < np; ++k) A->gx[(size_t)c * np + k] = g[k];
+}
+
+__global__ void mala_propose_kernel(const DaArgs* __restrict__ A, uint32_t t) {
+#pragma clang fp contract(off)
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A->n_chains) return;
+  const uint32_t np = A->n_params;
+  const double two_pi = 6.283185307179586;
+  const double* th = A->theta + (size_t)c * np;
+  double mx[kMaxParams], z[kMaxParams];
+  mala_drift(A, c, A->gx + (size_t)c * np, mx);
+  StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
+  for (uint32_t j = 0; j < np; ++j) {
+    mx[j] += th[j];
+    const double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
+    z[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+  }
+  const double* sd = A->proposal_sd + (size_t)c * np;
+  const double* L = A->proposal_l ? A->proposal_l + (size_t)c * np * np : nullptr;
+  for (uint32_t k = 0; k < np; ++k) {
+    double p;
+    if (!L) {
+      p = mx[k] + sd[k] * z[k];
+    } else {
+      p = mx[k];
+      for (uint32_t j = 0; j <= k; ++j) p += L[k * np + j] * z[j];
+    }
+    A->prop[(size_t)c * np + k] = p;
+    A->mean_x[(size_t)c * np + k] = mx[k];
+  }
+}
+
+__global__ void mala_accept_kernel(const DaArgs* __restrict__ A, uint32_t t) {
+#pragma clang fp contract(off)
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t C = A->n_chains;
+  if (c >= C) return;
+  const uint32_t np = A->n_params;
+  const double* pr = A->prop + (size_t)c * np;
+  double* th = A->theta + (size_t)c * np;
+  double g[kMaxParams], my[kMaxParams];
+  for (int k = 0; k < kMaxParams; ++k)
+    g[k] = (uint32_t)k < np ? A->gy[(size_t)c * np + k] : 0.0;
+  const double ll = A->ll_prop[c];
+  const double plp = ll + mala_prior(A, pr, g);
+  mala_drift(A, c, g, my);
+  for (uint32_t j = 0; j < np; ++j) my[j] += pr[j];
+  const double lq = mala_log_q(A, c, th, my) -
+                    mala_log_q(A, c, pr, A->mean_x + (size_t)c * np);
+  StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
+  const double u = us.get(2 * np);
+  const bool acc_ok =
+      isfinite(plp) && log(u) < (plp - A->cur[c]) / A->temperatures[c] + lq;
+  if (acc_ok) {
+    for (uint32_t j = 0; j < np; ++j) {
+      th[j] = pr[j];
+      A->gx[(size_t)c * np + j] = g[j];
+    }
+    A->cur[c] = plp;
+    A->accepts[c] += 1.0;
+  }
+  A->passes[c] += 1.0;
+  if (A->trace_plp) A->trace_plp[(size_t)t * C + c] = plp;
+  if (A->trace_ll) A->trace_ll[(size_t)t * C + c] = ll;
+  if (A->trace_prop)
+    for (uint32_t j = 0; j < np; ++j)
+      A->trace_prop[((size_t)t * C + c) * np + j] = pr[j];
+  if (A->trace_stage) A->trace_stage[(size_t)t * C + c] = acc_ok ? 2 : 1;
+  for (uint32_t j = 0; j < np; ++j)
+    A->draws_it[((size_t)t * np + j) * C + c] = (float)th[j];
+}
+
 static dim3 chain_grid(uint32_t n) { return dim3((n + 255) / 256); }
+
+hipError_t launch_mala_init(const DaArgs* A, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(mala_init_kernel, chain_grid(n), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+hipError_t launch_mala_propose(const DaArgs* A, uint32_t n, uint32_t t,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(mala_propose_kernel, chain_grid(n), dim3(256), 0, s, A, t);
+  return hipGetLastError();
+}
+hipError_t launch_mala_accept(const DaArgs* A, uint32_t n, uint32_t t,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(mala_accept_kernel, chain_grid(n), dim3(256), 0, s, A, t);
+  return hipGetLastError();
+}
 
 hipError_t launch_da_init(const DaArgs* A, uint32_t n, hipStream_t s) {
   hipLaunchKernelGGL(da_init_kernel, chain_grid(n), dim3(256), 0, s, A);

@@ -32,6 +32,8 @@ bool is_unary_op(uint32_t op) {
          op == OP_LGAMMA;
 }
 
+}  // namespace
+
 // The program with each instruction's operand positions on the tape: a
 // stack of the instructions whose values are live, as the reference's
 // forward pass keeps (cpu_native.rs:156-211).
@@ -56,6 +58,8 @@ std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops) {
   }
   return tape;
 }
+
+namespace {
 
 // Everything one call uploads.
 struct EvalSetup {
@@ -250,9 +254,10 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
   const uint32_t ns = (n_obs + kGradSliceObs - 1) / kGradSliceObs;
   const uint32_t chunk = (uint32_t)std::max<size_t>(
       1, std::min<size_t>(n_points,
-                          kGradPartialBytes / ((size_t)ns * n_params * 8)));
+                          kGradPartialBytes / ((size_t)ns * (n_params + 1) * 8)));
   DevBuf<double> d_part, d_out;
-  if (!d_part.alloc((size_t)chunk * ns * n_params, "allocating partial sums") ||
+  if (!d_part.alloc((size_t)chunk * ns * (n_params + 1),
+                    "allocating partial sums") ||
       !d_out.alloc((size_t)n_points * n_params, "allocating the output"))
     return 1;
   hipStream_t st = g_dev.stream;
@@ -268,7 +273,7 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
     e = launch_grad64(a, grid_for((uint64_t)ns * np), st);
     if (e == hipSuccess)
       e = launch_grad64_finish(d_part.p, np, n_params, ns,
-                               d_out.p + (size_t)p0 * n_params, st);
+                               d_out.p + (size_t)p0 * n_params, nullptr, st);
   }
   if (e == hipSuccess) e = hipEventRecord(ev.e1, st);
   if (e == hipSuccess)

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace gpum {
 
 // One reference instruction with the tape positions of its operands: the
@@ -40,15 +42,21 @@ struct Eval64Args {
   uint32_t depth;        // peak stack depth of the program
 };
 
+// The program with each instruction's operand positions on the tape (host;
+// the program must be valid, validate_program).
+std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops);
+
 // out[p * n_obs + i] = program at point p, observation i.
 hipError_t launch_pointwise64(const Eval64Args& a, uint32_t grid,
                               hipStream_t st);
-// partials[(p * n_slices + s) * n_params + k]: d/d param k of the sum over
-// slice s's observations at point p.
+// partials[(p * n_slices + s) * (n_params + 1) + k]: d/d param k of the
+// sum over slice s's observations at point p (k < n_params), and the sum
+// itself (k = n_params).
 hipError_t launch_grad64(const Eval64Args& a, uint32_t grid, hipStream_t st);
-// out[p * n_params + k] = sum over slices, in slice order.
+// out[p * n_params + k] = gradient summed over slices in slice order;
+// value[p] (when not NULL) = the summed value.
 hipError_t launch_grad64_finish(const double* partials, uint32_t n_points,
                                 uint32_t n_params, uint32_t n_slices,
-                                double* out, hipStream_t st);
+                                double* out, double* value, hipStream_t st);
 
 }  // namespace gpum

@@ -172,6 +172,10 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
   }
   m->n_params = n_params;
   m->n_obs = n_obs;
+  m->code_ref.assign(code, code + code_len);
+  m->consts_ref.assign(consts, consts + n_consts);
+  m->prior_ref.assign(prior_code, prior_code + prior_len);
+  m->n_cols_in = n_cols;
   // Columns: originals cast to FP32 (as the reference's GPU path casts,
   // chain_kernel.rs:501-505), then precomputed data-only subtrees evaluated
   // in FP64 on the original f64 values and rounded once.
@@ -440,6 +444,24 @@ hipError_t launch_module(hipFunction_t f, dim3 grid, dim3 block,
                                block.z, (unsigned)lds_bytes, s, args, nullptr);
 }
 
+bool ensure_eval64(gpum_hip_model* m) {
+  if (m->eval64_ready) return true;
+  const std::vector<TapeOp> tl = make_tape(m->code_ref.data(), m->code_ref.size() / 2);
+  const std::vector<TapeOp> tp = make_tape(m->prior_ref.data(), m->prior_ref.size() / 2);
+  // The original columns as the kernels see them (FP32), widened.
+  std::vector<double> d((size_t)m->n_cols_in * m->n_obs);
+  if (!m->L.empty)
+    for (size_t k = 0; k < d.size(); ++k) d[k] = (double)m->cols[k];
+  if (!m->tape_ll.upload(tl.data(), tl.size(), "upload program") ||
+      !m->tape_prior.upload(tp.data(), tp.size(), "upload prior") ||
+      !m->consts64.upload(m->consts_ref.data(), m->consts_ref.size(),
+                          "upload consts") ||
+      !m->data64.upload(d.data(), d.size(), "upload data"))
+    return false;
+  m->eval64_ready = true;
+  return true;
+}
+
 bool ensure_data(gpum_hip_model* m, uint32_t n_pad) {
   if (m->data.p && m->data_ld == n_pad) return true;
   const uint32_t ncol = std::max(1u, m->n_cols_total);
@@ -470,8 +492,8 @@ bool validate_run(const gpum_hip_model* m, const double* init,
     return false;
   }
   if (proposal_mode == 3) {
-    set_err("proposal_mode 3 (MALA) runs on the cpu backend; the hip "
-            "backend implements modes 0, 1 and 2");
+    // Routed to the Langevin sampler before a fused launch is prepared.
+    set_err("proposal_mode 3 (MALA) runs on the stepwise sampler");
     return false;
   }
   if (proposal_mode > 3) {
@@ -970,6 +992,9 @@ int gpum_hip_model_run(gpum_hip_model* m, const double* init,
     return 1;
   }
   if (!device_init()) return 1;
+  if (proposal_mode == 3)
+    return run_langevin(m, init, n_chains, proposal_sd, temperatures, n_iter,
+                        seed, proposal_l, ext, out);
   return model_run_locked(m, init, n_chains, proposal_sd, temperatures, n_iter,
                           seed, proposal_mode, gamma, de_noise, proposal_l,
                           ext, out, nullptr);
@@ -1016,6 +1041,9 @@ int gpum_hip_run_chains_ex(const uint32_t* code, size_t code_len,
                    prior_consts, n_prior_consts, n_params, data, n_cols, n_obs,
                    pre, fu))
     return 1;
+  if (proposal_mode == 3)
+    return run_langevin(&m, init, n_chains, proposal_sd, temperatures, n_iter,
+                        seed, proposal_l, ext, out);
   return model_run_locked(&m, init, n_chains, proposal_sd, temperatures,
                           n_iter, seed, proposal_mode, gamma, de_noise,
                           proposal_l, ext, out, nullptr);

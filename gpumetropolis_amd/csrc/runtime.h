@@ -9,6 +9,7 @@
 
 #include "../../include/gpum_hip.h"
 #include "chain_kernel.h"
+#include "eval64.h"
 #include "lower.h"
 
 namespace gpum {
@@ -114,6 +115,15 @@ struct gpum_hip_model {
   uint32_t data_chunk = 0;  // layout granularity the data was padded to
   // Launches run per-model compiled kernels (else the interpreter).
   bool jit = false;
+  // The programs as given and the original column count, for the FP64
+  // value-and-gradient path of Langevin proposals; its device state (tapes,
+  // the FP32 columns widened to FP64) is built on first use.
+  std::vector<uint32_t> code_ref, prior_ref;
+  std::vector<double> consts_ref;
+  uint32_t n_cols_in = 0;
+  bool eval64_ready = false;
+  gpum::rt::DevBuf<gpum::TapeOp> tape_ll, tape_prior;
+  gpum::rt::DevBuf<double> data64, consts64;
 };
 
 namespace gpum {
@@ -135,6 +145,14 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
                  const TilingForce* force = nullptr, int cw_req = 0);
 // Lay the model's columns out on the device with column stride n_pad.
 bool ensure_data(gpum_hip_model* m, uint32_t n_pad);
+// Langevin (MALA, proposal_mode 3) chains of gpum_hip_run_chains_ex /
+// gpum_hip_model_run, run by the stepwise sampler; the library lock held.
+int run_langevin(gpum_hip_model* m, const double* init, uint32_t n_chains,
+                 const double* proposal_sd, const double* temperatures,
+                 uint32_t n_iter, uint32_t seed, const double* proposal_l,
+                 gpum_run_ext* ext, gpum_chain_result* out);
+// The FP64 tapes and data of a model (Langevin proposals), built once.
+bool ensure_eval64(gpum_hip_model* m);
 // Validate, lower and upload a model (precompute / fuse: -1 default, 0, 1).
 bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
                  const double* consts, size_t n_consts,

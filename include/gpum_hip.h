@@ -24,9 +24,15 @@
  *     `proposal_l` is n_chains x n_params x n_params row-major lower
  *     triangular, or NULL.
  *   - proposal_mode: 0 diagonal random walk, 1 Differential Evolution against
- *     the `init` snapshot, 2 Cholesky random walk, 3 MALA (not available on
- *     this backend: rejected with an error, as the reference's GPU backends
- *     refuse it, R/api.R:373-377).
+ *     the `init` snapshot, 2 Cholesky random walk, 3 MALA (cpu_native.rs:
+ *     573-724: preconditioned, truncated Langevin drift with the
+ *     asymmetric-proposal correction; preconditioner proposal_l when given,
+ *     else proposal_sd).  The reference's GPU backends refuse MALA
+ *     (R/api.R:373-377); here it runs on the stepwise sampler with the
+ *     log-posterior and its gradient in FP64 (reverse mode over the
+ *     bytecode, the model's FP32 columns widened), at most 1024
+ *     instructions in the program and 256 in the prior; gpum_run_ext fused
+ *     batches (swap_every) are not available with it.
  *   - draws are f32, column-major (n_iter, n_chains, n_params):
  *     draws[t + n_iter * (c + n_chains * j)].
  *   - Uniforms come from a counter-based Philox4x32-10 stream keyed on
@@ -345,12 +351,13 @@ typedef struct gpum_da_ext {
   double de_noise;          /* proposal_mode 1: jitter scale on proposal_sd */
 } gpum_da_ext;
 
-/* Random-walk Metropolis (proposal_mode 0 or 2) or synchronous
- * Differential Evolution (proposal_mode 1, ext->gamma / ext->de_noise, at
- * least 3 chains, no surrogate: path B of cpu_native.rs:400-506, every
- * chain proposing against the previous generation's whole population) run
- * one iteration per launch sequence; the launch boundary is the
- * generation barrier.  With a surrogate (s non-NULL, n_in == n_params), each
+/* Random-walk Metropolis (proposal_mode 0 or 2), synchronous Differential
+ * Evolution (proposal_mode 1, ext->gamma / ext->de_noise, at least 3
+ * chains, no surrogate: path B of cpu_native.rs:400-506, every chain
+ * proposing against the previous generation's whole population; the launch
+ * boundary is the generation barrier) or MALA (proposal_mode 3, no
+ * surrogate; see proposal_mode above) run one iteration per launch
+ * sequence.  With a surrogate (s non-NULL, n_in == n_params), each
  * proposal first faces the surrogate's Metropolis test and only survivors
  * are evaluated exactly; the exact test divides out the surrogate ratio,
  * so the target is unchanged (two-stage delayed acceptance).  With

@@ -256,9 +256,10 @@ def test_same_seed_identical(hip):
 
 def test_errors_are_reported(hip):
     m = dsl.gpum_model("~ -((y - mu)^2) / 2", ["mu"], ["y"])
-    with pytest.raises(hip.BackendError, match="MALA|mala"):
+    with pytest.raises(hip.BackendError, match="proposal_mode 3"):
         hip.run_chains(m.loglik.code, m.loglik.consts, [], [], 1, [1.0], 1, 1,
-                       [[0.0]], [[0.1]], [1.0], 10, 1, proposal_mode=3)
+                       [[0.0]], [[0.1]], [1.0], 10, 1, proposal_mode=3,
+                       batches={"swap_every": 5, "swap_group": 1})
     with pytest.raises(hip.BackendError, match="opcode"):
         hip.run_chains([99, 0], [], [], [], 1, [1.0], 1, 1, [[0.0]], [[0.1]],
                        [1.0], 10, 1)
