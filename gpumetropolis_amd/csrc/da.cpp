@@ -521,6 +521,8 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   ea.n_points = C;
   ea.out = part.p;
   ea.n_slices = ns;
+  ea.depth = (uint32_t)m->L.ref_depth;
+  ea.n_slots = m->slots_ll;
   const uint32_t grid = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>((uint64_t)ns * C, (uint64_t)g_dev.n_cu * 8u));
   auto data_term = [&](const double* points, double* g) -> hipError_t {

@@ -345,7 +345,7 @@ __device__ double mala_prior(const DaArgs* __restrict__ A, const double* p,
   pa.n_ops = A->n_prior_ops;
   pa.n_params = A->n_params;
   pa.consts = A->prior_consts;
-  return grad_obs64<kMalaMaxPriorOps>(pa, p, 0u, g);
+  return grad_obs64<kMaxStack, kMalaMaxPriorOps>(pa, p, 0u, g);
 }
 
 __global__ void mala_init_kernel(const DaArgs* __restrict__ A) {

@@ -34,14 +34,18 @@ bool is_unary_op(uint32_t op) {
 
 }  // namespace
 
-// The program with each instruction's operand positions on the tape: a
-// stack of the instructions whose values are live, as the reference's
-// forward pass keeps (cpu_native.rs:156-211).
-std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops) {
+// The program with each instruction's operand positions: a stack of the
+// instructions whose values are live, as the reference's forward pass keeps
+// (cpu_native.rs:156-211).  Then the values the backward pass reads
+// (cpu_native.rs:218-277) get slots: the operands of MUL, the divisor of
+// DIV, the operands of LOG, LGAMMA and POW, and the results of DIV, EXP,
+// SQRT and POW, unless they are leaves (re-read instead).
+std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops,
+                              uint32_t* n_slots) {
   std::vector<TapeOp> tape(n_ops);
   std::vector<uint32_t> st;
   for (size_t pc = 0; pc < n_ops; ++pc) {
-    TapeOp o{code[2 * pc], code[2 * pc + 1], 0u, 0u};
+    TapeOp o{code[2 * pc], code[2 * pc + 1], 0u, 0u, kNoSlot};
     if (!is_leaf(o.op)) {
       if (is_unary_op(o.op)) {
         o.l = st.back();
@@ -56,6 +60,24 @@ std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops) {
     st.push_back((uint32_t)pc);
     tape[pc] = o;
   }
+  std::vector<char> keep(n_ops, 0);
+  for (size_t pc = 0; pc < n_ops; ++pc) {
+    const TapeOp& o = tape[pc];
+    switch (o.op) {
+      case OP_MUL: keep[o.l] = keep[o.r] = 1; break;
+      case OP_DIV: keep[o.r] = 1; keep[pc] = 1; break;
+      case OP_LOG:
+      case OP_LGAMMA: keep[o.l] = 1; break;
+      case OP_EXP:
+      case OP_SQRT: keep[pc] = 1; break;
+      case OP_POW: keep[o.l] = keep[o.r] = 1; keep[pc] = 1; break;
+      default: break;
+    }
+  }
+  uint32_t n = 0;
+  for (size_t pc = 0; pc < n_ops; ++pc)
+    if (keep[pc] && !is_leaf(tape[pc].op)) tape[pc].slot = n++;
+  if (n_slots) *n_slots = n;
   return tape;
 }
 
@@ -97,7 +119,8 @@ bool setup(const uint32_t* code, size_t code_len, const double* consts,
     return false;
   }
   const size_t n_ops = code_len / 2;
-  std::vector<TapeOp> tape = make_tape(code, n_ops);
+  uint32_t n_slots = 0;
+  std::vector<TapeOp> tape = make_tape(code, n_ops, &n_slots);
   // Row-major (n_obs x n_cols) in, column-major on the device.
   std::vector<double> cols((size_t)n_obs * n_cols);
   for (uint32_t i = 0; i < n_obs; ++i)
@@ -119,6 +142,7 @@ bool setup(const uint32_t* code, size_t code_len, const double* consts,
   a.n_points = n_points;
   a.points = S->points.p;
   a.depth = (uint32_t)depth;
+  a.n_slots = n_slots;
   return true;
 }
 

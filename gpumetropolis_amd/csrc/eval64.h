@@ -10,16 +10,20 @@
 
 namespace gpum {
 
-// One reference instruction with the tape positions of its operands: the
-// index of the instruction that produced each (every instruction produces
-// one value, so the instruction index is the tape index).  l / r are unused
-// by leaves, r by unary operations.
+// One reference instruction with the positions of its operands: the index
+// of the instruction that produced each (every instruction produces one
+// value).  l / r are unused by leaves, r by unary operations.  `slot`: where
+// the gradient's forward pass keeps this instruction's value for the
+// backward pass (kNoSlot when the backward pass never reads it; leaves are
+// re-read instead of kept).
 struct TapeOp {
   uint32_t op;
   uint32_t arg;
   uint32_t l;
   uint32_t r;
+  uint32_t slot;
 };
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 // Observations per (point, slice) work item of the gradient kernel: 256
 // threads, 16 observations each.  It depends on nothing but the kernel, so
@@ -40,11 +44,14 @@ struct Eval64Args {
   double* out;           // pointwise: n_points x n_obs; gradient: partials
   uint32_t n_slices;     // gradient: observation slices per point
   uint32_t depth;        // peak stack depth of the program
+  uint32_t n_slots;      // gradient: values kept for the backward pass
 };
 
-// The program with each instruction's operand positions on the tape (host;
-// the program must be valid, validate_program).
-std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops);
+// The program with each instruction's operand positions and kept-value
+// slots (host; the program must be valid, validate_program).  *n_slots
+// (may be NULL): slots used.
+std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops,
+                              uint32_t* n_slots = nullptr);
 
 // out[p * n_obs + i] = program at point p, observation i.
 hipError_t launch_pointwise64(const Eval64Args& a, uint32_t grid,

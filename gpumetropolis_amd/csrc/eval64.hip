@@ -34,8 +34,11 @@ __global__ __launch_bounds__(256) void pointwise64_kernel(Eval64Args a) {
   }
 }
 
-template <int MAXT>
-__global__ __launch_bounds__(256) void grad64_kernel(Eval64Args a) {
+// At least two waves per SIMD (at most 256 VGPRs): with one the FP64
+// arithmetic's latency is exposed, with four the stacks spill to scratch
+// (logistic-16, 33 points x 1e6 rows: 50 ms, 28.7 ms, 176 ms).
+template <int MAXD, int MAXS>
+__global__ __launch_bounds__(256, 2) void grad64_kernel(Eval64Args a) {
   __shared__ double red[4][kMaxParams + 1];
   const uint32_t ns = a.n_slices;
   const uint64_t total = (uint64_t)ns * a.n_points;
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(256) void grad64_kernel(Eval64Args a) {
     const uint32_t i0 = s * kGradSliceObs + threadIdx.x;
     for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
       const uint32_t i = i0 + j * 256u;
-      if (i < a.n_obs) val += grad_obs64<MAXT>(a, P, i, g);
+      if (i < a.n_obs) val += grad_obs64<MAXD, MAXS>(a, P, i, g);
     }
     // Wave sums (butterfly), then the four waves in order; slot kMaxParams
     // carries the value.
@@ -100,13 +103,17 @@ hipError_t launch_pointwise64(const Eval64Args& a, uint32_t grid,
 }
 
 hipError_t launch_grad64(const Eval64Args& a, uint32_t grid, hipStream_t st) {
-  if (a.n_ops <= 64)
-    hipLaunchKernelGGL(grad64_kernel<64>, dim3(grid), dim3(256), 0, st, a);
-  else if (a.n_ops <= 256)
-    hipLaunchKernelGGL(grad64_kernel<256>, dim3(grid), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL(grad64_kernel<kMaxTapeOps>, dim3(grid), dim3(256), 0,
+  // Shallow stacks and few kept values stay in registers.
+  if (a.depth <= 8 && a.n_slots <= 16)
+    hipLaunchKernelGGL((grad64_kernel<8, 16>), dim3(grid), dim3(256), 0, st, a);
+  else if (a.depth <= 8 && a.n_slots <= 64)
+    hipLaunchKernelGGL((grad64_kernel<8, 64>), dim3(grid), dim3(256), 0, st, a);
+  else if (a.n_slots <= 64)
+    hipLaunchKernelGGL((grad64_kernel<kMaxStack, 64>), dim3(grid), dim3(256), 0,
                        st, a);
+  else
+    hipLaunchKernelGGL((grad64_kernel<kMaxStack, kMaxTapeOps>), dim3(grid),
+                       dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

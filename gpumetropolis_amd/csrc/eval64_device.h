@@ -48,41 +48,58 @@ __device__ double eval_obs64(const Eval64Args& a, const double* P,
   return st[0];
 }
 
+// A leaf's value: the constant, the parameter or the observation's datum.
+__device__ __forceinline__ double leaf64(const Eval64Args& a, const TapeOp& o,
+                                         const double* P, uint32_t i) {
+  if (o.op == OP_PUSH_CONST) return a.consts[o.arg];
+  if (o.op == OP_PUSH_PARAM) return P[o.arg];
+  return data_at(a, o.arg, i);
+}
+
 // One observation's value and gradient, cpu_native.rs:148-279.  The forward
-// pass records every instruction's value on the tape.  Each value is the
-// operand of exactly one later instruction, so its adjoint receives exactly
-// one contribution; walking the program backwards, the pending adjoints form
-// a stack (the operands' adjoints are pushed, the right operand's last, and
+// pass is the stack machine; the values the backward pass reads are kept in
+// their slots (leaves are re-read instead).  Each value is the operand of
+// exactly one later instruction, so its adjoint receives exactly one
+// contribution: walking the program backwards, the pending adjoints form a
+// stack (the operands' adjoints are pushed, the right operand's last, and
 // the right operand is the instruction just before).  A zero adjoint
 // propagates zeros, as the reference skips it.  d/d param k accumulates
 // into g[k]; returns the value.  An empty program is not evaluated here.
-template <int MAXT>
+// MAXD bounds the stack depth, MAXS the kept values.
+template <int MAXD, int MAXS>
 __device__ double grad_obs64(const Eval64Args& a, const double* P, uint32_t i,
                              double (&g)[kMaxParams]) {
 #pragma clang fp contract(off)
-  double v[MAXT];
+  double st[MAXD];
+  double kept[MAXS];
+  int sp = 0;
   for (uint32_t pc = 0; pc < a.n_ops; ++pc) {
     const TapeOp o = a.prog[pc];
-    double x;
     switch (o.op) {
-      case OP_PUSH_CONST: x = a.consts[o.arg]; break;
-      case OP_PUSH_PARAM: x = P[o.arg]; break;
-      case OP_PUSH_DATA: x = data_at(a, o.arg, i); break;
-      case OP_ADD: x = v[o.l] + v[o.r]; break;
-      case OP_SUB: x = v[o.l] - v[o.r]; break;
-      case OP_MUL: x = v[o.l] * v[o.r]; break;
-      case OP_DIV: x = v[o.l] / v[o.r]; break;
-      case OP_NEG: x = -v[o.l]; break;
-      case OP_EXP: x = exp(v[o.l]); break;
-      case OP_LOG: x = log(v[o.l]); break;
-      case OP_SQRT: x = sqrt(v[o.l]); break;
-      case OP_LGAMMA: x = lgamma64(v[o.l]); break;
-      default: x = pow(v[o.l], v[o.r]); break;
+      case OP_PUSH_CONST: st[sp++] = a.consts[o.arg]; break;
+      case OP_PUSH_PARAM: st[sp++] = P[o.arg]; break;
+      case OP_PUSH_DATA: st[sp++] = data_at(a, o.arg, i); break;
+      case OP_ADD: st[sp - 2] = st[sp - 2] + st[sp - 1]; --sp; break;
+      case OP_SUB: st[sp - 2] = st[sp - 2] - st[sp - 1]; --sp; break;
+      case OP_MUL: st[sp - 2] = st[sp - 2] * st[sp - 1]; --sp; break;
+      case OP_DIV: st[sp - 2] = st[sp - 2] / st[sp - 1]; --sp; break;
+      case OP_NEG: st[sp - 1] = -st[sp - 1]; break;
+      case OP_EXP: st[sp - 1] = exp(st[sp - 1]); break;
+      case OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
+      case OP_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
+      case OP_LGAMMA: st[sp - 1] = lgamma64(st[sp - 1]); break;
+      default: st[sp - 2] = pow(st[sp - 2], st[sp - 1]); --sp; break;
     }
-    v[pc] = x;
+    if (o.slot != kNoSlot) kept[o.slot] = st[sp - 1];
   }
-  double adj[kMaxStack];
-  int sp = 0;
+  const double value = st[0];
+  // An operand's value in the backward pass.
+  auto val = [&](uint32_t k) -> double {
+    const TapeOp q = a.prog[k];
+    return q.op <= OP_PUSH_DATA ? leaf64(a, q, P, i) : kept[q.slot];
+  };
+  double adj[MAXD];
+  sp = 0;
   adj[sp++] = 1.0;
   for (int pc = (int)a.n_ops - 1; pc >= 0; --pc) {
     const TapeOp o = a.prog[pc];
@@ -102,17 +119,22 @@ __device__ double grad_obs64(const Eval64Args& a, const double* P, uint32_t i,
         break;
       case OP_ADD: dl = ga; dr = ga; break;
       case OP_SUB: dl = ga; dr = -ga; break;
-      case OP_MUL: dl = ga * v[o.r]; dr = ga * v[o.l]; break;
-      case OP_DIV: dl = ga / v[o.r]; dr = -(ga * v[pc] / v[o.r]); break;
+      case OP_MUL: dl = ga * val(o.r); dr = ga * val(o.l); break;
+      case OP_DIV: {
+        const double b = val(o.r);
+        dl = ga / b;
+        dr = -(ga * kept[o.slot] / b);
+        break;
+      }
       case OP_NEG: dl = -ga; break;
-      case OP_EXP: dl = ga * v[pc]; break;
-      case OP_LOG: dl = ga / v[o.l]; break;
-      case OP_SQRT: dl = ga * 0.5 / v[pc]; break;
-      case OP_LGAMMA: dl = ga * digamma64(v[o.l]); break;
+      case OP_EXP: dl = ga * kept[o.slot]; break;
+      case OP_LOG: dl = ga / val(o.l); break;
+      case OP_SQRT: dl = ga * 0.5 / kept[o.slot]; break;
+      case OP_LGAMMA: dl = ga * digamma64(val(o.l)); break;
       default: {
-        const double base = v[o.l], ex = v[o.r];
+        const double base = val(o.l), ex = val(o.r);
         dl = ga * ex * pow(base, ex - 1.0);
-        dr = base > 0.0 ? ga * v[pc] * log(base) : 0.0;
+        dr = base > 0.0 ? ga * kept[o.slot] * log(base) : 0.0;
         break;
       }
     }
@@ -121,7 +143,7 @@ __device__ double grad_obs64(const Eval64Args& a, const double* P, uint32_t i,
       if (o.op <= OP_DIV || o.op == OP_POW) adj[sp++] = live ? dr : 0.0;
     }
   }
-  return v[a.n_ops - 1];
+  return value;
 }
 
 }  // namespace gpum

@@ -446,7 +446,8 @@ hipError_t launch_module(hipFunction_t f, dim3 grid, dim3 block,
 
 bool ensure_eval64(gpum_hip_model* m) {
   if (m->eval64_ready) return true;
-  const std::vector<TapeOp> tl = make_tape(m->code_ref.data(), m->code_ref.size() / 2);
+  const std::vector<TapeOp> tl =
+      make_tape(m->code_ref.data(), m->code_ref.size() / 2, &m->slots_ll);
   const std::vector<TapeOp> tp = make_tape(m->prior_ref.data(), m->prior_ref.size() / 2);
   // The original columns as the kernels see them (FP32), widened.
   std::vector<double> d((size_t)m->n_cols_in * m->n_obs);

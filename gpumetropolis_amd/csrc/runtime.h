@@ -122,6 +122,7 @@ struct gpum_hip_model {
   std::vector<double> consts_ref;
   uint32_t n_cols_in = 0;
   bool eval64_ready = false;
+  uint32_t slots_ll = 0;  // values the gradient keeps for its backward pass
   gpum::rt::DevBuf<gpum::TapeOp> tape_ll, tape_prior;
   gpum::rt::DevBuf<double> data64, consts64;
 };
