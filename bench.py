@@ -366,6 +366,7 @@ def eval_section(a):
     from gpumetropolis_amd import _native, workloads
     from oracle import oracle
 
+    _native.set_eval_jit(a.jit)
     out = {}
     wg = workloads.gamma(n=a.n_obs)
     mg = wg.model
@@ -387,6 +388,7 @@ def eval_section(a):
     gbps = bpe * evals / (ms / 1e3) / 1e9
     out["pointwise"] = {
         "workload": f"gamma, {wg.n_obs} rows x 64 points, FP64",
+        "compiled": _native.last_eval_compiled(),
         "kernel_ms": ms, "evals_per_s": evals / (ms / 1e3),
         "wall_s_incl_pcie": wall,
         "roofline": {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK / 1e9,
@@ -412,6 +414,7 @@ def eval_section(a):
     out["gradient"] = {
         "workload": f"logistic-16, {wl.n_obs} rows x {len(stencil)} points "
                     "(Cramer-Rao stencil), FP64 reverse mode",
+        "compiled": _native.last_eval_compiled(),
         "kernel_ms": ms, "obs_grads_per_s": evals / (ms / 1e3),
         "wall_s_incl_upload": wall,
         "cpu_oracle_obs_grads_per_s": cpu_rate, "cpu_cores": 1}
@@ -465,6 +468,7 @@ def mala_section(a):
             w.data, w.n_cols, w.n_obs)
     _native.run_chains(*args, init, sd, np.ones(C), 2, a.seed, proposal_mode=3,
                        proposal_l=L)
+    _native.set_eval_jit(a.jit)
     dm = _native.DeviceModel(*args)
     r = dm.run_da(None, init, np.ones(C), T, a.seed, proposal_sd=sd,
                   proposal_l=L, langevin=True)

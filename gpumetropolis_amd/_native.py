@@ -144,6 +144,10 @@ def lib():
         fn.restype = C.c_int
     L.gpum_hip_last_eval_ms.argtypes = []
     L.gpum_hip_last_eval_ms.restype = C.c_double
+    L.gpum_hip_last_eval_compiled.argtypes = []
+    L.gpum_hip_last_eval_compiled.restype = C.c_int
+    L.gpum_hip_set_eval_jit.argtypes = [C.c_int32]
+    L.gpum_hip_set_eval_jit.restype = None
     L.gpum_hip_jit_check.argtypes = [_u32p, sz, _f64p, sz, _u32p, sz, _f64p,
                                      sz, C.c_uint32, C.c_uint32, C.c_int32,
                                      C.c_uint32, C.POINTER(C.c_double)]
@@ -462,7 +466,7 @@ def grad_batch(code, consts, n_params, data, n_cols, n_obs, points):
 
 
 JIT_KINDS = ("chain_stream", "chain_resident", "points_stream",
-             "points_resident", "batch")
+             "points_resident", "batch", "pointwise64", "grad64")
 
 
 def jit_check(code, consts, prior_code, prior_consts, n_params, n_cols,
@@ -486,6 +490,17 @@ def jit_check(code, consts, prior_code, prior_consts, n_params, n_cols,
 def last_eval_ms() -> float:
     """Kernel time of this thread's last loglik_pointwise / grad_batch."""
     return float(lib().gpum_hip_last_eval_ms())
+
+
+def last_eval_compiled() -> bool:
+    """Whether that call ran kernels compiled for its program."""
+    return bool(lib().gpum_hip_last_eval_compiled())
+
+
+def set_eval_jit(enable: bool) -> None:
+    """Compiled kernels for loglik_pointwise / grad_batch (default) or
+    always the interpreter; process-wide."""
+    lib().gpum_hip_set_eval_jit(1 if enable else 0)
 
 
 class DeviceModel:

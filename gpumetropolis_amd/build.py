@@ -25,12 +25,13 @@ SOURCES = ["chain_kernel_inst0.hip", "chain_kernel_inst1.hip",
            "lower.cpp", "eval64.cpp"]
 HEADERS = ["vm.h", "lower.h", "chain_kernel.h", "chain_kernels.h",
            "device_vm.h", "math64.h", "philox.h", "runtime.h", "step_common.h",
-           "bf16.h", "mlp.h", "da.h", "batch.h", "chain_body.h", "jit.h", "eval64_device.h",
+           "bf16.h", "mlp.h", "da.h", "batch.h", "chain_body.h", "jit.h", "eval64_device.h", "eval64_kernels.h",
            "eval64.h"]
 # Device headers embedded into the library for the per-model kernels
 # compiled at run time (csrc/jit.cpp).
 JIT_HEADERS = ["chain_body.h", "chain_kernel.h", "device_vm.h", "math64.h",
-               "philox.h", "step_common.h", "vm.h"]
+               "philox.h", "step_common.h", "vm.h", "eval64.h",
+               "eval64_device.h", "eval64_kernels.h"]
 JIT_INC = os.path.join(CSRC, "jit_sources.inc")
 
 HIPCC_FLAGS = [

@@ -525,6 +525,10 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   ea.n_slots = m->slots_ll;
   const uint32_t grid = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>((uint64_t)ns * C, (uint64_t)g_dev.n_cu * 8u));
+  // The gradient compiled for the program where the process can, as the
+  // batch evaluation's (same values either way).
+  const hipFunction_t gjf = eval64_compiled_kernel(
+      m->code_ref.data(), m->code_ref.size(), m->consts_ref.data(), JIT_GRAD64);
   auto data_term = [&](const double* points, double* g) -> hipError_t {
     if (n_ops == 0) {
       hipError_t r = hipMemsetAsync(ll.p, 0, C * sizeof(double), st);
@@ -532,7 +536,13 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
     }
     Eval64Args e = ea;
     e.points = points;
-    hipError_t r = launch_grad64(e, grid, st);
+    hipError_t r;
+    if (gjf) {
+      void* kargs[] = {(void*)&e};
+      r = launch_module(gjf, dim3(grid), dim3(256), 0, st, kargs);
+    } else {
+      r = launch_grad64(e, grid, st);
+    }
     return r == hipSuccess ? launch_grad64_finish(part.p, C, np, ns, g, ll.p, st)
                            : r;
   };

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "eval64.h"
+#include "jit.h"
 #include "runtime.h"
 
 namespace gpum {
@@ -20,6 +21,10 @@ using rt::set_err;
 using rt::set_hip_err;
 
 thread_local double g_last_eval_ms = 0.0;
+thread_local int g_last_eval_compiled = 0;
+// Kernels compiled per program when the process can (else, and when this
+// is off, the interpreter).
+bool g_eval_jit = true;
 
 // Device buffers bounded per launch: a pointwise chunk's output and a
 // gradient chunk's partial sums.
@@ -33,6 +38,14 @@ bool is_unary_op(uint32_t op) {
 }
 
 }  // namespace
+
+hipFunction_t eval64_compiled_kernel(const uint32_t* code, size_t code_len,
+                                     const double* consts, int kind) {
+  if (!g_eval_jit || code_len == 0) return nullptr;
+  std::string err;
+  return jit_kernel(jit_eval64_source(code, code_len, consts), (JitKind)kind,
+                    &err);
+}
 
 // The program with each instruction's operand positions: a stack of the
 // instructions whose values are live, as the reference's forward pass keeps
@@ -158,6 +171,18 @@ double elapsed_ms(hipEvent_t e0, hipEvent_t e1) {
   return (double)ms;
 }
 
+hipFunction_t compiled(const uint32_t* code, size_t code_len,
+                       const double* consts, JitKind kind) {
+  return eval64_compiled_kernel(code, code_len, consts, kind);
+}
+
+hipError_t launch_compiled(hipFunction_t f, const Eval64Args& a, uint32_t grid,
+                           hipStream_t st) {
+  Eval64Args arg = a;
+  void* kargs[] = {(void*)&arg};
+  return rt::launch_module(f, dim3(grid), dim3(256), 0, st, kargs);
+}
+
 struct Events {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   Events() {
@@ -185,6 +210,7 @@ int gpum_hip_loglik_pointwise(const uint32_t* code, size_t code_len,
                               double* out) {
   std::lock_guard<std::mutex> lk(rt::g_mu);
   g_last_eval_ms = 0.0;
+  g_last_eval_compiled = 0;
   if (!rt::device_init()) return 1;
   hipSetDevice(g_dev.id);
   int depth = 0;
@@ -212,6 +238,8 @@ int gpum_hip_loglik_pointwise(const uint32_t* code, size_t code_len,
   DevBuf<double> d_out;
   if (!d_out.alloc((size_t)chunk * n_obs, "allocating the output")) return 1;
   hipStream_t st = g_dev.stream;
+  const hipFunction_t jf = compiled(code, code_len, consts, JIT_POINTWISE64);
+  g_last_eval_compiled = jf ? 1 : 0;
   Events ev;
   double ms = 0.0;
   for (uint32_t p0 = 0; p0 < n_points; p0 += chunk) {
@@ -220,9 +248,12 @@ int gpum_hip_loglik_pointwise(const uint32_t* code, size_t code_len,
     a.points = S.points.p + (size_t)p0 * n_params;
     a.n_points = np;
     a.out = d_out.p;
-    const uint64_t blocks = (uint64_t)((n_obs + 255u) / 256u) * np;
+    const uint64_t blocks =
+        (uint64_t)((n_obs + kGradSliceObs - 1) / kGradSliceObs) * np;
     hipError_t e = hipEventRecord(ev.e0, st);
-    if (e == hipSuccess) e = launch_pointwise64(a, grid_for(blocks), st);
+    if (e == hipSuccess)
+      e = jf ? launch_compiled(jf, a, grid_for(blocks), st)
+             : launch_pointwise64(a, grid_for(blocks), st);
     if (e == hipSuccess) e = hipEventRecord(ev.e1, st);
     if (e == hipSuccess)
       e = hipMemcpyAsync(out + (size_t)p0 * n_obs, d_out.p,
@@ -246,6 +277,7 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
                         double* out) {
   std::lock_guard<std::mutex> lk(rt::g_mu);
   g_last_eval_ms = 0.0;
+  g_last_eval_compiled = 0;
   if (!rt::device_init()) return 1;
   hipSetDevice(g_dev.id);
   int depth = 0;
@@ -285,6 +317,8 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
       !d_out.alloc((size_t)n_points * n_params, "allocating the output"))
     return 1;
   hipStream_t st = g_dev.stream;
+  const hipFunction_t jf = compiled(code, code_len, consts, JIT_GRAD64);
+  g_last_eval_compiled = jf ? 1 : 0;
   Events ev;
   hipError_t e = hipEventRecord(ev.e0, st);
   for (uint32_t p0 = 0; e == hipSuccess && p0 < n_points; p0 += chunk) {
@@ -294,7 +328,8 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
     a.n_points = np;
     a.out = d_part.p;
     a.n_slices = ns;
-    e = launch_grad64(a, grid_for((uint64_t)ns * np), st);
+    e = jf ? launch_compiled(jf, a, grid_for((uint64_t)ns * np), st)
+           : launch_grad64(a, grid_for((uint64_t)ns * np), st);
     if (e == hipSuccess)
       e = launch_grad64_finish(d_part.p, np, n_params, ns,
                                d_out.p + (size_t)p0 * n_params, nullptr, st);
@@ -313,5 +348,12 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
 }
 
 double gpum_hip_last_eval_ms(void) { return g_last_eval_ms; }
+
+int gpum_hip_last_eval_compiled(void) { return g_last_eval_compiled; }
+
+void gpum_hip_set_eval_jit(int32_t enable) {
+  std::lock_guard<std::mutex> lk(rt::g_mu);
+  g_eval_jit = enable != 0;
+}
 
 }  // extern "C"

@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#if !defined(__HIPCC_RTC__)
 #include <vector>
+#endif
 
 namespace gpum {
 
@@ -47,11 +49,21 @@ struct Eval64Args {
   uint32_t n_slots;      // gradient: values kept for the backward pass
 };
 
+#if !defined(__HIPCC_RTC__)
 // The program with each instruction's operand positions and kept-value
 // slots (host; the program must be valid, validate_program).  *n_slots
 // (may be NULL): slots used.
 std::vector<TapeOp> make_tape(const uint32_t* code, size_t n_ops,
                               uint32_t* n_slots = nullptr);
+#endif
+
+#if !defined(__HIPCC_RTC__)
+// The program's kernel of jit kind `kind` (JIT_POINTWISE64 / JIT_GRAD64)
+// compiled per program, or nullptr when compiled FP64 evaluation is
+// switched off (gpum_hip_set_eval_jit) or unavailable in this process: the
+// caller then runs the interpreter, which gives the same values.
+hipFunction_t eval64_compiled_kernel(const uint32_t* code, size_t code_len,
+                                     const double* consts, int kind);
 
 // out[p * n_obs + i] = program at point p, observation i.
 hipError_t launch_pointwise64(const Eval64Args& a, uint32_t grid,
@@ -65,5 +77,6 @@ hipError_t launch_grad64(const Eval64Args& a, uint32_t grid, hipStream_t st);
 hipError_t launch_grad64_finish(const double* partials, uint32_t n_points,
                                 uint32_t n_params, uint32_t n_slices,
                                 double* out, double* value, hipStream_t st);
+#endif
 
 }  // namespace gpum

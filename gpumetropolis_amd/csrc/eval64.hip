@@ -8,30 +8,23 @@
 // multiply-add), so values agree with the reference's CPU path up to the
 // last-place differences of exp / log / pow between math libraries.
 //
-// The program is wave-uniform: each instruction is read with a scalar load,
-// every lane evaluates it for its own observation.  Data reads are coalesced
-// (consecutive lanes, consecutive observations of a column).
+// The interpreter's program is wave-uniform: each instruction is read with
+// a scalar load, every lane evaluates it for its own observation.  Data
+// reads are coalesced (consecutive lanes, consecutive observations of a
+// column).  The kernel bodies are in eval64_kernels.h, shared with the
+// kernels compiled per program.
 #include "eval64.h"
 #include "eval64_device.h"
+#include "eval64_kernels.h"
 #include "math64.h"
 #include "vm.h"
-
-#pragma clang fp contract(off)
 
 namespace gpum {
 namespace {
 
 template <int MAXD>
 __global__ __launch_bounds__(256) void pointwise64_kernel(Eval64Args a) {
-  const uint32_t nb = (a.n_obs + 255u) / 256u;
-  const uint64_t total = (uint64_t)nb * a.n_points;
-  for (uint64_t b = blockIdx.x; b < total; b += gridDim.x) {
-    const uint32_t p = (uint32_t)(b / nb);
-    const uint32_t i = (uint32_t)(b % nb) * 256u + threadIdx.x;
-    const uint32_t ic = i < a.n_obs ? i : a.n_obs - 1;
-    const double x = eval_obs64<MAXD>(a, a.points + (size_t)p * a.n_params, ic);
-    if (i < a.n_obs) a.out[(size_t)p * a.n_obs + i] = x;
-  }
+  pointwise64_body<InterpEval64<MAXD, 1>>(a);
 }
 
 // At least two waves per SIMD (at most 256 VGPRs): with one the FP64
@@ -39,40 +32,7 @@ __global__ __launch_bounds__(256) void pointwise64_kernel(Eval64Args a) {
 // (logistic-16, 33 points x 1e6 rows: 50 ms, 28.7 ms, 176 ms).
 template <int MAXD, int MAXS>
 __global__ __launch_bounds__(256, 2) void grad64_kernel(Eval64Args a) {
-  __shared__ double red[4][kMaxParams + 1];
-  const uint32_t ns = a.n_slices;
-  const uint64_t total = (uint64_t)ns * a.n_points;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (uint64_t b = blockIdx.x; b < total; b += gridDim.x) {
-    const uint32_t p = (uint32_t)(b / ns), s = (uint32_t)(b % ns);
-    const double* P = a.points + (size_t)p * a.n_params;
-    double g[kMaxParams];
-#pragma unroll
-    for (int k = 0; k < kMaxParams; ++k) g[k] = 0.0;
-    double val = 0.0;
-    const uint32_t i0 = s * kGradSliceObs + threadIdx.x;
-    for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
-      const uint32_t i = i0 + j * 256u;
-      if (i < a.n_obs) val += grad_obs64<MAXD, MAXS>(a, P, i, g);
-    }
-    // Wave sums (butterfly), then the four waves in order; slot kMaxParams
-    // carries the value.
-#pragma unroll
-    for (int k = 0; k <= kMaxParams; ++k) {
-      double x = k < kMaxParams ? g[k] : val;
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-      if (lane == 0) red[w][k] = x;
-    }
-    __syncthreads();
-    const uint32_t np = a.n_params;
-    if (threadIdx.x <= np) {
-      const int k = threadIdx.x < np ? (int)threadIdx.x : kMaxParams;
-      const double t = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
-      a.out[((size_t)p * ns + s) * (np + 1) + threadIdx.x] = t;
-    }
-    __syncthreads();
-  }
+  grad64_body<InterpEval64<MAXD, MAXS>>(a);
 }
 
 __global__ void grad64_finish_kernel(const double* part, uint32_t n_points,

@@ -1,0 +1,92 @@
+// eval64_kernels.h -- the FP64 batch-evaluation kernel bodies, generic over
+// the per-observation evaluator E:
+//   E::value(a, P, i)      the program at point P, observation i;
+//   E::grad(a, P, i, g)    the same value, d/d param k added into g[k].
+// The interpreter (InterpEval64) and code generated per program share them.
+//
+// A work item is (point, slice of kSliceObs observations): 256 threads, 16
+// observations each, the point fixed across a thread's loop -- so in
+// generated code the compiler computes parameter-only subexpressions once
+// per item (the arithmetic, and so every result, is unchanged).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eval64.h"
+#include "eval64_device.h"
+#include "vm.h"
+
+namespace gpum {
+
+// out[p * n_obs + i] = the program at point p, observation i.
+template <class E>
+__device__ __forceinline__ void pointwise64_body(const Eval64Args& a) {
+  const uint32_t ns = (a.n_obs + kGradSliceObs - 1) / kGradSliceObs;
+  const uint64_t total = (uint64_t)ns * a.n_points;
+  for (uint64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const uint32_t p = (uint32_t)(b / ns), s = (uint32_t)(b % ns);
+    const double* P = a.points + (size_t)p * a.n_params;
+    double* out = a.out + (size_t)p * a.n_obs;
+    const uint32_t i0 = s * kGradSliceObs + threadIdx.x;
+    for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
+      const uint32_t i = i0 + j * 256u;
+      if (i < a.n_obs) out[i] = E::value(a, P, i);
+    }
+  }
+}
+
+// partials[(p * n_slices + s) * (n_params + 1) + k]: the gradient (k <
+// n_params) and the value (k = n_params) summed over slice s at point p.
+// Lanes, then waves (butterfly), then the four waves in order.
+template <class E>
+__device__ __forceinline__ void grad64_body(const Eval64Args& a) {
+  __shared__ double red[4][kMaxParams + 1];
+  const uint32_t ns = a.n_slices;
+  const uint64_t total = (uint64_t)ns * a.n_points;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (uint64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const uint32_t p = (uint32_t)(b / ns), s = (uint32_t)(b % ns);
+    const double* P = a.points + (size_t)p * a.n_params;
+    double g[kMaxParams];
+#pragma unroll
+    for (int k = 0; k < kMaxParams; ++k) g[k] = 0.0;
+    double val = 0.0;
+    const uint32_t i0 = s * kGradSliceObs + threadIdx.x;
+    for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
+      const uint32_t i = i0 + j * 256u;
+      if (i < a.n_obs) val += E::grad(a, P, i, g);
+    }
+#pragma unroll
+    for (int k = 0; k <= kMaxParams; ++k) {
+      double x = k < kMaxParams ? g[k] : val;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+      if (lane == 0) red[w][k] = x;
+    }
+    __syncthreads();
+    const uint32_t np = a.n_params;
+    if (threadIdx.x <= np) {
+      const int k = threadIdx.x < np ? (int)threadIdx.x : kMaxParams;
+      const double t = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+      a.out[((size_t)p * ns + s) * (np + 1) + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// The interpreter: MAXD bounds the stack depth, MAXS the values the
+// gradient keeps.
+template <int MAXD, int MAXS>
+struct InterpEval64 {
+  static __device__ __forceinline__ double value(const Eval64Args& a,
+                                                 const double* P, uint32_t i) {
+    return eval_obs64<MAXD>(a, P, i);
+  }
+  static __device__ __forceinline__ double grad(const Eval64Args& a,
+                                                const double* P, uint32_t i,
+                                                double (&g)[kMaxParams]) {
+    return grad_obs64<MAXD, MAXS>(a, P, i, g);
+  }
+};
+
+}  // namespace gpum

@@ -688,12 +688,14 @@ int gpum_hip_jit_check(const uint32_t* code, size_t code_len,
                                                   prior_consts);
   const std::string src = jit_evaluator_source(L, prior.data(), prior.size(),
                                                prior_consts, v.ns, v.opt, v.cw);
+  const std::string src64 = jit_eval64_source(code, code_len, consts);
   double total = 0.0;
   for (int k = 0; k < JIT_NUM_KINDS; ++k) {
     if (!(kinds & (1u << k))) continue;
     std::string err;
     double ms = 0.0;
-    if (!jit_compile_only(src, (JitKind)k, &err, &ms)) {
+    if (!jit_compile_only(k < kJitChainKinds ? src : src64, (JitKind)k, &err,
+                          &ms)) {
       set_err("%s", err.c_str());
       return 1;
     }

@@ -56,6 +56,12 @@ const char* kKernels[JIT_NUM_KINDS] = {
     "gpum_jit_kernel(const gpum::BatchWork* w, const gpum::ChainArgs* a,\n"
     "                const gpum::ChainHot* h, gpum::BatchStep s) {\n"
     "  gpum::batch_body<gpum::JitEval>(w, a, h, s);\n}\n",
+    "extern \"C\" __global__ void __launch_bounds__(256)\n"
+    "gpum_jit_kernel(gpum::Eval64Args a) {\n"
+    "  gpum::pointwise64_body<gpum::JitEval64>(a);\n}\n",
+    "extern \"C\" __global__ void __launch_bounds__(256, 2)\n"
+    "gpum_jit_kernel(gpum::Eval64Args a) {\n"
+    "  gpum::grad64_body<gpum::JitEval64>(a);\n}\n",
 };
 
 struct Entry {
@@ -217,6 +223,100 @@ std::string fp64_program(const uint32_t* code, size_t n_words,
 }
 
 }  // namespace
+
+std::string jit_eval64_source(const uint32_t* code, size_t n_words,
+                              const double* consts) {
+  const size_t n = n_words / 2;
+  std::vector<uint32_t> lhs(n, 0), rhs(n, 0), st;
+  std::ostringstream fwd;
+  auto v = [](size_t k) { return "v" + std::to_string(k); };
+  auto adj = [](size_t k) { return "a" + std::to_string(k); };
+  for (size_t pc = 0; pc < n; ++pc) {
+    const uint32_t op = code[2 * pc], arg = code[2 * pc + 1];
+    fwd << "    const double " << v(pc) << " = ";
+    if (op == OP_PUSH_CONST) {
+      fwd << f64_literal(consts[arg]);
+    } else if (op == OP_PUSH_PARAM) {
+      fwd << "P[" << arg << "]";
+    } else if (op == OP_PUSH_DATA) {
+      fwd << "a.data[(size_t)" << arg << " * a.n_obs + i]";
+    } else if (op == OP_NEG || op == OP_EXP || op == OP_LOG || op == OP_SQRT ||
+               op == OP_LGAMMA) {
+      lhs[pc] = st.back();
+      st.pop_back();
+      const char* f = op == OP_NEG ? "-" : op == OP_EXP ? "exp" : op == OP_LOG ? "log"
+                      : op == OP_SQRT ? "sqrt" : "lgamma64";
+      fwd << f << "(" << v(lhs[pc]) << ")";
+    } else {
+      rhs[pc] = st.back();
+      st.pop_back();
+      lhs[pc] = st.back();
+      st.pop_back();
+      const std::string l = v(lhs[pc]), r = v(rhs[pc]);
+      switch (op) {
+        case OP_ADD: fwd << l << " + " << r; break;
+        case OP_SUB: fwd << l << " - " << r; break;
+        case OP_MUL: fwd << l << " * " << r; break;
+        case OP_DIV: fwd << l << " / " << r; break;
+        default: fwd << "pow(" << l << ", opaque64(" << r << "))"; break;
+      }
+    }
+    fwd << ";\n";
+    st.push_back((uint32_t)pc);
+  }
+  const std::string root = n ? v(n - 1) : "0.0";
+  // Backward: each value has one consumer, which defines its adjoint; a
+  // zero adjoint propagates zeros (cpu_native.rs:218-277).
+  std::ostringstream bwd;
+  if (n) bwd << "    const double " << adj(n - 1) << " = 1.0;\n";
+  for (size_t k = n; k-- > 0;) {
+    const uint32_t op = code[2 * k], arg = code[2 * k + 1];
+    const std::string g = adj(k), live = "(" + adj(k) + " != 0.0)";
+    const std::string l = v(lhs[k]), r = v(rhs[k]), me = v(k);
+    auto def = [&](uint32_t t, const std::string& e) {
+      bwd << "    const double " << adj(t) << " = " << live << " ? (" << e
+          << ") : 0.0;\n";
+    };
+    switch (op) {
+      case OP_PUSH_CONST:
+      case OP_PUSH_DATA: break;
+      case OP_PUSH_PARAM:
+        bwd << "    if " << live << " g[" << arg << "] += " << g << ";\n";
+        break;
+      case OP_ADD: def(lhs[k], g); def(rhs[k], g); break;
+      case OP_SUB: def(lhs[k], g); def(rhs[k], "-" + g); break;
+      case OP_MUL: def(lhs[k], g + " * " + r); def(rhs[k], g + " * " + l); break;
+      case OP_DIV:
+        def(lhs[k], g + " / " + r);
+        def(rhs[k], "-(" + g + " * " + me + " / " + r + ")");
+        break;
+      case OP_NEG: def(lhs[k], "-" + g); break;
+      case OP_EXP: def(lhs[k], g + " * " + me); break;
+      case OP_LOG: def(lhs[k], g + " / " + l); break;
+      case OP_SQRT: def(lhs[k], g + " * 0.5 / " + me); break;
+      case OP_LGAMMA: def(lhs[k], g + " * digamma64(" + l + ")"); break;
+      default:
+        def(lhs[k], g + " * " + r + " * pow(" + l + ", opaque64(" + r + " - 1.0))");
+        def(rhs[k], l + " > 0.0 ? " + g + " * " + me + " * log(" + l + ") : 0.0");
+        break;
+    }
+  }
+  std::ostringstream o;
+  o << "#include \"eval64_kernels.h\"\n#include \"step_common.h\"\n"
+    << "namespace gpum {\n"
+    << "struct JitEval64 {\n"
+    << "  static __device__ __forceinline__ double value(const Eval64Args& a,\n"
+    << "      const double* __restrict__ P, uint32_t i) {\n"
+    << "#pragma clang fp contract(off)\n"
+    << fwd.str() << "    return " << root << ";\n  }\n"
+    << "  static __device__ __forceinline__ double grad(const Eval64Args& a,\n"
+    << "      const double* __restrict__ P, uint32_t i,\n"
+    << "      double (&g)[kMaxParams]) {\n"
+    << "#pragma clang fp contract(off)\n"
+    << fwd.str() << bwd.str() << "    return " << root << ";\n  }\n"
+    << "};\n}  // namespace gpum\n";
+  return o.str();
+}
 
 std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
                                  size_t n_prior, const double* prior_consts,

@@ -17,7 +17,11 @@ namespace gpum {
 struct Lowered;
 
 enum JitKind { JIT_CHAIN_STREAM = 0, JIT_CHAIN_RESIDENT, JIT_POINTS_STREAM,
-               JIT_POINTS_RESIDENT, JIT_BATCH, JIT_NUM_KINDS };
+               JIT_POINTS_RESIDENT, JIT_BATCH, JIT_POINTWISE64, JIT_GRAD64,
+               JIT_NUM_KINDS };
+// Kinds compiled from jit_evaluator_source; JIT_POINTWISE64 and JIT_GRAD64
+// from jit_eval64_source.
+constexpr int kJitChainKinds = JIT_POINTWISE64;
 
 // Source of the evaluator for a lowered program run with variant
 // (ns, opt, cw): ns spill slots, opt observations per lane per chunk, cw
@@ -26,6 +30,13 @@ enum JitKind { JIT_CHAIN_STREAM = 0, JIT_CHAIN_RESIDENT, JIT_POINTS_STREAM,
 std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
                                  size_t n_prior, const double* prior_consts,
                                  int ns, int opt, int cw);
+
+// Source of the FP64 evaluator of a reference program (valid,
+// validate_program): its value and, by reverse mode, its gradient as
+// straight-line code, every operation rounded as the interpreter's
+// (eval64_device.h).  Instantiated into the bodies of eval64_kernels.h.
+std::string jit_eval64_source(const uint32_t* code, size_t n_words,
+                              const double* consts);
 
 // The kernel of `kind` for an evaluator source (compiled on first use).
 // On failure returns nullptr and fills *err.

@@ -210,6 +210,19 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
  * library's stream; copies excluded).  No counterpart in the reference. */
 double gpum_hip_last_eval_ms(void);
 
+/* 1 if that call ran kernels compiled for its program (hiprtc, cached per
+ * process by program), 0 if it ran the interpreter.  No counterpart in the
+ * reference. */
+int gpum_hip_last_eval_compiled(void);
+
+/* Let the FP64 evaluation -- gpum_hip_loglik_pointwise,
+ * gpum_hip_grad_batch and the gradient of MALA (proposal_mode 3) -- compile
+ * kernels per program (1, the default; it falls back to the interpreter
+ * where the process cannot) or always interpret (0).  Both give the same
+ * values, operation for operation.  Process-wide.  No counterpart in the
+ * reference. */
+void gpum_hip_set_eval_jit(int32_t enable);
+
 /* A model with its lowered program and data resident on the device, for
  * repeated batches without re-uploading the data (the reference uploads
  * every call, chain_kernel.rs:393-413).  precompute_data / fuse: -1
@@ -238,7 +251,8 @@ int gpum_hip_lower_info(const uint32_t* code, size_t code_len,
  * gpum_hip_model_set_jit would compile for this program and prior with
  * chains_per_wave chains per wave (0 or 1: one).  kinds: bit k selects
  * kernel kind k (0 chain streaming, 1 chain resident, 2 points streaming,
- * 3 points resident, 4 batched).  compile_ms (may be NULL): total compile
+ * 3 points resident, 4 batched, 5 FP64 pointwise, 6 FP64 gradient).
+ * compile_ms (may be NULL): total compile
  * time.  No counterpart in the reference; it checks that a model's compiled
  * kernels build on a host without a GPU. */
 int gpum_hip_jit_check(const uint32_t* code, size_t code_len,

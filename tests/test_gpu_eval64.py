@@ -28,6 +28,14 @@ CASES = {
 }
 
 
+@pytest.fixture(params=[True, False], ids=["compiled", "interpreter"])
+def eval_mode(request, hip):
+    """Run the test on kernels compiled per program and on the interpreter."""
+    hip.set_eval_jit(request.param)
+    yield request.param
+    hip.set_eval_jit(True)
+
+
 def _points(rng, centre, scale, n):
     centre = np.asarray(centre, float)
     return centre + scale * rng.standard_normal((n, centre.size))
@@ -43,7 +51,7 @@ def _grad_scale(orc, code, consts, n_params, data, pts):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-def test_pointwise_matches_fp64(hip, orc, name):
+def test_pointwise_matches_fp64(hip, orc, eval_mode, name):
     kw, centre, scale = CASES[name]
     w = workloads.by_name(name, **kw)
     m = w.model
@@ -60,7 +68,7 @@ def test_pointwise_matches_fp64(hip, orc, name):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-def test_gradient_matches_fp64(hip, orc, name):
+def test_gradient_matches_fp64(hip, orc, eval_mode, name):
     kw, centre, scale = CASES[name]
     w = workloads.by_name(name, **kw)
     m = w.model
@@ -105,7 +113,7 @@ OPCODE_FORMS = [
 
 @pytest.mark.parametrize("n_obs", [1, 257, 4097])
 @pytest.mark.parametrize("form", OPCODE_FORMS, ids=lambda f: f[0])
-def test_opcodes_and_ragged_sizes(hip, orc, form, n_obs):
+def test_opcodes_and_ragged_sizes(hip, orc, eval_mode, form, n_obs):
     formula, centre, exact = form
     m = dsl.gpum_model(formula, ["a", "b"], ["y", "x"])
     rng = np.random.default_rng(n_obs)
@@ -129,6 +137,30 @@ def test_opcodes_and_ragged_sizes(hip, orc, form, n_obs):
     scale_k = _grad_scale(orc, m.loglik.code, m.loglik.consts, 2, data, pts)
     err = np.abs(g - g_ref)[finite] / scale_k[finite]
     assert err.size == 0 or err.max() <= GRAD_TOL, (formula, err.max())
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_compiled_equals_interpreter(hip, name):
+    """The kernels compiled per program run the interpreter's operations in
+    its order: identical bits, for the matrix and the gradient."""
+    kw, centre, scale = CASES[name]
+    w = workloads.by_name(name, **kw)
+    m = w.model
+    pts = _points(np.random.default_rng(6), centre, scale, 7)
+    args = (m.loglik.code, m.loglik.consts, m.n_params, w.data, w.n_cols,
+            w.n_obs, pts)
+    out = {}
+    for jit in (True, False):
+        hip.set_eval_jit(jit)
+        pw = hip.loglik_pointwise(*args)
+        c1 = hip.last_eval_compiled()
+        g = hip.grad_batch(*args)
+        c2 = hip.last_eval_compiled()
+        assert c1 == c2 == jit
+        out[jit] = (pw, g)
+    hip.set_eval_jit(True)
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][1], out[False][1])
 
 
 def test_pointwise_over_device_chunks(hip, orc):

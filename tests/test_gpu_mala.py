@@ -31,7 +31,9 @@ def _chol_factors(rng, C, d, scale):
     ("gamma", [np.log(2.5), np.log(0.7)], 0.01, True),
     ("logistic16", [0.0] * 16, 0.02, True),
 ])
-def test_mala_matches_oracle_draw_for_draw(hip, orc, name, centre, sd, chol):
+@pytest.mark.parametrize("compiled", [True, False], ids=["compiled", "interpreter"])
+def test_mala_matches_oracle_draw_for_draw(hip, orc, name, centre, sd, chol,
+                                           compiled):
     w = workloads.by_name(name, n=3000)
     m = w.model
     d = m.n_params
@@ -46,7 +48,11 @@ def test_mala_matches_oracle_draw_for_draw(hip, orc, name, centre, sd, chol):
     print(f"[mala] {name} chol={chol} C={C} T={T}", flush=True)
     args = (m.loglik.code, m.loglik.consts, m.prior.code, m.prior.consts, d,
             data, w.n_cols, w.n_obs, init, sdm, temps, T, seed)
-    g = hip.run_chains(*args, proposal_mode=3, proposal_l=L, trace=True)
+    hip.set_eval_jit(compiled)
+    try:
+        g = hip.run_chains(*args, proposal_mode=3, proposal_l=L, trace=True)
+    finally:
+        hip.set_eval_jit(True)
     o = orc.run(*args, proposal_mode=3, proposal_l=L, trace=True)
     rep = replay_compare(g, o, orc, seed, temps, 3, d, prop_rtol=1e-10)
     assert rep.chains_diverged == 0, rep

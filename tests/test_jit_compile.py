@@ -15,7 +15,9 @@ def lib():
     ("gamma", 2, _native.JIT_KINDS),
     ("gamma", 4, ("chain_stream",)),
     ("logistic16", 1, ("chain_stream", "points_stream", "points_resident")),
-    ("mixture", 1, ("chain_stream", "chain_resident")),
+    ("mixture", 1, ("chain_stream", "chain_resident", "pointwise64",
+                    "grad64")),
+    ("logistic16", 1, ("grad64",)),
 ])
 def test_compiled_kernels_build(lib, name, cw, kinds):
     w = workloads.by_name(name, n=64)
