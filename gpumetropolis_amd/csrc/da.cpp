@@ -438,9 +438,9 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
             "instructions", kMaxTapeOps);
     return 1;
   }
-  if (m->prior_ref.size() / 2 > kMalaMaxPriorOps) {
+  if (m->prior_ref.size() / 2 > kMaxTapeOps) {
     set_err("Langevin proposals differentiate priors of at most %u "
-            "instructions", kMalaMaxPriorOps);
+            "instructions", kMaxTapeOps);
     return 1;
   }
   if (!ensure_eval64(m)) return 1;
@@ -449,7 +449,7 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   const size_t ncp = (size_t)C * np;
   const size_t n_draws = (size_t)n_iter * ncp;
   DevBuf<double> theta, prop, cur, acc, passes, temps, sd, L, gx, gy, mean_x,
-      ll, part;
+      ll, part, vpr, gpr, ppart;
   DevBuf<float> draws, draws_it;
   DevBuf<double> tr_plp, tr_prop, tr_ll;
   DevBuf<uint8_t> tr_stage;
@@ -461,6 +461,8 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
             mean_x.alloc(ncp, "allocate state") && ll.alloc(C, "allocate state") &&
             part.alloc(std::max<size_t>(1, (size_t)C * ns * (np + 1)),
                        "allocate partial sums") &&
+            vpr.alloc(C, "allocate state") && gpr.alloc(ncp, "allocate state") &&
+            ppart.alloc((size_t)C * (np + 1), "allocate partial sums") &&
             temps.upload(temperatures, C, "upload temperatures") &&
             sd.upload(proposal_sd, ncp, "upload proposal_sd") &&
             draws.alloc(n_draws, "allocate draws") &&
@@ -499,8 +501,8 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   a.gy = gy.p;
   a.mean_x = mean_x.p;
   a.ll_prop = ll.p;
-  a.prior_tape = m->tape_prior.p;
-  a.n_prior_ops = (uint32_t)(m->prior_ref.size() / 2);
+  a.v_prior = vpr.p;
+  a.g_prior = gpr.p;
   a.draws = draws.p;
   a.draws_it = draws_it.p;
   a.trace_plp = X.trace_plp ? tr_plp.p : nullptr;
@@ -529,6 +531,42 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   // batch evaluation's (same values either way).
   const hipFunction_t gjf = eval64_compiled_kernel(
       m->code_ref.data(), m->code_ref.size(), m->consts_ref.data(), JIT_GRAD64);
+  // The prior: a program without data, one "observation" per point.
+  const uint32_t n_prior_ops = (uint32_t)(m->prior_ref.size() / 2);
+  Eval64Args pe{};
+  pe.prog = m->tape_prior.p;
+  pe.n_ops = n_prior_ops;
+  pe.n_params = np;
+  pe.consts = m->prior_consts.p;
+  pe.n_obs = 1;
+  pe.n_points = C;
+  pe.out = ppart.p;
+  pe.n_slices = 1;
+  pe.depth = m->prior_depth;
+  pe.n_slots = m->slots_prior;
+  const hipFunction_t pjf = eval64_compiled_kernel(
+      m->prior_ref.data(), m->prior_ref.size(), m->prior_consts_h.data(),
+      JIT_GRAD64);
+  const uint32_t pgrid = std::min<uint32_t>(C, (uint32_t)g_dev.n_cu * 8u);
+  auto prior_term = [&](const double* points) -> hipError_t {
+    if (n_prior_ops == 0) {
+      hipError_t r = hipMemsetAsync(vpr.p, 0, C * sizeof(double), st);
+      return r == hipSuccess ? hipMemsetAsync(gpr.p, 0, ncp * sizeof(double), st)
+                             : r;
+    }
+    Eval64Args e = pe;
+    e.points = points;
+    hipError_t r;
+    if (pjf) {
+      void* kargs[] = {(void*)&e};
+      r = launch_module(pjf, dim3(pgrid), dim3(256), 0, st, kargs);
+    } else {
+      r = launch_grad64(e, pgrid, st);
+    }
+    return r == hipSuccess
+               ? launch_grad64_finish(ppart.p, C, np, 1, gpr.p, vpr.p, st)
+               : r;
+  };
   auto data_term = [&](const double* points, double* g) -> hipError_t {
     if (n_ops == 0) {
       hipError_t r = hipMemsetAsync(ll.p, 0, C * sizeof(double), st);
@@ -550,7 +588,8 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   auto chk = [&](hipError_t r) {
     if (e == hipSuccess) e = r;
   };
-  chk(data_term(theta.p, gx.p));
+  chk(data_term(theta.p, gy.p));
+  chk(prior_term(theta.p));
   chk(launch_mala_init(d_args.p, C, st));
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -559,6 +598,7 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   for (uint32_t it = 0; it < n_iter && e == hipSuccess; ++it) {
     chk(launch_mala_propose(d_args.p, C, it, st));
     chk(data_term(prop.p, gy.p));
+    chk(prior_term(prop.p));
     chk(launch_mala_accept(d_args.p, C, it, st));
   }
   if (e == hipSuccess) e = launch_da_draws(draws_it.p, draws.p, n_iter, C * np, st);

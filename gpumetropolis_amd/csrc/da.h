@@ -5,7 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "eval64.h"
+#include "vm.h"
 
 namespace gpum {
 
@@ -52,14 +52,15 @@ struct DaArgs {
   uint16_t* Xb;              // network input rows (bf16, 16 per row)
   const float* s_raw;        // network output per row
   // Langevin proposals (mode 3, all FP64): gradients of the log-posterior
-  // at the states and at the proposals, the proposals' means, the
-  // proposals' data terms, and the prior as a tape.
+  // at the states, the data term's value and gradient at the points just
+  // evaluated (ll_prop, gy), the prior's (v_prior, g_prior), and the
+  // proposals' means.
   double* gx;
   double* gy;
   double* mean_x;
   double* ll_prop;
-  const TapeOp* prior_tape;
-  uint32_t n_prior_ops;
+  double* v_prior;
+  double* g_prior;
   // outputs
   float* draws_it;       // iteration-major staging: (t * np + j) * C + c
   float* draws;          // reference layout t + n_iter (c + C j)
@@ -71,15 +72,14 @@ struct DaArgs {
 };
 
 // Langevin steps (mode 3), one thread per chain: the log-posterior and
-// gradient at the states from the data terms in ll_prop / gx; the
-// proposals; the decisions from the data terms in ll_prop / gy.
+// gradient at the states from the data and prior terms in ll_prop / gy and
+// v_prior / g_prior; the proposals; the decisions from the same terms at
+// the proposals.
 hipError_t launch_mala_init(const DaArgs* A, uint32_t n_chains, hipStream_t s);
 hipError_t launch_mala_propose(const DaArgs* A, uint32_t n_chains, uint32_t t,
                                hipStream_t s);
 hipError_t launch_mala_accept(const DaArgs* A, uint32_t n_chains, uint32_t t,
                               hipStream_t s);
-// Longest prior the Langevin kernels differentiate.
-constexpr uint32_t kMalaMaxPriorOps = 256;
 
 hipError_t launch_da_init(const DaArgs* A, uint32_t n_chains, hipStream_t s);
 hipError_t launch_da_init_finish(const DaArgs* A, uint32_t n_chains,

@@ -29,7 +29,6 @@
 
 #include "bf16.h"
 #include "da.h"
-#include "eval64_device.h"
 #include "mlp.h"
 #include "philox.h"
 #include "step_common.h"
@@ -335,28 +334,28 @@ __device__ double mala_log_q(const DaArgs* __restrict__ A, uint32_t c,
   return q;
 }
 
-// The prior's value at p, its gradient added into g (after the data
-// term's, as the reference adds them, cpu_native.rs:285-298).
-__device__ double mala_prior(const DaArgs* __restrict__ A, const double* p,
-                             double (&g)[kMaxParams]) {
-  if (A->n_prior_ops == 0) return 0.0;
-  Eval64Args pa{};
-  pa.prog = A->prior_tape;
-  pa.n_ops = A->n_prior_ops;
-  pa.n_params = A->n_params;
-  pa.consts = A->prior_consts;
-  return grad_obs64<kMaxStack, kMalaMaxPriorOps>(pa, p, 0u, g);
+// The log-posterior's gradient at a point: the data term's, then the
+// prior's added (as the reference adds the prior's adjoints after the data
+// loop, cpu_native.rs:285-298).
+__device__ __forceinline__ void mala_gradient(const DaArgs* __restrict__ A,
+                                              uint32_t c,
+                                              double (&g)[kMaxParams]) {
+#pragma clang fp contract(off)
+  const uint32_t np = A->n_params;
+  for (int k = 0; k < kMaxParams; ++k)
+    g[k] = (uint32_t)k < np ? A->gy[(size_t)c * np + k] +
+                                  A->g_prior[(size_t)c * np + k]
+                            : 0.0;
 }
 
 __global__ void mala_init_kernel(const DaArgs* __restrict__ A) {
+#pragma clang fp contract(off)
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= A->n_chains) return;
   const uint32_t np = A->n_params;
   double g[kMaxParams];
-  for (int k = 0; k < kMaxParams; ++k)
-    g[k] = (uint32_t)k < np ? A->gx[(size_t)c * np + k] : 0.0;
-  const double pv = mala_prior(A, A->theta + (size_t)c * np, g);
-  A->cur[c] = A->ll_prop[c] + pv;
+  mala_gradient(A, c, g);
+  A->cur[c] = A->ll_prop[c] + A->v_prior[c];
   for (uint32_t k = 0; k < np; ++k) A->gx[(size_t)c * np + k] = g[k];
 }
 
@@ -399,10 +398,9 @@ __global__ void mala_accept_kernel(const DaArgs* __restrict__ A, uint32_t t) {
   const double* pr = A->prop + (size_t)c * np;
   double* th = A->theta + (size_t)c * np;
   double g[kMaxParams], my[kMaxParams];
-  for (int k = 0; k < kMaxParams; ++k)
-    g[k] = (uint32_t)k < np ? A->gy[(size_t)c * np + k] : 0.0;
+  mala_gradient(A, c, g);
   const double ll = A->ll_prop[c];
-  const double plp = ll + mala_prior(A, pr, g);
+  const double plp = ll + A->v_prior[c];
   mala_drift(A, c, g, my);
   for (uint32_t j = 0; j < np; ++j) my[j] += pr[j];
   const double lq = mala_log_q(A, c, th, my) -

@@ -1,13 +1,15 @@
 // eval64_kernels.h -- the FP64 batch-evaluation kernel bodies, generic over
 // the per-observation evaluator E:
-//   E::value(a, P, i)      the program at point P, observation i;
-//   E::grad(a, P, i, g)    the same value, d/d param k added into g[k].
+//   E::Hoisted, E::prepare(a, P)
+//                          what depends on the point only, computed once per
+//                          work item (generated code: parameter-only
+//                          subexpressions; the interpreter: nothing);
+//   E::value(a, h, P, i)   the program at point P, observation i;
+//   E::grad(a, h, P, i, g) the same value, d/d param k added into g[k].
 // The interpreter (InterpEval64) and code generated per program share them.
 //
-// A work item is (point, slice of kSliceObs observations): 256 threads, 16
-// observations each, the point fixed across a thread's loop -- so in
-// generated code the compiler computes parameter-only subexpressions once
-// per item (the arithmetic, and so every result, is unchanged).
+// A work item is (point, slice of kGradSliceObs observations): 256 threads,
+// 16 observations each, the point fixed across a thread's loop.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,10 +29,11 @@ __device__ __forceinline__ void pointwise64_body(const Eval64Args& a) {
     const uint32_t p = (uint32_t)(b / ns), s = (uint32_t)(b % ns);
     const double* P = a.points + (size_t)p * a.n_params;
     double* out = a.out + (size_t)p * a.n_obs;
+    const typename E::Hoisted h = E::prepare(a, P);
     const uint32_t i0 = s * kGradSliceObs + threadIdx.x;
     for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
       const uint32_t i = i0 + j * 256u;
-      if (i < a.n_obs) out[i] = E::value(a, P, i);
+      if (i < a.n_obs) out[i] = E::value(a, h, P, i);
     }
   }
 }
@@ -51,10 +54,11 @@ __device__ __forceinline__ void grad64_body(const Eval64Args& a) {
 #pragma unroll
     for (int k = 0; k < kMaxParams; ++k) g[k] = 0.0;
     double val = 0.0;
+    const typename E::Hoisted h = E::prepare(a, P);
     const uint32_t i0 = s * kGradSliceObs + threadIdx.x;
     for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
       const uint32_t i = i0 + j * 256u;
-      if (i < a.n_obs) val += E::grad(a, P, i, g);
+      if (i < a.n_obs) val += E::grad(a, h, P, i, g);
     }
 #pragma unroll
     for (int k = 0; k <= kMaxParams; ++k) {
@@ -78,11 +82,18 @@ __device__ __forceinline__ void grad64_body(const Eval64Args& a) {
 // gradient keeps.
 template <int MAXD, int MAXS>
 struct InterpEval64 {
+  struct Hoisted {};
+  static __device__ __forceinline__ Hoisted prepare(const Eval64Args&,
+                                                    const double*) {
+    return Hoisted{};
+  }
   static __device__ __forceinline__ double value(const Eval64Args& a,
+                                                 const Hoisted&,
                                                  const double* P, uint32_t i) {
     return eval_obs64<MAXD>(a, P, i);
   }
   static __device__ __forceinline__ double grad(const Eval64Args& a,
+                                                const Hoisted&,
                                                 const double* P, uint32_t i,
                                                 double (&g)[kMaxParams]) {
     return grad_obs64<MAXD, MAXS>(a, P, i, g);

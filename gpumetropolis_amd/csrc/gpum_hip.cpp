@@ -175,6 +175,7 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
   m->code_ref.assign(code, code + code_len);
   m->consts_ref.assign(consts, consts + n_consts);
   m->prior_ref.assign(prior_code, prior_code + prior_len);
+  m->prior_depth = (uint32_t)pdepth;
   m->n_cols_in = n_cols;
   // Columns: originals cast to FP32 (as the reference's GPU path casts,
   // chain_kernel.rs:501-505), then precomputed data-only subtrees evaluated
@@ -448,7 +449,8 @@ bool ensure_eval64(gpum_hip_model* m) {
   if (m->eval64_ready) return true;
   const std::vector<TapeOp> tl =
       make_tape(m->code_ref.data(), m->code_ref.size() / 2, &m->slots_ll);
-  const std::vector<TapeOp> tp = make_tape(m->prior_ref.data(), m->prior_ref.size() / 2);
+  const std::vector<TapeOp> tp =
+      make_tape(m->prior_ref.data(), m->prior_ref.size() / 2, &m->slots_prior);
   // The original columns as the kernels see them (FP32), widened.
   std::vector<double> d((size_t)m->n_cols_in * m->n_obs);
   if (!m->L.empty)

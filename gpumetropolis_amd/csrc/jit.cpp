@@ -228,42 +228,81 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
                               const double* consts) {
   const size_t n = n_words / 2;
   std::vector<uint32_t> lhs(n, 0), rhs(n, 0), st;
-  std::ostringstream fwd;
+  // hoist[k]: an operation whose operands depend on no data (constants,
+  // parameters, other such operations): computed once per point.
+  std::vector<char> data_dep(n, 0), hoist(n, 0);
   auto v = [](size_t k) { return "v" + std::to_string(k); };
   auto adj = [](size_t k) { return "a" + std::to_string(k); };
+  auto is_unary = [](uint32_t op) {
+    return op == OP_NEG || op == OP_EXP || op == OP_LOG || op == OP_SQRT ||
+           op == OP_LGAMMA;
+  };
   for (size_t pc = 0; pc < n; ++pc) {
-    const uint32_t op = code[2 * pc], arg = code[2 * pc + 1];
-    fwd << "    const double " << v(pc) << " = ";
-    if (op == OP_PUSH_CONST) {
-      fwd << f64_literal(consts[arg]);
-    } else if (op == OP_PUSH_PARAM) {
-      fwd << "P[" << arg << "]";
-    } else if (op == OP_PUSH_DATA) {
-      fwd << "a.data[(size_t)" << arg << " * a.n_obs + i]";
-    } else if (op == OP_NEG || op == OP_EXP || op == OP_LOG || op == OP_SQRT ||
-               op == OP_LGAMMA) {
-      lhs[pc] = st.back();
-      st.pop_back();
-      const char* f = op == OP_NEG ? "-" : op == OP_EXP ? "exp" : op == OP_LOG ? "log"
-                      : op == OP_SQRT ? "sqrt" : "lgamma64";
-      fwd << f << "(" << v(lhs[pc]) << ")";
-    } else {
-      rhs[pc] = st.back();
-      st.pop_back();
-      lhs[pc] = st.back();
-      st.pop_back();
-      const std::string l = v(lhs[pc]), r = v(rhs[pc]);
-      switch (op) {
-        case OP_ADD: fwd << l << " + " << r; break;
-        case OP_SUB: fwd << l << " - " << r; break;
-        case OP_MUL: fwd << l << " * " << r; break;
-        case OP_DIV: fwd << l << " / " << r; break;
-        default: fwd << "pow(" << l << ", opaque64(" << r << "))"; break;
+    const uint32_t op = code[2 * pc];
+    if (op == OP_PUSH_DATA) {
+      data_dep[pc] = 1;
+    } else if (op > OP_PUSH_DATA) {
+      if (is_unary(op)) {
+        lhs[pc] = st.back();
+        st.pop_back();
+        data_dep[pc] = data_dep[lhs[pc]];
+      } else {
+        rhs[pc] = st.back();
+        st.pop_back();
+        lhs[pc] = st.back();
+        st.pop_back();
+        data_dep[pc] = data_dep[lhs[pc]] | data_dep[rhs[pc]];
       }
+      hoist[pc] = !data_dep[pc];
     }
-    fwd << ";\n";
     st.push_back((uint32_t)pc);
   }
+  // The value of instruction pc as an expression of its operands' names.
+  auto expr = [&](size_t pc) -> std::string {
+    const uint32_t op = code[2 * pc], arg = code[2 * pc + 1];
+    const std::string l = v(lhs[pc]), r = v(rhs[pc]);
+    switch (op) {
+      case OP_PUSH_CONST: return f64_literal(consts[arg]);
+      case OP_PUSH_PARAM: return "P[" + std::to_string(arg) + "]";
+      case OP_PUSH_DATA:
+        return "a.data[(size_t)" + std::to_string(arg) + " * a.n_obs + i]";
+      case OP_ADD: return l + " + " + r;
+      case OP_SUB: return l + " - " + r;
+      case OP_MUL: return l + " * " + r;
+      case OP_DIV: return l + " / " + r;
+      case OP_NEG: return "-(" + l + ")";
+      case OP_EXP: return "exp(" + l + ")";
+      case OP_LOG: return "log(" + l + ")";
+      case OP_SQRT: return "sqrt(" + l + ")";
+      case OP_LGAMMA: return "lgamma64(" + l + ")";
+      default: return "pow(" + l + ", opaque64(" + r + "))";
+    }
+  };
+  // Once per point: the hoisted values and the parameter-only factors of
+  // their derivatives (digamma, and pow / log of a power's operands).
+  std::ostringstream fields, prep;
+  for (size_t pc = 0; pc < n; ++pc) {
+    const uint32_t op = code[2 * pc];
+    if (op == OP_PUSH_DATA) continue;
+    if (op <= OP_PUSH_PARAM || hoist[pc])
+      prep << "    const double " << v(pc) << " = " << expr(pc) << ";\n";
+    if (!hoist[pc]) continue;
+    fields << " double " << v(pc) << ";";
+    prep << "    h." << v(pc) << " = " << v(pc) << ";\n";
+    const std::string l = v(lhs[pc]), r = v(rhs[pc]);
+    if (op == OP_LGAMMA) {
+      fields << " double dg" << pc << ";";
+      prep << "    h.dg" << pc << " = digamma64(" << l << ");\n";
+    } else if (op == OP_POW) {
+      fields << " double pw" << pc << "; double lg" << pc << ";";
+      prep << "    h.pw" << pc << " = pow(" << l << ", opaque64(" << r
+           << " - 1.0));\n    h.lg" << pc << " = log(" << l << ");\n";
+    }
+  }
+  std::ostringstream fwd;
+  for (size_t pc = 0; pc < n; ++pc)
+    fwd << "    const double " << v(pc) << " = "
+        << (hoist[pc] ? "h." + v(pc) : expr(pc)) << ";\n";
   const std::string root = n ? v(n - 1) : "0.0";
   // Backward: each value has one consumer, which defines its adjoint; a
   // zero adjoint propagates zeros (cpu_native.rs:218-277).
@@ -273,6 +312,7 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
     const uint32_t op = code[2 * k], arg = code[2 * k + 1];
     const std::string g = adj(k), live = "(" + adj(k) + " != 0.0)";
     const std::string l = v(lhs[k]), r = v(rhs[k]), me = v(k);
+    const std::string kk = std::to_string(k);
     auto def = [&](uint32_t t, const std::string& e) {
       bwd << "    const double " << adj(t) << " = " << live << " ? (" << e
           << ") : 0.0;\n";
@@ -294,23 +334,34 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
       case OP_EXP: def(lhs[k], g + " * " + me); break;
       case OP_LOG: def(lhs[k], g + " / " + l); break;
       case OP_SQRT: def(lhs[k], g + " * 0.5 / " + me); break;
-      case OP_LGAMMA: def(lhs[k], g + " * digamma64(" + l + ")"); break;
-      default:
-        def(lhs[k], g + " * " + r + " * pow(" + l + ", opaque64(" + r + " - 1.0))");
-        def(rhs[k], l + " > 0.0 ? " + g + " * " + me + " * log(" + l + ") : 0.0");
+      case OP_LGAMMA:
+        def(lhs[k], g + " * " + (hoist[k] ? "h.dg" + kk : "digamma64(" + l + ")"));
         break;
+      default: {
+        const std::string pw =
+            hoist[k] ? "h.pw" + kk : "pow(" + l + ", opaque64(" + r + " - 1.0))";
+        const std::string lg = hoist[k] ? "h.lg" + kk : "log(" + l + ")";
+        def(lhs[k], g + " * " + r + " * " + pw);
+        def(rhs[k], l + " > 0.0 ? " + g + " * " + me + " * " + lg + " : 0.0");
+        break;
+      }
     }
   }
   std::ostringstream o;
   o << "#include \"eval64_kernels.h\"\n#include \"step_common.h\"\n"
     << "namespace gpum {\n"
     << "struct JitEval64 {\n"
+    << "  struct Hoisted {" << fields.str() << " };\n"
+    << "  static __device__ __forceinline__ Hoisted prepare(const Eval64Args& a,\n"
+    << "      const double* __restrict__ P) {\n"
+    << "#pragma clang fp contract(off)\n"
+    << "    Hoisted h;\n" << prep.str() << "    return h;\n  }\n"
     << "  static __device__ __forceinline__ double value(const Eval64Args& a,\n"
-    << "      const double* __restrict__ P, uint32_t i) {\n"
+    << "      const Hoisted& h, const double* __restrict__ P, uint32_t i) {\n"
     << "#pragma clang fp contract(off)\n"
     << fwd.str() << "    return " << root << ";\n  }\n"
     << "  static __device__ __forceinline__ double grad(const Eval64Args& a,\n"
-    << "      const double* __restrict__ P, uint32_t i,\n"
+    << "      const Hoisted& h, const double* __restrict__ P, uint32_t i,\n"
     << "      double (&g)[kMaxParams]) {\n"
     << "#pragma clang fp contract(off)\n"
     << fwd.str() << bwd.str() << "    return " << root << ";\n  }\n"

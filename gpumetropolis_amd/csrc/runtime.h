@@ -123,6 +123,8 @@ struct gpum_hip_model {
   uint32_t n_cols_in = 0;
   bool eval64_ready = false;
   uint32_t slots_ll = 0;  // values the gradient keeps for its backward pass
+  uint32_t slots_prior = 0;
+  uint32_t prior_depth = 0;
   gpum::rt::DevBuf<gpum::TapeOp> tape_ll, tape_prior;
   gpum::rt::DevBuf<double> data64, consts64;
 };

@@ -31,7 +31,7 @@
  *     (R/api.R:373-377); here it runs on the stepwise sampler with the
  *     log-posterior and its gradient in FP64 (reverse mode over the
  *     bytecode, the model's FP32 columns widened), at most 1024
- *     instructions in the program and 256 in the prior; gpum_run_ext fused
+ *     instructions in the program and in the prior; gpum_run_ext fused
  *     batches (swap_every) are not available with it.
  *   - draws are f32, column-major (n_iter, n_chains, n_params):
  *     draws[t + n_iter * (c + n_chains * j)].
