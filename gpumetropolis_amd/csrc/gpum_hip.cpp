@@ -251,18 +251,20 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
     if (ext && ext->struct_size >= offsetof(gpum_run_ext, chains_per_wave) + 4 &&
         ext->chains_per_wave > 0)
       cw_req = ext->chains_per_wave;
-    // Automatic: two chains per wave when the program keeps no stack, the
-    // chains fill every CU with pairs and the launch runs a compiled kernel
-    // (the interpreter's dispatch holds both chains' registers at each
-    // opcode merge and spills).  A ladder or a pinned workgroup must hold
-    // whole pairs.
+    // Automatic: several chains per wave when the program keeps no stack,
+    // the chains fill every CU with such groups and the launch runs a
+    // compiled kernel (the interpreter's dispatch holds every chain's
+    // registers at each opcode merge and spills).  A ladder or a pinned
+    // workgroup must hold whole groups.
     if (cw_req == 0) {
       const bool has_tiling =
           ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4;
-      // Four chains per wave (an explicit choice, affine programs) measured
-      // slower for 4096 gamma chains: with 4 waves per group each wave
-      // gets 2 chunks per tile and the per-tile work dominates.
-      for (uint32_t cw : {2u}) {
+      // Four chains per wave for affine programs on the chain kernel: each
+      // LDS read feeds four chains' FMAs (4096 gamma chains over 1e6
+      // observations: 67.7 ms per 250-iteration launch, against 71.9 with
+      // two and 83.7 with one).  Otherwise two.
+      for (uint32_t cw : {4u, 2u}) {
+        if (cw == 4 && (points || !model_affine(m))) continue;
         const bool split_ladder = ext && ext->swap_every > 0 &&
                                   ext->swap_group > 1 && ext->swap_group % cw;
         const bool split_pin = has_tiling && ext->chains_per_block > 0 &&
