@@ -25,6 +25,10 @@ CANDIDATES: Tuple[Tuple[int, ...], ...] = ((16, 1), (8, 2), (4, 4))
 # Two chains per wave, for compiled batches whose programs all have depth 1.
 PAIR_CANDIDATES: Tuple[Tuple[int, ...], ...] = ((32, 1, 2), (16, 2, 2),
                                                 (8, 4, 2))
+# Four chains per wave, when those programs are also all affine (a program
+# that keeps a value per observation holds four chains' values and spills:
+# 512 regressions, 2206 ms with two chains per wave, 4240 ms with four).
+QUAD_CANDIDATES: Tuple[Tuple[int, ...], ...] = ((16, 4, 4),)
 
 _cache: Dict[str, Tuple[int, ...]] = {}
 

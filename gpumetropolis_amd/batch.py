@@ -103,6 +103,8 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
             cands = autotune.CANDIDATES
             if compiled and all(i["vm_depth"] <= 1 for i in infos):
                 cands = cands + autotune.PAIR_CANDIDATES
+                if all(i["sum_mode"] == 2 for i in infos):
+                    cands = cands + autotune.QUAD_CANDIDATES
             tiling, timings = autotune.choose(key, time_candidate,
                                               candidates=cands,
                                               cache_path=tile_cache)

@@ -66,9 +66,10 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
       o.chains_per_block > 0 ? (uint32_t)o.chains_per_block : 16u * cw;
   const uint32_t wpc = o.waves_per_chain > 0 ? (uint32_t)o.waves_per_chain
                                              : 16u / std::max(1u, cpb / cw);
-  if (cw > 2 || cpb % cw != 0 || cpb / cw * wpc != 16) {
-    set_err("batched runs need chains_per_wave 1 or 2, chains_per_block a "
-            "multiple of it and chains_per_block / chains_per_wave * "
+  if ((cw != 1 && cw != 2 && cw != 4) || cpb % cw != 0 ||
+      cpb / cw * wpc != 16) {
+    set_err("batched runs need chains_per_wave 1, 2 or 4, chains_per_block "
+            "a multiple of it and chains_per_block / chains_per_wave * "
             "waves_per_chain == 16");
     return 1;
   }
@@ -90,8 +91,8 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
   }
   TilingForce force;
   force.v = pick_variant(max_depth, all_affine);
-  if (cw == 2 && force.v.ns != 0) {
-    set_err("chains_per_wave 2 needs every model's program at depth 1");
+  if (cw > 1 && force.v.ns != 0) {
+    set_err("chains_per_wave %u needs every model's program at depth 1", cw);
     return 1;
   }
   force.v.cw = (int)cw;
@@ -265,6 +266,12 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
                             lds + kBatchArgsLds, &failed);
       if (failed) return 1;
     }
+  }
+  if (!jf && cw > 2) {
+    // The interpreter's batched kernels hold one or two chains per wave.
+    set_err("chains_per_wave %u runs on compiled kernels only (every model "
+            "compiled, one program)", cw);
+    return 1;
   }
   auto enqueue = [&]() -> hipError_t {
     uint32_t t0 = 0;

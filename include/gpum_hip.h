@@ -449,7 +449,8 @@ typedef struct gpum_batch_opts {
   double kernel_ms;           /* out: device time of the run */
   uint32_t n_workgroups;      /* out: workgroups per batch launch */
   int32_t chains_per_wave;    /* 0 or 1: one chain per wave; 2: two (every
-                                 model of vm_depth 1) */
+                                 model of vm_depth 1); 4: four (also every
+                                 model compiled, one program) */
 } gpum_batch_opts;
 
 int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,

@@ -148,3 +148,23 @@ def test_two_chains_per_wave_batch(hip):
         b = gpu_metropolis_batch(ms, ds, tiling=(16, 2, 2), jit=jit, **kw)
         for fa, fb in zip(a.fits, b.fits):
             assert np.array_equal(fa.draws, fb.draws)
+
+
+def test_four_chains_per_wave_batch(hip):
+    """Affine programs (gamma) batched with four chains per wave on the
+    compiled kernel: the draws of one chain per wave, bit for bit; the
+    interpreter's batched kernels refuse four."""
+    print("[quads] batch", flush=True)
+    wls = [workloads.gamma(n=n, seed=40 + k) for k, n in enumerate((3000, 7000))]
+    rng = np.random.default_rng(5)
+    inits = [np.array([np.log(2.5), np.log(0.7)]) + 0.01 * rng.standard_normal((16, 2))
+             for _ in wls]
+    kw = dict(n_chains=16, n_iter=60, warmup=20, de_every=20, init=inits,
+              proposal_sd=0.01, seed=4)
+    ms, ds = [w.model for w in wls], [w.data for w in wls]
+    a = gpu_metropolis_batch(ms, ds, tiling=(16, 1), **kw)
+    b = gpu_metropolis_batch(ms, ds, tiling=(16, 4, 4), jit=True, **kw)
+    for fa, fb in zip(a.fits, b.fits):
+        assert np.array_equal(fa.draws, fb.draws)
+    with pytest.raises(hip.BackendError, match="compiled kernels only"):
+        gpu_metropolis_batch(ms, ds, tiling=(16, 4, 4), jit=False, **kw)
