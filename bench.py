@@ -120,8 +120,9 @@ def cpu_baseline(w, target_s):
                    np.ones(C), iters, 1, n_threads=cores, loglik_fn=addr)
         return time.perf_counter() - t
 
-    t1 = timed(1)
-    iters = max(1, int(target_s / max(t1, 1e-3)))
+    # One iteration is two data passes (the initial state and the proposal).
+    per_pass = timed(1) / 2.0
+    iters = max(1, int(target_s / max(per_pass, 1e-3)) - 1)
     dt = timed(iters)
     evals = C * w.n_obs * (iters + 1)  # + the initial evaluation
     return {"value": evals / dt, "unit": "obs-loglik evals/s", "cores": cores,
