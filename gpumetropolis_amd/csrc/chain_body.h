@@ -207,8 +207,29 @@ __device__ __forceinline__ void step_params(const ChainArgs* __restrict__ A,
   r.prior_lp() = E::prior(A->prior, A->n_prior, A->prior_consts, r.prop);
 }
 
-// Prepare iteration it: Philox uniforms, Box-Muller, the proposal
-// (cpu_native.rs:726-784), then its parameter programs.  One call per
+// The proposal's normal deviates of iteration it, in parallel: lane
+// 16 k + j of a chain group's first wave draws z_j of the group's chain k
+// (Box-Muller on the Philox slots of cpu_native.rs:728-735, after the pair
+// and the scale in mode 1) into the chain's record, where
+// chain_prepare_iter reads them.  n_params <= 16 and at most four chains
+// per wave, so every (chain, parameter) has its lane.
+__device__ __noinline__ void chain_normals(const ChainArgs* __restrict__ A,
+                                           uint32_t it, uint32_t chain_first,
+                                           int n_group, int lane,
+                                           double* st_first, int SW) {
+  const int np = (int)A->n_params;
+  const int k = lane >> 4, j = lane & 15;
+  const uint32_t chain = chain_first + (uint32_t)k;
+  if (k >= n_group || j >= np || chain >= A->n_chains) return;
+  const uint32_t zo = A->mode == 1 ? 3u : 0u;
+  const double two_pi = 6.283185307179586;
+  StepUniforms us{A->seed, A->stream_id, A->iter_offset + it, chain};
+  const double u1 = us.get(zo + 2 * j), u2 = us.get(zo + 2 * j + 1);
+  make_rec(st_first + k * SW, np).z[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+}
+
+// Prepare iteration it: the proposal (cpu_native.rs:726-784) from the
+// deviates chain_normals drew, then its parameter programs.  One call per
 // iteration from the kernel body (each call saves the body's live
 // registers).
 template <class E>
@@ -218,22 +239,13 @@ __device__ __noinline__ void chain_prepare_iter(const ChainArgs* __restrict__ A,
   const int np = (int)A->n_params;
   const uint32_t C = A->n_chains;
   const uint32_t mode = A->mode;
-  const double two_pi = 6.283185307179586;
   Rec r = make_rec(st, np);
   StepUniforms us{A->seed, A->stream_id, A->iter_offset + it, chain};
   const double* sd = r.sd;
   if (mode == 0) {
-    for (int j = 0; j < np; ++j) {
-      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-      double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-      r.prop[j] = r.state[j] + sd[j] * z;
-    }
+    for (int j = 0; j < np; ++j) r.prop[j] = r.state[j] + sd[j] * r.z[j];
   } else if (mode == 2) {
     const double* L = A->proposal_l + (size_t)chain * np * np;
-    for (int j = 0; j < np; ++j) {
-      double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-      r.z[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
-    }
     for (int k = 0; k < np; ++k) {
       double a = r.state[k];
       for (int j = 0; j <= k; ++j) a += L[k * np + j] * r.z[j];
@@ -249,10 +261,8 @@ __device__ __noinline__ void chain_prepare_iter(const ChainArgs* __restrict__ A,
     if (ib == ia) ib = (ib + 1) % C;
     double g = us.get(2) < 0.1 ? 1.0 : A->gamma;
     for (int j = 0; j < np; ++j) {
-      double u1 = us.get(3 + 2 * j), u2 = us.get(4 + 2 * j);
-      double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
       double diff = A->init[(size_t)ia * np + j] - A->init[(size_t)ib * np + j];
-      r.prop[j] = r.state[j] + g * diff + A->de_noise * sd[j] * z;
+      r.prop[j] = r.state[j] + g * diff + A->de_noise * sd[j] * r.z[j];
     }
   }
   step_params<E>(A, st, U);
@@ -566,6 +576,10 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
         __syncthreads();
       }
     }
+    if (part == 0 && active && it < H.n_iter)
+      chain_normals(Ap, it, chain0 + (uint32_t)c_first, CW, lane,
+                    st_all + c_first * SW, SW);
+    __syncthreads();
     if (lead && it < H.n_iter)
       chain_prepare_iter<typename E::Params>(Ap, it, chain, st, U);
     __syncthreads();
