@@ -39,8 +39,8 @@ VALU_PEAK = 256 * 64 * 2.4e9  # FP32 lane-ops/s, non-packed (one slot per op)
 # profiles/compiled_chain_kernel_gamma_1e6obs_4096chains.md.  Reported only
 # for the launch shape it was measured on.
 PROFILED_TRAFFIC = {
-    ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 23_737_993 * 1024 * 2.0,
-                                         "write_bytes": 437_232 * 1024 * 1.0},
+    ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 29_462_164 * 1024 * 2.0,
+                                         "write_bytes": 70_256 * 1024 * 1.0},
 }
 HBM_PEAK = 8.0e12             # B/s
 
