@@ -103,7 +103,9 @@ def gpu_metropolis_batch(models: Sequence, datas: Sequence, n_chains=64,
             cands = autotune.CANDIDATES
             if compiled and all(i["vm_depth"] <= 1 for i in infos):
                 cands = cands + autotune.PAIR_CANDIDATES
-                if all(i["sum_mode"] == 2 for i in infos):
+                # affine or square sums: every model the same form
+                modes = {i["sum_mode"] for i in infos}
+                if modes == {2} or modes == {1}:
                     cands = cands + autotune.QUAD_CANDIDATES
             tiling, timings = autotune.choose(key, time_candidate,
                                               candidates=cands,

@@ -74,7 +74,7 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     return 1;
   }
   int max_depth = 0;
-  bool all_affine = true;
+  bool all_affine = true, all_square = true;
   for (uint32_t k = 0; k < n_models; ++k) {
     const gpum_batch_model& b = models[k];
     if (!b.model || !b.init || !b.proposal_sd || b.n_chains == 0) {
@@ -88,9 +88,12 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     }
     max_depth = std::max(max_depth, b.model->L.vm_depth);
     all_affine = all_affine && model_affine(b.model);
+    all_square = all_square && model_sum_mode(b.model) == SUM_SQUARE;
   }
   TilingForce force;
-  force.v = pick_variant(max_depth, all_affine);
+  force.v = pick_variant(max_depth, all_affine   ? (uint32_t)SUM_AFFINE
+                                    : all_square ? (uint32_t)SUM_SQUARE
+                                                 : (uint32_t)SUM_VALUE);
   if (cw > 1 && force.v.ns != 0) {
     set_err("chains_per_wave %u needs every model's program at depth 1", cw);
     return 1;

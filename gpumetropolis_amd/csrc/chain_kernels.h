@@ -56,6 +56,8 @@ points_kernel(const PointsArgs* __restrict__ P, ChainHot H) {
   X(0, 16, 2)            \
   X(0, 8, 1)             \
   X(0, 8, 2)             \
+  X(0, 4, 1)             \
+  X(0, 4, 2)             \
   X(1, 16, 1)            \
   X(3, 16, 1)            \
   X(7, 8, 1)             \

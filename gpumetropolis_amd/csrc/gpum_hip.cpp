@@ -104,14 +104,16 @@ bool device_init() {
   return true;
 }
 
-Variant pick_variant(int vm_depth, bool affine) {
+Variant pick_variant(int vm_depth, uint32_t sum_mode) {
   int need = vm_depth > 1 ? vm_depth - 1 : 0;
   // A depth-1 program that keeps a value per observation (not affine)
-  // works on 8 observations per lane, so that two chains per wave hold
-  // their values, their data and their uniforms without spilling; the
-  // choice depends on the program only, so one and two chains per wave
-  // compute alike.
-  if (need == 0) return {0, affine ? 16 : 8, 1};
+  // works on fewer observations per lane, so that several chains per wave
+  // hold their values, their data and their uniforms: 8 for a value sum,
+  // 4 for a square sum (the batched regression kernel spills 128 B per
+  // lane at 8 with two chains per wave, none at 4).  The choice depends on
+  // the program only, so every chains-per-wave count computes alike.
+  if (need == 0)
+    return {0, sum_mode == SUM_AFFINE ? 16 : sum_mode == SUM_SQUARE ? 4 : 8, 1};
   if (need <= 1) return {1, 16, 1};
   if (need <= 3) return {3, 16, 1};
   if (need <= 7) return {7, 8, 1};
@@ -287,7 +289,7 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
 static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
                            const gpum_run_ext* ext, Tiling* t, bool points,
                            int cw_in, const TilingForce* force) {
-  t->v = force ? force->v : pick_variant(m->L.vm_depth, model_affine(m));
+  t->v = force ? force->v : pick_variant(m->L.vm_depth, model_sum_mode(m));
   const size_t static_lds =
       kStreamStaticLds + (force ? force->extra_static
                                 : (points ? kPointsExtraLds : 0));
@@ -579,7 +581,7 @@ int gpum_hip_model_set_jit(gpum_hip_model* m, int32_t enable,
   }
   // Compile the streamed chain kernel of the variant of large launches now
   // (a failure is reported here); other kinds and variants on first use.
-  Variant v = pick_variant(m->L.vm_depth, model_affine(m));
+  Variant v = pick_variant(m->L.vm_depth, model_sum_mode(m));
   if (v.ns == 0) v.cw = 2;
   std::string src = model_jit_source(m, v);
   std::string err;
@@ -681,7 +683,7 @@ int gpum_hip_jit_check(const uint32_t* code, size_t code_len,
     set_err("log-likelihood: %s", e.c_str());
     return 1;
   }
-  Variant v = pick_variant(L.vm_depth, !L.empty && L.sum_mode == SUM_AFFINE);
+  Variant v = pick_variant(L.vm_depth, L.empty ? (uint32_t)SUM_VALUE : L.sum_mode);
   const int cw = chains_per_wave > 0 ? chains_per_wave : 1;
   if (cw != 1 && !((cw == 2 || cw == 4) && v.ns == 0)) {
     set_err("chains_per_wave must be 1, or 2 or 4 for a program of depth 1");

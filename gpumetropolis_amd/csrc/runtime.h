@@ -82,7 +82,8 @@ struct Variant {
   int ns, opt, cw;
 };
 
-Variant pick_variant(int vm_depth, bool affine);
+// The variant of a program of depth vm_depth summed in sum_mode (SUM_*).
+Variant pick_variant(int vm_depth, uint32_t sum_mode);
 uint32_t round_up(uint32_t x, uint32_t m);
 
 struct Tiling {
@@ -168,6 +169,10 @@ bool model_build(gpum_hip_model* m, const uint32_t* code, size_t code_len,
 // after the program's padding instruction).
 inline bool model_affine(const gpum_hip_model* m) {
   return !m->L.empty && m->L.sum_mode == SUM_AFFINE;
+}
+
+inline uint32_t model_sum_mode(const gpum_hip_model* m) {
+  return m->L.empty ? (uint32_t)SUM_VALUE : m->L.sum_mode;
 }
 
 inline void set_sum_fields(const gpum_hip_model* m, ChainHot* h) {
