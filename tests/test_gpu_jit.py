@@ -168,3 +168,27 @@ def test_four_chains_per_wave_batch(hip):
         assert np.array_equal(fa.draws, fb.draws)
     with pytest.raises(hip.BackendError, match="compiled kernels only"):
         gpu_metropolis_batch(ms, ds, tiling=(16, 4, 4), jit=False, **kw)
+
+
+def test_four_chains_per_wave_square_batch(hip):
+    """Square-sum programs (regressions, 4 observations per lane) batched
+    with four chains per wave on the compiled kernel: the draws of the
+    interpreter at one chain per wave, bit for bit, and the closed-form
+    posterior mean of b0 within 4 MCSE."""
+    print("[quads] square batch", flush=True)
+    wls = workloads.regressions(n_models=3, n_min=1500, n_max=6000, seed=7)
+    rng = np.random.default_rng(6)
+    inits = [w.meta["true"] + 0.01 * rng.standard_normal((16, 10)) for w in wls]
+    kw = dict(n_chains=16, n_iter=1600, warmup=600, de_every=20, init=inits,
+              proposal_sd=0.003, seed=9)
+    ms, ds = [w.model for w in wls], [w.data for w in wls]
+    a = gpu_metropolis_batch(ms, ds, tiling=(16, 1), jit=False, **kw)
+    b = gpu_metropolis_batch(ms, ds, tiling=(16, 4, 4), jit=True, **kw)
+    for fa, fb in zip(a.fits, b.fits):
+        assert np.array_equal(fa.draws, fb.draws)
+    from gpumetropolis_amd.diagnostics import summarize
+    post = workloads.regression_posterior(wls[0].data)
+    s = summarize(b.fits[0].draws)["params"][0]
+    print(f"[quads] b0 {s['mean']:.5f} vs {post['mean'][0]:.5f} "
+          f"mcse {s['mcse']:.2e}", flush=True)
+    assert abs(s["mean"] - post["mean"][0]) <= 4 * s["mcse"]
