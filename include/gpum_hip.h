@@ -112,7 +112,8 @@ typedef struct gpum_run_ext {
   double* final_scale;    /* out: n_chains RM scales */
   double* trace_ll;       /* out: n_iter * n_chains data term of each proposal */
   /* Chains evaluated together by one wave on every observation it reads
-   * (0 = auto; 2 needs a program without a stack: vm_depth 1). */
+   * (0 = auto; 2 and 4 need a program without a stack, vm_depth 1; 4 runs
+   * on the model's compiled kernel). */
   int32_t chains_per_wave;
 } gpum_run_ext;
 
@@ -360,7 +361,9 @@ typedef struct gpum_da_ext {
   double exact_evals;       /* out: proposals evaluated exactly */
   double* trace_prop;       /* out: n_iter * n_chains * n_params proposals */
   double* trace_ll;         /* out: n_iter * n_chains data term (NaN: not run) */
-  uint32_t chains_per_wave; /* exact-stage slots per wave (0 auto, 1 or 2) */
+  uint32_t chains_per_wave; /* exact-stage slots per wave (0 auto, 1, 2 or
+                               4; 2 and 4 need vm_depth 1, 4 a compiled
+                               kernel) */
   double gamma;             /* proposal_mode 1: difference scale (> 0) */
   double de_noise;          /* proposal_mode 1: jitter scale on proposal_sd */
 } gpum_da_ext;
