@@ -50,30 +50,64 @@ __device__ __forceinline__ float vm_pow(float b, float e) {
   return r;
 }
 
-// FP32 log-gamma for the data-dependent lgamma opcode: shift the argument up
-// past 8 with lgamma(x) = lgamma(x + 1) - ln x (accumulated as one product
-// and one log), then Stirling's series with three correction terms
-// (truncation error below 3e-10 at x >= 8); reflection below 0.5.
-__device__ __forceinline__ float vm_lgamma(float xin) {
-  bool refl = xin < 0.5f;
-  float x = refl ? 1.0f - xin : xin;
-  float prod = 1.0f;
+// FP32 log-gamma of a positive argument, for the data-dependent lgamma
+// opcode.
+//   x >= 8:     Stirling's series with three correction terms (truncation
+//               error below 3e-10).
+//   0 < x < 8:  shift to z = x - k in [2, 3), k = floor(x) - 2:
+//                 k = -2, -1:  lgamma(x) = lgamma(z) - ln(x (x + 1)) or - ln x
+//                 k = 1..5:    lgamma(x) = lgamma(z) + ln((x-1) ... (x-k))
+//               (every factor x - j is exact), and lgamma(z) = (z - 2) q(z)
+//               with q a degree-7 polynomial in z - 2.5 (coefficients from
+//               csrc/lgamma_fit.py: least squares against a 50-digit
+//               log-gamma, rounded to FP32).
+// One log serves both branches.  The relative error is about 2e-7 of
+// max(|lgamma(x)|, 1), where Stirling after shifting up to 8 lost ~3e-6.
+__device__ __forceinline__ float vm_lgamma_pos(float x) {
+  const bool big = x >= 8.0f;
+  const float k = floorf(x) - 2.0f;
+  const float z = x - k;
+  float pd = 1.0f;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    bool sh = x < 8.0f;
-    prod = sh ? prod * x : prod;
-    x = sh ? x + 1.0f : x;
-  }
-  float inv = vm_rcp(x);
-  float inv2 = inv * inv;
-  float series =
+  for (int j = 1; j <= 5; ++j)
+    pd = (float)j <= k ? pd * (x - (float)j) : pd;
+  const float pu = x < 1.0f ? x * (x + 1.0f) : x;
+  const float p = k < 0.0f ? pu : pd;
+  const float lg = vm_log(big ? x : p);
+  const float t = z - 2.5f;
+  float q = 1.149600794e-04f;
+  q = fmaf(q, t, -3.346131707e-04f);
+  q = fmaf(q, t, 9.595991578e-04f);
+  q = fmaf(q, t, -3.092080355e-03f);
+  q = fmaf(q, t, 1.087710634e-02f);
+  q = fmaf(q, t, -4.480599985e-02f);
+  q = fmaf(q, t, 2.675817907e-01f);
+  q = fmaf(q, t, 5.693657398e-01f);
+  const float poly = (z - 2.0f) * q;
+  const float small = k < 0.0f ? poly - lg : poly + lg;
+  const float inv = vm_rcp(x);
+  const float inv2 = inv * inv;
+  const float series =
       inv * (0.083333333333333333f -
              inv2 * (0.0027777777777777778f - inv2 * 0.00079365079365079365f));
-  float lx = vm_log(x);
-  float r = (x - 0.5f) * lx - x + 0.91893853320467274f + series - vm_log(prod);
+  const float stirling = (x - 0.5f) * lg - x + (0.91893853320467274f + series);
+  return big ? stirling : small;
+}
+
+// FP32 log-gamma: vm_lgamma_pos for x > 0; for x <= 0 the reflection
+// lgamma(x) = ln(pi / |sin(pi x)|) - lgamma(1 - x).  The hardware sine takes
+// revolutions, sin(2 pi t), on a limited range, so t = x/2 is first reduced
+// to [-1/2, 1/2] (exact: t - rint(t)); at the poles (integers x <= 0) the
+// reduced argument is a whole half-revolution and the result +inf or very
+// large.
+__device__ __forceinline__ float vm_lgamma(float xin) {
+  const bool refl = xin <= 0.0f;
+  float r = vm_lgamma_pos(refl ? 1.0f - xin : xin);
   if (refl) {
     const float pi = 3.14159265358979f;
-    float s = __builtin_amdgcn_sinf(xin * 0.5f);  // sin(2*pi*t) with t = xin/2
+    float t = xin * 0.5f;
+    t = t - rintf(t);
+    const float s = __builtin_amdgcn_sinf(t);
     r = vm_log(fabsf(pi * vm_rcp(s))) - r;
   }
   return r;

@@ -44,17 +44,12 @@ def ess_chains(draws: np.ndarray) -> np.ndarray:
     if npair < 1:
         return out
     pair = rho[0:2 * npair:2] + rho[1:2 * npair:2]
-    res = np.empty(pair.shape[1])
-    for j in range(pair.shape[1]):
-        neg = np.nonzero(pair[:, j] <= 0)[0]
-        last = neg[0] if neg.size else npair
-        if last < 1:
-            res[j] = n
-            continue
-        tau = 1.0 + 2.0 * pair[:last, j].sum()
-        tau = max(tau, 1.0)
-        res[j] = min(n / tau, n)
-    out[ok] = res
+    # Pairs are kept up to (not including) the first non-positive one.
+    nonpos = pair <= 0
+    last = np.where(nonpos.any(axis=0), nonpos.argmax(axis=0), npair)
+    keep = np.arange(npair)[:, None] < last[None, :]
+    tau = np.maximum(1.0 + 2.0 * np.where(keep, pair, 0.0).sum(axis=0), 1.0)
+    out[ok] = np.where(last < 1, float(n), np.minimum(n / tau, n))
     return out
 
 

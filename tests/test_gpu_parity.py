@@ -108,6 +108,83 @@ def test_summation_forms_match_fp64(hip, orc, form, n_obs):
         dm.close()
 
 
+# The FP32 per-observation log-gamma (csrc/device_vm.h): its argument mixes
+# data and parameters, so it is neither hoisted nor precomputed and runs on
+# the VM.  Data terms against the oracle's FP64 Lanczos log-gamma
+# (cpu_native.rs:34-60), interpreted and compiled.  Arguments: y > 0 from a
+# gamma law, x in two bands that keep negative arguments at least 0.1 away
+# from the poles (the argument's own FP32 rounding is amplified by
+# 1/distance there).
+LGAMMA_FORMS = [
+    ("~ lgamma(y + a) - b * y", [2.5, 0.7]),        # arguments in (2.5, 15)
+    ("~ lgamma(y * y * b + a) - y", [0.1, 0.5]),    # from 0.1 across 0.5
+    ("~ lgamma(x + a) * b + y", [0.0, 0.6]),        # negative: reflection
+    ("~ lgamma(y * 4 + a) * b", [1.0, 0.5]),        # beyond 8: Stirling
+]
+
+
+@pytest.mark.parametrize("n_obs", [1025, 70001])
+@pytest.mark.parametrize("form", LGAMMA_FORMS, ids=lambda f: f[0])
+def test_lgamma_data_terms_match_fp64(hip, orc, form, n_obs):
+    formula, centre = form
+    m = dsl.gpum_model(formula, ["a", "b"], ["y", "x"])
+    rng = np.random.default_rng(n_obs + 3)
+    y = rng.gamma(2.0, 1.5, n_obs)
+    x = rng.uniform(0.15, 0.85, n_obs) * np.where(rng.random(n_obs) < 0.5, -1, 1)
+    x = x - np.where(rng.random(n_obs) < 0.3, 1.0, 0.0)     # (-1.85, -1.15) too
+    data = np.column_stack([y, x]).astype(np.float32).astype(np.float64)
+    pts = _points(rng, centre, 0.03, 8)
+    ref = orc.loglik_batch(m.loglik.code, m.loglik.consts, 2, data, 2, n_obs, pts)
+    assert np.all(np.isfinite(ref))
+    scale = np.abs(ref) + 1e-7 * n_obs
+    for jit in (False, True):
+        print(f"[lgamma] {formula} n={n_obs} jit={jit}", flush=True)
+        dm = hip.DeviceModel(m.loglik.code, m.loglik.consts, [], [], 2, data, 2,
+                             n_obs, jit=jit)
+        assert dm.info()["n_vm_instr"] > 0
+        got = dm.loglik(pts)
+        dm.close()
+        assert np.max(np.abs(got - ref) / scale) <= REL_TOL, (jit, got, ref)
+
+
+NEG_BINOMIAL = ("~ lgamma(y + exp(lr)) - lgamma(exp(lr)) - lgamma(y + 1)"
+                " - exp(lr) * log(1 + exp(-lp)) - y * log(1 + exp(lp))")
+
+
+@pytest.mark.parametrize("jit", [False, True])
+def test_replay_negative_binomial_lgamma(hip, orc, jit):
+    """Negative-binomial counts (size r = exp(lr), p = logistic(lp)): the
+    data-dependent lgamma(y + r) runs per observation on the device; the
+    replay must keep every decision."""
+    rng = np.random.default_rng(21)
+    n = 20000
+    y = rng.negative_binomial(3.0, 0.4, n).astype(np.float64)
+    m = dsl.gpum_model(NEG_BINOMIAL, ["lr", "lp"], ["y"],
+                       prior="~ -0.5 * (lr / 10)^2 - 0.5 * (lp / 10)^2")
+    w = workloads.Workload("negbin", m, y.reshape(-1, 1), 16, 120)
+    C, T, P = 16, 120, 2
+    init = np.array([np.log(3.0), np.log(0.4 / 0.6)]) + 0.02 * rng.standard_normal((C, P))
+    sd = np.full((C, P), 0.02)
+    temps = np.ones(C)
+    print(f"[replay] negative binomial jit={jit}", flush=True)
+    dm = hip.DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
+                         m.prior.consts, P, w.data, 1, n, jit=jit)
+    info = dm.info()
+    g = dm.run(init, sd, temps, T, 13, trace=True)
+    dm.close()
+    o = orc.run(m.loglik.code, m.loglik.consts, m.prior.code, m.prior.consts,
+                P, w.data, 1, n, init, sd, temps, T, 13,
+                stream=orc.STREAM_PHILOX, trace=True)
+    rep = replay_compare(g, o, orc, 13, temps, 0, P,
+                         init_lp_gpu=g.init_log_post,
+                         init_lp_orc=_init_lp_orc(orc, w, init))
+    print(rep, info, flush=True)
+    assert info["n_vm_instr"] > 0
+    assert rep.uncertified_flips == 0
+    assert rep.max_plp_rel <= REL_TOL
+    assert rep.steps_compared >= C * T // 2
+
+
 @pytest.mark.parametrize("n_obs", [1, 7, 255, 1023, 1025, 4097, 70001])
 def test_ragged_sizes(hip, orc, n_obs):
     rng = np.random.default_rng(n_obs)
