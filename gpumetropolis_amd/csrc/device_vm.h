@@ -118,6 +118,16 @@ __device__ __forceinline__ float vm_softplus(float x) {
   return fmaxf(x, 0.0f) + vm_log(1.0f + vm_exp(-fabsf(x)));
 }
 
+// log(exp(a) + exp(b)) that neither overflows nor underflows: the larger
+// argument plus log(1 + exp(-|a - b|)), one exp and one log.  Equal
+// arguments (equal infinities included) give a + ln 2, so two -inf terms
+// stay -inf; a NaN argument propagates through the difference.
+__device__ __forceinline__ float vm_lse(float a, float b) {
+  const float m = fmaxf(a, b);
+  const float dist = a == b ? 0.0f : fabsf(a - b);
+  return m + vm_log(1.0f + vm_exp(-dist));
+}
+
 template <int OPT>
 __device__ __forceinline__ void vm_load_col(const float* __restrict__ col,
                                             int lane, float (&v)[OPT]) {
@@ -253,6 +263,7 @@ case NAME:                                                             \
     GPUM_SOP(VM_MUL_S, s * t)
     GPUM_SOP(VM_DIV_S, s * vm_rcp(t))
     GPUM_SOP(VM_POW_S, vm_pow(s, t))
+    GPUM_SOP(VM_LSE_S, vm_lse(s, t))
 #undef GPUM_SOP
 #define GPUM_UOP(NAME, EXPR)                              \
 case NAME:                                              \

@@ -45,6 +45,8 @@ double vm_weight(uint32_t op) {
     case VM_LGAMMA: return 32.0;
     case VM_CUBE: case VM_POW4: return 2.0;
     case VM_SOFTPLUS: return 9.0;
+    // one EXP and one LOG, plus the max, difference and add
+    case VM_LSE_S: return 11.0;
     default: return 1.0;
   }
 }
@@ -484,6 +486,29 @@ struct Emitter {
     emit(i, d);
   }
 
+  // exp(A), c * exp(A) or exp(A) * c with a finite constant c > 0.
+  bool match_cexp(int i, int* arg, double* c) const {
+    const Node& n = N(i);
+    if (n.op == OP_EXP) {
+      *arg = n.k0;
+      *c = 1.0;
+      return true;
+    }
+    if (n.op != OP_MUL) return false;
+    int e = -1, k = -1;
+    if (N(n.k1).op == OP_EXP && N(n.k0).kind == K_CONST) {
+      e = n.k1;
+      k = n.k0;
+    } else if (N(n.k0).op == OP_EXP && N(n.k1).kind == K_CONST) {
+      e = n.k0;
+      k = n.k1;
+    }
+    if (e < 0 || !(N(k).val > 0.0) || !std::isfinite(N(k).val)) return false;
+    *arg = N(e).k0;
+    *c = N(k).val;
+    return true;
+  }
+
   // MUL(U, D) or MUL(D, U): returns true and the operands.
   bool match_mul_ud(int i, int* u, int* dnode) const {
     const Node& n = N(i);
@@ -516,6 +541,25 @@ struct Emitter {
           if (e >= 0) {
             value(N(e).k0, d);
             put(VM_SOFTPLUS);
+            return;
+          }
+          // log(c1 exp(A1) + c2 exp(A2)), constants c > 0 (1 if absent):
+          // log-sum-exp of A1 + log c1 and A2 + log c2, which cannot
+          // underflow where both exponentials do in FP32 (mixture terms far
+          // from a chain's means).  Equal weights add log c once, after.
+          int a1 = -1, a2 = -1;
+          double c1 = 1.0, c2 = 1.0;
+          if (match_cexp(a.k0, &a1, &c1) && match_cexp(a.k1, &a2, &c2)) {
+            const bool same = c1 == c2;
+            value(a1, d);
+            if (!same && c1 != 1.0)
+              put(VM_ADD_U, 0, (uint32_t)const_slot(out, (float)std::log(c1)));
+            value(a2, d + 1);
+            if (!same && c2 != 1.0)
+              put(VM_ADD_U, 0, (uint32_t)const_slot(out, (float)std::log(c2)));
+            put(VM_LSE_S, (uint32_t)d);
+            if (same && c1 != 1.0)
+              put(VM_ADD_U, 0, (uint32_t)const_slot(out, (float)std::log(c1)));
             return;
           }
         }
@@ -791,7 +835,7 @@ std::string disassemble(const Lowered& L) {
       "RDIV_U", "RPOW_U", "ADD_D", "SUB_D", "MUL_D", "DIV_D", "POW_D",
       "RSUB_D", "RDIV_D", "RPOW_D", "NEG", "EXP", "LOG", "SQRT", "LGAMMA",
       "SQR", "CUBE", "POW4", "RCP", "FMA_UU", "FMA_UD", "FMA_DU",
-      "SOFTPLUS", "SAVE", "LOAD_S", "PUSH_S"};
+      "SOFTPLUS", "SAVE", "LOAD_S", "PUSH_S", "LSE_S"};
   std::ostringstream s;
   for (const VmInstr& i : L.vm) {
     uint32_t op = vm_op(i);

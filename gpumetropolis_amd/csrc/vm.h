@@ -86,6 +86,8 @@ enum VmOp : uint32_t {
   VM_SAVE,    // S[slot] <- T
   VM_LOAD_S,  // T <- S[a] (stack was empty)
   VM_PUSH_S,  // S[slot] <- T ; T <- S[a]
+  // T <- log(exp(S[slot]) + exp(T)) without underflow (pops one)
+  VM_LSE_S,
   VM_NUM_OPS
 };
 

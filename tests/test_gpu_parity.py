@@ -147,6 +147,42 @@ def test_lgamma_data_terms_match_fp64(hip, orc, form, n_obs):
         assert np.max(np.abs(got - ref) / scale) <= REL_TOL, (jit, got, ref)
 
 
+# log(c1 exp(A1) + c2 exp(A2)) is evaluated as a log-sum-exp (csrc/lower.cpp,
+# VM_LSE_S): it must match FP64 near the data as the plain FP32 expression
+# did, and also far from it, where both exponentials underflow in FP32 (the
+# plain expression gives -inf there while the FP64 oracle stays finite).
+LSE_FORMS = [
+    ("~ log(0.5 * exp(-((y - a)^2) / 2) + 0.5 * exp(-((y - b)^2) / 2))",
+     [[-0.3, 1.2], [0.5, 0.6], [20.0, 25.0], [-30.0, 14.0]]),
+    ("~ log(0.3 * exp(x - a) + exp(y * b) * 0.7)",
+     [[0.2, 0.4], [-1.0, 2.0], [120.0, -60.0], [-95.0, 40.0]]),
+    ("~ log(exp(y - a) + exp(x * b))",
+     [[0.1, 0.3], [130.0, -70.0], [-99.0, 45.0]]),
+]
+
+
+@pytest.mark.parametrize("form", LSE_FORMS, ids=lambda f: f[0])
+def test_log_sum_exp_matches_fp64_near_and_far(hip, orc, form):
+    formula, pts = form
+    m = dsl.gpum_model(formula, ["a", "b"], ["y", "x"])
+    rng = np.random.default_rng(17)
+    n = 30001
+    data = np.column_stack([rng.normal(0.5, 1.0, n), rng.normal(-0.2, 1.5, n)])
+    data = data.astype(np.float32).astype(np.float64)
+    pts = np.asarray(pts, float)
+    ref = orc.loglik_batch(m.loglik.code, m.loglik.consts, 2, data, 2, n, pts)
+    assert np.all(np.isfinite(ref))
+    print(f"[lse] {formula}", flush=True)
+    for jit in (False, True):
+        dm = hip.DeviceModel(m.loglik.code, m.loglik.consts, [], [], 2, data, 2,
+                             n, jit=jit)
+        got = dm.loglik(pts)
+        dm.close()
+        rel = np.abs(got - ref) / (np.abs(ref) + 1e-7 * n)
+        assert np.all(np.isfinite(got)), (jit, got)
+        assert rel.max() <= REL_TOL, (jit, got, ref)
+
+
 NEG_BINOMIAL = ("~ lgamma(y + exp(lr)) - lgamma(exp(lr)) - lgamma(y + 1)"
                 " - exp(lr) * log(1 + exp(-lp)) - y * log(1 + exp(lp))")
 

@@ -63,6 +63,101 @@ def test_pt_fused_matches_oracle(hip, orc):
     assert (g.swap_hist[g.swap_hist != 255] == 1).mean() > 0.05
 
 
+def _current_lp(init_lp, plp, acc, swap_hist, sizes):
+    """Per-iteration current log-posterior of every chain, rebuilt from a
+    traced PT run: accepted proposals replace it, and at each batch end the
+    accepted swaps exchange it between the pair.  Also the end-of-batch
+    values the swap sweep saw."""
+    T, C = plp.shape
+    cur = np.asarray(init_lp, float).copy()
+    out = np.empty((T, C))
+    ends = np.empty((len(sizes), C))
+    t = 0
+    for b, n in enumerate(sizes):
+        for _ in range(n):
+            out[t] = cur
+            cur = np.where(acc[t], plp[t], cur)
+            t += 1
+        ends[b] = cur
+        for lo in np.nonzero(swap_hist[b] == 1)[0]:
+            cur[lo], cur[lo + 1] = cur[lo + 1], cur[lo]
+    return out, ends
+
+
+def test_pt_mixture_ladders_match_oracle(hip, orc):
+    """The configuration's ladder shape: two independent 16-rung ladders,
+    beta 1 .. 0.01 (T = 1 .. 100), on the 2-parameter mixture (20,000 rows).
+    Per ladder, everything up to its first differing decision is identical
+    (proposals, log-posteriors to 1e-6, swap decisions, batch acceptances),
+    and that first decision is certified: its FP64 margin lies within the
+    GPU's own evaluation error.  |log-post| ~ 4e4 here, so FP32 sums leave
+    ~1e-3 nats and a few near-ties are expected."""
+    from oracle import pt as opt
+    from gpumetropolis_amd import workloads
+    w = workloads.mixture(n=20000)
+    m = w.model
+    G, NG, T, se, seed = 16, 2, 400, 10, 8
+    C, P = G * NG, 2
+    temps = np.tile(1.0 / np.geomspace(1.0, 0.01, G), NG)
+    rng = np.random.default_rng(6)
+    init = rng.normal(0.0, 3.0, (C, P))
+    sd = 0.02 * np.sqrt(temps)[:, None] * np.ones((C, P))
+    args = (m.loglik.code, m.loglik.consts, [], [], P, w.data, 1, w.n_obs,
+            init, sd, temps, T, seed)
+    print("[pt] mixture 2 x 16 ladders vs oracle", flush=True)
+    g = hip.run_chains(*args, trace=True, final_state=True,
+                       batches={"swap_every": se, "swap_group": G,
+                                "adapt_batches": 10})
+    o = opt.run_pt(*args, swap_every=se, group=G, adapt_batches=10, trace=True)
+    sizes = o["batch_sizes"]
+    nb = len(sizes)
+    ll0 = orc.loglik_batch(m.loglik.code, m.loglik.consts, P, w.data, 1,
+                           w.n_obs, init)
+    cur_o, end_o = _current_lp(ll0, o["trace_plp"], o["trace_accept"],
+                               o["swap_hist"], sizes)
+    cur_g, end_g = _current_lp(g.init_log_post, g.trace_plp, g.trace_accept,
+                               g.swap_hist, sizes)
+    compared = 0
+    for g0 in range(0, C, G):
+        cs = slice(g0, g0 + G)
+        dif = np.argwhere(g.trace_accept[:, cs] != o["trace_accept"][:, cs])
+        t_acc = int(dif[:, 0].min()) if dif.size else T
+        sdif = np.argwhere(g.swap_hist[:, cs] != o["swap_hist"][:, cs])
+        b_sw = int(sdif[:, 0].min()) if sdif.size else nb
+        t_ok = min(t_acc, (b_sw + 1) * se)
+        print(f"group {g0 // G}: first decision flip t={t_acc}, first swap "
+              f"flip batch={b_sw}", flush=True)
+        assert np.allclose(g.trace_prop[:t_ok, cs], o["trace_prop"][:t_ok, cs],
+                           rtol=1e-12, atol=1e-12)
+        rel = np.abs(g.trace_plp[:t_ok, cs] - o["trace_plp"][:t_ok, cs]) / \
+            np.abs(o["trace_plp"][:t_ok, cs])
+        assert rel.max() <= 1e-6
+        b_ok = t_ok // se
+        b_sw_ok = min(b_ok, b_sw)
+        assert np.array_equal(g.swap_hist[:b_sw_ok, cs], o["swap_hist"][:b_sw_ok, cs])
+        assert np.array_equal(g.acc_hist[:b_ok, cs], o["acc_hist"][:b_ok, cs])
+        if dif.size and t_acc < (b_sw + 1) * se:   # a decision flipped
+            c = g0 + int(dif[dif[:, 0] == t_acc][0, 1])
+            u = orc.uniform_philox(seed, 0, 0, t_acc, c, 2 * P)
+            d64 = (o["trace_plp"][t_acc, c] - cur_o[t_acc, c]) / temps[c]
+            dg = (g.trace_plp[t_acc, c] - cur_g[t_acc, c]) / temps[c]
+            assert abs(math.log(u) - d64) <= abs(dg - d64) * (1 + 1e-9) + 1e-12
+        elif sdif.size:                       # a swap decision flipped
+            lo = g0 + int(sdif[sdif[:, 0] == b_sw][0, 1])
+            b1 = b_sw + 1
+            start = 0 if b1 % 2 == 1 else 1
+            k = (lo - g0 - start) // 2
+            wv = orc.philox([b1, k, g0 // G, opt.TAG_SWAP], [seed, 0])
+            u = (float(wv[0]) + 0.5) / 4294967296.0
+            dt = 1.0 / temps[lo] - 1.0 / temps[lo + 1]
+            d64 = (end_o[b_sw, lo + 1] - end_o[b_sw, lo]) * dt
+            dg = (end_g[b_sw, lo + 1] - end_g[b_sw, lo]) * dt
+            assert abs(math.log(u) - d64) <= abs(dg - d64) * (1 + 1e-9) + 1e-12
+        compared += t_ok
+    assert compared >= T * NG // 3
+    assert (g.swap_hist[g.swap_hist != 255] == 1).mean() > 0.05
+
+
 def test_pt_fused_equals_batched_launches(hip):
     """The in-kernel batches reproduce one launch per batch exactly (same
     tiling, same stream)."""

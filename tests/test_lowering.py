@@ -54,9 +54,37 @@ def test_logistic_shares_its_linear_predictor():
     assert on["sum_mode"] == VALUE and on["n_sum_tail"] == 0
 
 
-def test_mixture_keeps_its_value():
-    i = info(workloads.mixture(n=10).model)
-    assert (i["sum_mode"], i["n_sum_tail"]) == (VALUE, 0)
+def test_mixture_is_a_log_sum_exp():
+    m = workloads.mixture(n=10).model
+    on, off = info(m), info(m, fuse=0)
+    # log(0.5 exp(A1) + 0.5 exp(A2)): each A = -(y - mu)^2 / 2 is
+    # LOAD_D SUB_U SQR MUL_U (3 slots), then one LSE_S (11); the shared
+    # log 0.5 moves to the sum tail
+    assert (on["sum_mode"], on["n_sum_tail"]) == (VALUE, 1)
+    assert on["w_exec"] == 17.0
+    assert on["vm_depth"] == 2
+    # unfused: two EXPs, the weights' multiplies, the add and the LOG
+    assert off["w_exec"] == 23.0
+
+
+@pytest.mark.parametrize("formula,tail", [
+    ("~ log(0.3 * exp(y - a) + exp(x * b) * 0.7)", 0),   # unequal weights
+    ("~ log(exp(y - a) + exp(x * b))", 0),               # no weights
+    ("~ log(2 * exp(y * a) + 2 * exp(x - b))", 1),       # equal weights
+])
+def test_log_sum_exp_forms(formula, tail):
+    m = dsl.gpum_model(formula, ["a", "b"], ["y", "x"])
+    i = info(m)
+    assert i["n_sum_tail"] == tail
+    # one LSE (11) and the two arguments (one op each), plus an ADD_U of
+    # log c per unequal weight
+    assert i["w_exec"] == 13.0 + (2.0 if "0.3" in formula else 0.0)
+
+
+def test_log_of_sum_with_a_negative_weight_is_not_fused():
+    m = dsl.gpum_model("~ log(2 * exp(y - a) - exp(x * b))", ["a", "b"],
+                       ["y", "x"])
+    assert info(m)["w_exec"] == info(m, fuse=0)["w_exec"]
 
 
 @pytest.mark.parametrize("formula,mode,tail", [
