@@ -6,18 +6,22 @@ data are synthetic and seeded; there are no model weights.
 
 One step = one batch of `--iters-per-step` Metropolis iterations of all 4096
 chains over the full data set on the GPU (one launch of the fused chain
-kernel).  The W warmup steps run the reference's adaptive warmup
-(Welford + Robbins-Monro + pooled Cholesky, R/adaptive.R:101-254) untimed;
-the K timed steps continue the chains from their exact FP64 states with the
-frozen per-chain Cholesky proposal.  Defaults: W = K = 10, 250 iterations per
-step: 2500 warmup + 2500 sampling iterations = the 5000-iteration workload.
+kernel).  The W warmup steps are W x iters-per-step warmup iterations, run
+untimed as the reference's adaptive warmup with its own batch plan (10 to
+40 batches of about 50, R/adaptive.R:112-118; Welford + Robbins-Monro +
+pooled Cholesky, :101-254); the K timed steps continue the chains from
+their exact FP64 states with the frozen per-chain Cholesky proposal.
+Defaults: W = K = 10, 250 iterations per step: 2500 warmup + 2500 sampling
+iterations = the 5000-iteration workload.
 
 value  = observation-loglik evaluations per second over the timed region
          (chains x observations x iterations / wall time, inputs resident in
          HBM, draws copied back to the host inside the region).
 Also reported: ESS/s (min over parameters of the summed per-chain Geyer ESS
-of the sampling draws / timed wall) with the max split R-hat, the roofline
-of the chain kernel, and the oracle timed on the host's cores.
+of the sampling draws / timed wall) with the max split R-hat and the
+sampling acceptance, the posterior moments against the exact posterior
+(quadrature) and its Laplace approximation, the roofline of the chain
+kernel, and the oracle timed on the host's cores.
 """
 import argparse
 import json
@@ -81,7 +85,11 @@ def parse():
     p.add_argument("--no-batch", action="store_true",
                    help="skip the many-model (512 regressions) section")
     p.add_argument("--batch-models", type=int, default=512)
-    p.add_argument("--batch-iters", type=int, default=4000)
+    p.add_argument("--batch-iters", type=int, default=10000,
+                   help="DE-MC iterations per chain of the many-model run; "
+                        "64 chains of 4000 (half warmup) leave ~55 ESS per "
+                        "chain, which holds split R-hat near 1.02")
+    p.add_argument("--batch-warmup", type=int, default=2000)
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device-to-device copy of MIB MiB "
                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -99,36 +107,121 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4}), flush=True)
 
 
-def cpu_baseline(w, target_s):
+def _host_threads():
+    """Threads for the CPU baseline: the cores this process may run on,
+    capped by OMP_NUM_THREADS where the host sets it (a GPU host that
+    allots each GPU's jobs a share of its cores says so there)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, target_s, start, sd, L, seed):
     """The oracle (C restatement of the reference's f64 CPU sampler with the
     per-model compiled log-likelihood mirroring its JIT) on the host cores,
-    on a bounded sample of the same workload: `cores` chains (one per
-    thread) over the full data set for a calibrated number of iterations."""
+    on bounded samples of the same workload (one chain per thread, the full
+    data set):
+      evals/s  all threads' chains for a calibrated number of iterations;
+      ESS/s    the same number of chains started from the GPU run's final
+               (stationary) states with its adapted proposal, run for a
+               calibrated length; per-chain Geyer ESS summed, as the GPU's.
+    """
     from oracle import codegen, oracle
+    from gpumetropolis_amd.diagnostics import summarize
 
     m = w.model
-    cores = min(16, os.cpu_count() or 1)
+    P = m.n_params
+    cores = _host_threads()
     _, _, addr = codegen.compile_loglik(m.loglik.code, m.loglik.consts)
     C = cores
     init = np.tile([math.log(2.5), math.log(0.7)], (C, 1))
-    sd = np.full((C, 2), 0.001)
+    sd0 = np.full((C, 2), 0.001)
 
-    def timed(iters):
+    def timed(iters, init_m, sd_m, mode=0, Lm=None, s=1):
         t = time.perf_counter()
-        oracle.run(m.loglik.code, m.loglik.consts, m.prior.code,
-                   m.prior.consts, 2, w.data, 1, w.n_obs, init, sd,
-                   np.ones(C), iters, 1, n_threads=cores, loglik_fn=addr)
-        return time.perf_counter() - t
+        r = oracle.run(m.loglik.code, m.loglik.consts, m.prior.code,
+                       m.prior.consts, P, w.data, 1, w.n_obs, init_m, sd_m,
+                       np.ones(C), iters, s, proposal_mode=mode,
+                       proposal_l=Lm, n_threads=cores, loglik_fn=addr)
+        return time.perf_counter() - t, r
 
     # One iteration is two data passes (the initial state and the proposal).
-    per_pass = timed(1) / 2.0
+    per_pass = timed(1, init, sd0)[0] / 2.0
     iters = max(1, int(target_s / max(per_pass, 1e-3)) - 1)
-    dt = timed(iters)
+    dt, _ = timed(iters, init, sd0)
+    per_pass = dt / (iters + 1)
     evals = C * w.n_obs * (iters + 1)  # + the initial evaluation
-    return {"value": evals / dt, "unit": "obs-loglik evals/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{C} chains x {iters} iterations x {w.n_obs} obs "
-                      f"(compiled f64 loglik, {cores} threads, {dt:.1f} s)"}
+    out = {"value": evals / dt, "unit": "obs-loglik evals/s", "cores": cores,
+           "kind": "port",
+           "host_cores": os.cpu_count(), "cpu_model": _cpu_model(),
+           "threads_note": "one thread per core this process may use "
+                           "(affinity, capped by OMP_NUM_THREADS where the "
+                           "host sets it for one GPU's share)",
+           "sample": f"{C} chains x {iters} iterations x {w.n_obs} obs "
+                     f"(compiled f64 loglik, {cores} threads, {dt:.1f} s)"}
+    # ESS/s leg: stationary starts, the GPU run's adapted proposal.
+    mode = 2 if L is not None else 0
+    Lm = None if L is None else np.asarray(L).reshape(-1, P * P)[:C].ravel()
+    ess_iters = max(200, int(target_s / max(per_pass, 1e-6)))
+    dt2, r = timed(ess_iters, np.asarray(start)[:C], np.asarray(sd)[:C], mode,
+                   Lm, seed)
+    summ = summarize(r.draws.astype(np.float64))
+    out.update({
+        "ess_per_s": summ["min_ess"] / dt2,
+        "ess_min": summ["min_ess"], "ess_rhat_max": summ["max_rhat"],
+        "ess_accept": float(np.mean(r.accept_rate)),
+        "ess_sample": f"{C} chains x {ess_iters} iterations from the GPU "
+                      f"chains' final states with their adapted proposal "
+                      f"(mode {mode}), {dt2:.1f} s; ESS/s is chain-count "
+                      "invariant on a saturated CPU"})
+    return out
+
+
+def gaussian_mean_section(a):
+    """Plain RWM on the Gaussian mean with known sd 2 and a flat prior:
+    20,000 draws of N(3.4, 2), 8 chains x 3000 iterations with the
+    reference's adaptive warmup (half), overdispersed starts
+    mean(y) + 2 * seq(-2, 2) and proposal sd 2.4 * 2 / sqrt(n), as the
+    reference's M1 benchmark (benchmark/adapters/cpu_adapters.R:25-61).
+    ESS/s is the post-warmup ESS over the whole call's wall time, as there;
+    the closed-form posterior is N(mean(y), 4 / n)."""
+    from gpumetropolis_amd import workloads
+    from gpumetropolis_amd.api import gpu_metropolis
+    from gpumetropolis_amd.diagnostics import summarize
+
+    w = workloads.gaussian_mean()
+    C, T = 8, 3000
+    init = w.data.mean() + 2.0 * np.linspace(-2, 2, C).reshape(C, 1)
+    psd = 2.4 * 2.0 / math.sqrt(w.n_obs)
+    kw = dict(init=init, proposal_sd=psd, n_iter=T, seed=a.seed, jit=a.jit)
+    gpu_metropolis(w.model, w.data, **kw)        # first call: module loads
+    t0 = time.perf_counter()
+    fit = gpu_metropolis(w.model, w.data, **kw)
+    wall = time.perf_counter() - t0
+    s = summarize(fit.draws)
+    p = s["params"][0]
+    gate = s["max_rhat"] < 1.01
+    return {"workload": f"Gaussian mean, {w.n_obs} obs, {C} chains x {T} "
+                        f"iterations ({T - fit.draws.shape[0]} adaptive warmup)",
+            "engine": fit.adaptation.get("engine"),
+            "wall_s": wall, "ms_per_iteration": wall / T * 1e3,
+            "obs_evals_per_s": C * w.n_obs * T / wall,
+            "ess": s["min_ess"], "rhat_max": s["max_rhat"],
+            "ess_per_s": s["min_ess"] / wall if gate else None,
+            "closed_form_mean_err_mcse": abs(p["mean"] - w.meta["post_mean"]) / p["mcse"],
+            "sd_ratio": p["sd"] / w.meta["post_sd"],
+            "sampling_accept": float(np.mean(fit.accept_rate))}
 
 
 MLP_FLOP_PER_ROW = 2 * (16 * 256 + 256 * 256)  # matrix-core flops, forward
@@ -293,20 +386,32 @@ def pt_section(a):
     wall = time.perf_counter() - t0
     evals = C * wl.n_obs * a.pt_iters
     cold = fit.draws[:, ::G, :]                 # beta = 1 member of each ladder
-    s = summarize(cold)
-    # the posterior's two modes are the label swaps (mu1, mu2) = (-3, 3), (3, -3)
+    # The posterior's two modes are the label swaps (mu1, mu2) = (-3, 3),
+    # (3, -3): the likelihood is symmetric in the labels.  Diagnostics are
+    # taken on the label-invariant summary (min, max) of each draw; on the
+    # raw labels R-hat measures how often replicas cross between the
+    # mirror images, not convergence of the posterior.
     upper = float(np.mean(cold[:, :, 0] > cold[:, :, 1]))
+    srt = np.sort(cold.astype(np.float64), axis=2)
+    s = summarize(srt)
+    raw = summarize(cold)
+    gate = s["max_rhat"] < 1.01
     sh = fit.adaptation["swap_history"]
+    ks = fit.kernel_ms / 1e3
     return {"workload": f"bimodal mixture, {wl.n_obs} FP32 rows, {G} temperatures "
                         f"x {R} replicas, {a.pt_iters} iterations, swaps every "
                         f"{wl.meta['swap_every']}",
-            "obs_evals_per_s": evals / (fit.kernel_ms / 1e3),
+            "obs_evals_per_s": evals / ks,
             "kernel_ms": fit.kernel_ms, "wall_s": wall,
             "swap_accept": float(np.nanmean(sh)),
             "cold_draws_in_upper_mode": upper,
+            "cold_summary": "label-invariant (min(mu1, mu2), max(mu1, mu2))",
             "cold_ess_min": s["min_ess"],
-            "cold_ess_per_s": s["min_ess"] / (fit.kernel_ms / 1e3),
+            "cold_ess_per_s": s["min_ess"] / ks if gate else None,
             "cold_rhat_max": s["max_rhat"],
+            "cold_rhat_max_raw_labels": raw["max_rhat"],
+            "cold_mean": [p["mean"] for p in s["params"]],
+            "cold_sd": [p["sd"] for p in s["params"]],
             "engine": "compiled per model" if fit.adaptation.get("compiled")
                       else "interpreter"}
 
@@ -324,26 +429,37 @@ def batch_section(a):
     gen_s = time.perf_counter() - t0
     n_tot = sum(w.n_obs for w in wls)
     rng = np.random.default_rng(a.seed)
-    post = [workloads.regression_posterior(w.data) for w in wls[:8]]
     # chains start around the truth (posterior sds are 1e-3 .. 3e-2)
     inits = [w.meta["true"] + 0.01 * rng.standard_normal((64, 10)) for w in wls]
     t0 = time.perf_counter()
     bf = gpu_metropolis_batch([w.model for w in wls], [w.data for w in wls],
                               n_chains=64, n_iter=a.batch_iters,
+                              warmup=a.batch_warmup,
                               init=inits, proposal_sd=0.003, seed=a.seed,
                               tiling="auto", jit=a.jit)
     wall = time.perf_counter() - t0
     evals = n_tot * 64 * a.batch_iters
-    # closed-form check on the first models
-    worst = 0.0
-    for fit, p in zip(bf.fits[:8], post):
+    # Every model against its closed-form posterior, with its R-hat and ESS.
+    t0 = time.perf_counter()
+    worst, rh, es, sdr, zs = 0.0, [], [], [], []
+    for fit, w in zip(bf.fits, wls):
+        p = workloads.regression_posterior(w.data)
         s = summarize(fit.draws)
+        fit.draws = None                 # free the host copy as we go
+        rh.append(s["max_rhat"])
+        es.append(s["min_ess"])
         for j, row in enumerate(s["params"]):
-            worst = max(worst, abs(row["mean"] - p["mean"][j]) / row["mcse"])
+            zs.append((row["mean"] - p["mean"][j]) / row["mcse"])
+            sdr.append(row["sd"] / p["sd"][j])
+    worst = float(np.max(np.abs(zs)))
+    check_s = time.perf_counter() - t0
     info = dict(bf.fits[0].adaptation)
+    rh, es = np.array(rh), np.array(es)
+    z = np.array(zs)
     return {"workload": f"{len(wls)} linear regressions, {n_tot} FP32 rows x "
                         f"9 cols, DE-MC 64 chains each, {a.batch_iters} "
-                        f"iterations ({bf.batch_sizes[0]} per batch)",
+                        f"iterations ({a.batch_warmup} warmup, "
+                        f"{bf.batch_sizes[0]} per batch)",
             "obs_evals_per_s": evals / (bf.kernel_ms / 1e3),
             "graph_ms": bf.kernel_ms, "wall_s_incl_setup": wall,
             "data_gen_s": gen_s,
@@ -352,7 +468,17 @@ def batch_section(a):
                                 for k, v in bf.tile_timings.items()},
             "workgroups_per_launch": bf.n_workgroups,
             "launches": 2 * info["n_batches"],
-            "closed_form_max_mean_err_mcse_first8": worst,
+            "closed_form_max_mean_err_mcse": worst,
+            "closed_form_frac_within_3_mcse": float(np.mean(np.abs(z) <= 3)),
+            "closed_form_err_mcse_rms": float(np.sqrt(np.mean(z * z))),
+            "closed_form_sd_ratio_range": [float(min(sdr)), float(max(sdr))],
+            "rhat_max": float(rh.max()),
+            "models_rhat_below_1_01": int((rh < 1.01).sum()),
+            "ess_min": float(es.min()),
+            "ess_median": float(np.median(es)),
+            "ess_per_s_min_model": float(es.min() / (bf.kernel_ms / 1e3)),
+            "ess_per_s_all_models": float(es.sum() / (bf.kernel_ms / 1e3)),
+            "check_s": check_s,
             "mean_accept": float(np.mean([f.accept_rate.mean()
                                           for f in bf.fits]))}
 
@@ -530,12 +656,13 @@ def main():
     init = normals(a.seed, C * P).reshape(C, P)
     sd0 = np.full((C, P), 0.1)
     t0 = time.perf_counter()
-    wu = adaptive_warmup(call, P, C, init, sd0, [a.iters_per_step] * a.warmup,
-                         a.seed)
+    plan = ad.warmup_batches(a.warmup * a.iters_per_step)
+    wu = adaptive_warmup(call, P, C, init, sd0, plan, a.seed)
     warm_s = time.perf_counter() - t0
-    print(f"[bench] warmup {a.warmup} x {a.iters_per_step} it done in "
-          f"{warm_s:.1f} s, accept {wu.accept_history[:, -1].mean():.3f}",
-          file=sys.stderr, flush=True)
+    warm_acc = float(wu.accept_history[:, -1].mean())
+    print(f"[bench] warmup {sum(plan)} it in {len(plan)} batches done in "
+          f"{warm_s:.1f} s, last batch accept {warm_acc:.3f} (target "
+          f"{ad.target_accept(P)})", file=sys.stderr, flush=True)
 
     state["kernel_ms"].clear()
     draws = []
@@ -543,6 +670,7 @@ def main():
     # final_state of the last warmup batch is the exact FP64 state; the
     # warmup hands back the f32 draws, so continue from them as R does.
     mode = 2 if wu.L is not None else 0
+    accs = []
     t0 = time.perf_counter()
     for k in range(a.steps):
         # The draws arrive on the host in the library's (param, chain,
@@ -552,6 +680,7 @@ def main():
                  mode=mode, L=wu.L, raw_draws=True)
         cur = r.final_state
         draws.append(r.draws_pct)
+        accs.append(r.accept_rate)
         print(f"[bench] step {k + 1}/{a.steps} kernel {r.kernel_ms:.1f} ms",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
@@ -562,6 +691,19 @@ def main():
     d = np.concatenate([x.transpose(2, 1, 0) for x in draws]).astype(np.float64)
     summ = summarize(d)
     ess_min = summ["min_ess"]
+    # The exact posterior of (log_a, log_b) from the data's sufficient
+    # statistics, and its Laplace approximation N(mode, I^-1).
+    gp = workloads.gamma_posterior(w.data)
+    mcse = np.array([p["mcse"] for p in summ["params"]])
+    pmean = np.array([p["mean"] for p in summ["params"]])
+    psd = np.array([p["sd"] for p in summ["params"]])
+    posterior_check = {
+        "reference": "exact posterior by quadrature on the sufficient "
+                     "statistics; Laplace N(mode, I^-1) beside it",
+        "exact_mean": gp["mean"].tolist(), "exact_sd": gp["sd"].tolist(),
+        "max_err_mcse": float(np.max(np.abs(pmean - gp["mean"]) / mcse)),
+        "laplace_max_err_mcse": float(np.max(np.abs(pmean - gp["mode"]) / mcse)),
+        "sd_ratio": (psd / gp["sd"]).tolist()}
     kms = np.array(state["kernel_ms"])
     avg_launch_s = kms.mean() / 1e3
     evals_per_launch = C * N * a.iters_per_step
@@ -595,9 +737,14 @@ def main():
                    "n_obs": N, "chains": C,
                    "iters_per_step": a.iters_per_step,
                    "parallelism": "single GPU"},
-        "ess_per_s": ess_min / wall,
+        "ess_per_s": ess_min / wall if summ["max_rhat"] < 1.01 else None,
         "rhat_max": summ["max_rhat"],
         "ess_min": ess_min,
+        "sampling_accept": float(np.mean(accs)),
+        "warmup": {"iterations": sum(plan), "batches": len(plan),
+                   "last_batch_accept": warm_acc,
+                   "target_accept": ad.target_accept(P), "wall_s": warm_s},
+        "posterior_check": posterior_check,
         "posterior": [{"mean": p["mean"], "sd": p["sd"], "mcse": p["mcse"]}
                       for p in summ["params"]],
         "roofline": {
@@ -636,6 +783,7 @@ def main():
             "valu_frac": w_exec * evals_per_launch / (vm_avg / 1e3) / VALU_PEAK}
     out["engine"] = "compiled per model" if a.jit else "interpreter"
     dm.close()
+    out["gaussian_mean"] = gaussian_mean_section(a)
     if not a.no_da:
         out["surrogate_da"] = da_section(a)
     if not a.no_pt:
@@ -647,7 +795,8 @@ def main():
         out["de_sync"] = de_sync_section(a)
         out["mala"] = mala_section(a)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds, cur, wu.sd, wu.L,
+                                           a.seed + wu.batches_run + a.steps)
     print(json.dumps(out), flush=True)
 
 

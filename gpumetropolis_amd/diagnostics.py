@@ -9,19 +9,29 @@ Restates the reference's R diagnostics so the numbers are comparable:
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
+import scipy.fft as _fft
+
+# FFT worker threads: the cores this process may use, capped by
+# OMP_NUM_THREADS where the host sets it.
+_WORKERS = len(os.sched_getaffinity(0))
+if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+    _WORKERS = max(1, min(_WORKERS, int(os.environ["OMP_NUM_THREADS"])))
 
 
 def _acf(x: np.ndarray, lag_max: int) -> np.ndarray:
     """Sample autocorrelations 1..lag_max of each column (stats::acf with
-    demean=TRUE: autocovariances divided by n, normalised by lag 0)."""
+    demean=TRUE: autocovariances divided by n, normalised by lag 0).  The
+    transform is zero-padded to at least n + lag_max points, so the circular
+    products at lags 0..lag_max are the linear ones."""
     n = x.shape[0]
     xc = x - x.mean(axis=0, keepdims=True)
-    nfft = 1
-    while nfft < 2 * n:
-        nfft *= 2
-    f = np.fft.rfft(xc, n=nfft, axis=0)
-    acov = np.fft.irfft(f * np.conj(f), n=nfft, axis=0)[: lag_max + 1] / n
+    nfft = _fft.next_fast_len(n + lag_max + 1, real=True)
+    f = _fft.rfft(xc, n=nfft, axis=0, workers=_WORKERS)
+    pw = f.real * f.real + f.imag * f.imag
+    acov = _fft.irfft(pw, n=nfft, axis=0, workers=_WORKERS)[: lag_max + 1] / n
     return acov[1:] / acov[0:1]
 
 
@@ -80,20 +90,40 @@ def rhat(draws: np.ndarray) -> float:
     return float(np.sqrt(var_hat / within))
 
 
+def _rhat_params(d: np.ndarray) -> np.ndarray:
+    """rhat() of every parameter of an (n_iter, n_chains, n_params) array."""
+    n = d.shape[0]
+    half = n // 2
+    if half < 2:
+        raise ValueError("need at least 4 post-warmup iterations per chain "
+                         "for split R-hat.")
+    splits = np.concatenate([d[:half], d[n - half:]], axis=1)
+    sub_means = splits.mean(axis=0)                  # (2C, P)
+    sub_vars = splits.var(axis=0, ddof=1)
+    between = half * sub_means.var(axis=0, ddof=1)
+    within = sub_vars.mean(axis=0)
+    var_hat = ((half - 1) * within + between) / half
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.sqrt(var_hat / within)
+    return np.where(within > 0, r, np.nan)
+
+
 def summarize(draws: np.ndarray) -> dict:
     """Per-parameter mean, sd, R-hat, ESS and MCSE (R/diagnose.R:64-84) of an
-    (n_iter, n_chains, n_params) array."""
+    (n_iter, n_chains, n_params) array; every parameter's chains go through
+    one autocorrelation transform."""
     d = np.asarray(draws, dtype=np.float64)
     if d.ndim == 2:
         d = d[:, :, None]
-    rows = []
-    for p in range(d.shape[2]):
-        M = d[:, :, p]
-        v = M.ravel()
-        e = ess(M)
-        rows.append({"mean": float(v.mean()), "sd": float(v.std(ddof=1)),
-                     "rhat": rhat(M), "ess": e,
-                     "mcse": float(v.std(ddof=1) / np.sqrt(max(e, 1.0)))})
+    n, C, P = d.shape
+    e = ess_chains(d.reshape(n, C * P)).reshape(C, P).sum(axis=0)
+    r = _rhat_params(d)
+    flat = d.reshape(n * C, P)
+    mean = flat.mean(axis=0)
+    sd = flat.std(axis=0, ddof=1)
+    rows = [{"mean": float(mean[p]), "sd": float(sd[p]), "rhat": float(r[p]),
+             "ess": float(e[p]),
+             "mcse": float(sd[p] / np.sqrt(max(e[p], 1.0)))} for p in range(P)]
     return {"params": rows,
-            "max_rhat": max(r["rhat"] for r in rows),
-            "min_ess": min(r["ess"] for r in rows)}
+            "max_rhat": max(x["rhat"] for x in rows),
+            "min_ess": min(x["ess"] for x in rows)}

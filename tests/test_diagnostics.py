@@ -140,3 +140,20 @@ def test_summarize_mcse_of_iid_mean():
         assert abs(p["mean"]) < 4 * p["mcse"]
     assert s["max_rhat"] < 1.01
     assert s["min_ess"] > 15000
+
+
+def test_summarize_equals_the_per_parameter_functions():
+    """summarize() transforms all parameters at once; each parameter's ESS and
+    R-hat equal ess() and rhat() of its (n_iter, n_chains) matrix."""
+    x = _ar1(3000, 0.8, 12, seed=11).reshape(3000, 4, 3)
+    x[:, :, 2] = x[:, :, 2] * 3.0 + 1.0
+    x[:, 1, 1] += 0.5            # one chain off: R-hat above 1
+    s = dg.summarize(x)
+    for p in range(3):
+        M = x[:, :, p]
+        row = s["params"][p]
+        assert row["ess"] == pytest.approx(dg.ess(M), rel=1e-12)
+        assert row["rhat"] == pytest.approx(dg.rhat(M), rel=1e-12)
+        assert row["mean"] == pytest.approx(M.mean(), rel=1e-12)
+        assert row["sd"] == pytest.approx(M.std(ddof=1), rel=1e-12)
+    assert s["max_rhat"] == pytest.approx(dg.rhat(x[:, :, 1]), rel=1e-12)
