@@ -307,7 +307,8 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
   if (!setup(code, code_len, consts, n_consts, n_params, data, n_cols, n_obs,
              points, n_points, depth, &S))
     return 1;
-  const uint32_t ns = (n_obs + kGradSliceObs - 1) / kGradSliceObs;
+  const uint32_t ns =
+      (uint32_t)(((uint64_t)n_obs + kGradSliceObs - 1) / kGradSliceObs);
   const uint32_t chunk = (uint32_t)std::max<size_t>(
       1, std::min<size_t>(n_points,
                           kGradPartialBytes / ((size_t)ns * (n_params + 1) * 8)));
