@@ -21,7 +21,7 @@ OBJDIR = os.path.join(os.path.dirname(HERE), "_build")
 SOURCES = ["chain_kernel_inst0.hip", "chain_kernel_inst1.hip",
            "chain_kernel_inst2.hip", "chain_kernel_inst3.hip",
            "da_kernels.hip", "batch_kernels.hip", "mlp.hip", "chain_kernel.hip",
-           "eval64.hip", "gpum_hip.cpp", "da.cpp", "batch.cpp", "jit.cpp",
+           "eval64.hip", "noalloc.hip", "gpum_hip.cpp", "da.cpp", "batch.cpp", "jit.cpp",
            "lower.cpp", "eval64.cpp"]
 HEADERS = ["vm.h", "lower.h", "chain_kernel.h", "chain_kernels.h",
            "device_vm.h", "math64.h", "philox.h", "runtime.h", "step_common.h",

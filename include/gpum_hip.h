@@ -206,6 +206,30 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
                         uint32_t n_obs, const double* points, uint32_t n_points,
                         double* out);
 
+/* gpum_hip_grad_batch without device allocation.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, serving loops, harnesses that forbid it).  `d_data` (FP64,
+ * column-major: column c at d_data[c * n_obs]), `d_points` (point-major) and
+ * `d_out` (n_points * n_params, point-major) are DEVICE buffers the caller
+ * owns; the program's tape, its constants and the partial sums live in an
+ * 8 MiB array of the library, one per GPU, held until the process ends.
+ * Limits beyond gpum_hip_grad_batch's: at most 4096 constants, and
+ * ceil(n_obs / 4096) * (n_params + 1) * 8 bytes of partial sums per point
+ * must fit in the array (about 2.4e8 observations at 16 parameters); larger
+ * calls return an error.  Points are processed in chunks that fit.  Results
+ * equal gpum_hip_grad_batch's bit for bit (the interpreted kernels; this
+ * entry never compiles a per-program kernel, which would allocate).  Calls
+ * are serialised by the library's lock; every call reuses the array, so
+ * calls from two processes sharing one GPU each have their own copy, and
+ * the array keeps the last call's contents.  Returns after the results are
+ * in d_out. */
+int gpum_hip_grad_batch_noalloc(const uint32_t* code, size_t code_len,
+                                const double* consts, size_t n_consts,
+                                uint32_t n_params, const double* d_data,
+                                uint32_t n_cols, uint32_t n_obs,
+                                const double* d_points, uint32_t n_points,
+                                double* d_out);
+
 /* Device time (ms) of the kernels of this thread's last
  * gpum_hip_loglik_pointwise or gpum_hip_grad_batch call (HIP events on the
  * library's stream; copies excluded).  No counterpart in the reference. */
@@ -330,7 +354,9 @@ typedef struct gpum_train_opts {
   uint32_t seed;      /* minibatch sampling stream */
   double lr;          /* initial learning rate */
   double lr_final;    /* cosine decay target */
-  double beta1, beta2, eps;
+  double beta1;       /* Adam moment decays and epsilon */
+  double beta2;
+  double eps;
 } gpum_train_opts;
 
 /* Train on n (theta, loglik) pairs; loss_trace (steps doubles, may be NULL)
