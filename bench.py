@@ -43,9 +43,15 @@ VALU_PEAK = 256 * 64 * 2.4e9  # FP32 lane-ops/s, non-packed (one slot per op)
 # profiles/compiled_chain_kernel_gamma_1e6obs_4096chains.md.  Reported only
 # for the launch shape it was measured on.
 PROFILED_TRAFFIC = {
-    ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 29_462_164 * 1024 * 2.0,
-                                         "write_bytes": 70_256 * 1024 * 1.0},
+    ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 17_180_000 * 1024 * 2.0,
+                                         "write_bytes": 70_260 * 1024 * 1.0},
 }
+# Instruction mix of the parallel-tempering launch (4000 iterations, 4096
+# chains, 2e5 rows; SQ_INSTS_VALU and SQ_INSTS_VALU_TRANS_F32 over 64
+# evaluations), from the rocprofv3 pass in
+# profiles/pt_chain_kernel_mixture_4096chains.md.
+PT_PROFILED_MIX = {"valu_insts_per_64_evals": 17.63,
+                   "trans_insts_per_64_evals": 2.01}
 HBM_PEAK = 8.0e12             # B/s
 
 
@@ -79,9 +85,14 @@ def parse():
     p.add_argument("--no-pt", action="store_true",
                    help="skip the parallel-tempering workload")
     p.add_argument("--pt-iters", type=int, default=4000)
+    p.add_argument("--pt-chunk", type=int, default=2000,
+                   help="iterations per continuation launch of the PT run")
+    p.add_argument("--pt-max-iters", type=int, default=16000)
     p.add_argument("--no-eval", action="store_true",
                    help="skip the FP64 pointwise / gradient, synchronous "
                         "DE and MALA sections")
+    p.add_argument("--no-gaussian", action="store_true",
+                   help="skip the Gaussian-mean (8 chains) section")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the many-model (512 regressions) section")
     p.add_argument("--batch-models", type=int, default=512)
@@ -90,6 +101,10 @@ def parse():
                         "64 chains of 4000 (half warmup) leave ~55 ESS per "
                         "chain, which holds split R-hat near 1.02")
     p.add_argument("--batch-warmup", type=int, default=2000)
+    p.add_argument("--only", choices=["pt", "batch", "da", "eval",
+                                      "gaussian"],
+                   help="run only this section and print its JSON (for "
+                        "profiling one kernel family); no headline line")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device-to-device copy of MIB MiB "
                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -367,53 +382,99 @@ def pt_section(a):
     """Parallel tempering on the bimodal mixture: 200,000 FP32 rows, 16
     geometric temperatures (beta 1 .. 0.01) x 256 replicas, swaps every 10
     iterations; batches, per-chain adaptation and swap sweeps run inside one
-    launch."""
-    from gpumetropolis_amd import workloads
-    from gpumetropolis_amd.api import gpu_metropolis
+    launch.  The configuration's 4000 iterations (half warmup, adapting) are
+    one launch; the exact states then continue, with the adapted proposals
+    and the same Philox stream and sweep parity, in chunks until the cold
+    chains' split R-hat is below 1.01 (as gpu_metropolis's R/adaptive.R
+    orchestration would by calling again).  ESS/s is over all the launches'
+    kernel time."""
+    from gpumetropolis_amd import _native, adaptive as ad, workloads
     from gpumetropolis_amd.diagnostics import summarize
 
     wl = workloads.mixture(n_iter=a.pt_iters)
+    m = wl.model
     G, R = wl.meta["n_temps"], wl.meta["n_replicas"]
     C = G * R
+    se = wl.meta["swap_every"]
+    temps = np.tile(wl.meta["temperatures"], R)
     rng = np.random.default_rng(a.seed)
     init = rng.normal(0.0, 4.0, (C, 2))
-    t0 = time.perf_counter()
-    fit = gpu_metropolis(wl.model, wl.data, init=init, proposal_sd=0.3,
-                         n_iter=a.pt_iters, method="pt",
-                         temperatures=wl.meta["temperatures"],
-                         swap_every=wl.meta["swap_every"], ladder_size=G,
-                         seed=a.seed, jit=a.jit)
-    wall = time.perf_counter() - t0
-    evals = C * wl.n_obs * a.pt_iters
-    cold = fit.draws[:, ::G, :]                 # beta = 1 member of each ladder
-    # The posterior's two modes are the label swaps (mu1, mu2) = (-3, 3),
-    # (3, -3): the likelihood is symmetric in the labels.  Diagnostics are
-    # taken on the label-invariant summary (min, max) of each draw; on the
-    # raw labels R-hat measures how often replicas cross between the
-    # mirror images, not convergence of the posterior.
-    upper = float(np.mean(cold[:, :, 0] > cold[:, :, 1]))
-    srt = np.sort(cold.astype(np.float64), axis=2)
-    s = summarize(srt)
-    raw = summarize(cold)
+    dm = _native.DeviceModel(m.loglik.code, m.loglik.consts, m.prior.code,
+                             m.prior.consts, 2, wl.data, 1, wl.n_obs,
+                             jit=a.jit)
+    warm = a.pt_iters // 2
+    sizes, nb = ad.split_batches(a.pt_iters, se)
+    wb = warm // se
+    seed = a.seed & 0xFFFFFFFF
+    r = dm.run(init, np.full((C, 2), 0.3), temps, a.pt_iters, seed,
+               final_state=True,
+               batches={"swap_every": se, "swap_group": G, "adapt_batches": wb})
+    first_ms = r.kernel_ms
+    kms = r.kernel_ms
+    sh = [r.swap_hist]
+    cold = [r.draws[warm:, ::G, :]]
+    state, sd, t, b = r.final_state, r.final_sd, a.pt_iters, nb
+    upper0 = float(np.mean(cold[0][:, :, 0] > cold[0][:, :, 1]))
+
+    def label_free(x):
+        # The likelihood is symmetric in the labels; the posterior's two
+        # modes are (mu1, mu2) = (-3, 3) and (3, -3).  On the raw labels
+        # R-hat measures how often replicas cross between the mirror
+        # images, not convergence; (min, max) is the invariant summary.
+        return np.sort(np.concatenate(x).astype(np.float64), axis=2)
+
+    s = summarize(label_free(cold))
+    while s["max_rhat"] >= 1.01 and t < a.pt_max_iters:
+        n = a.pt_chunk
+        r = dm.run(state, sd, temps, n, seed, final_state=True,
+                   batches={"swap_every": se, "swap_group": G,
+                            "adapt_batches": 0, "iter_offset": t,
+                            "batch0": b})
+        kms += r.kernel_ms
+        sh.append(r.swap_hist)
+        cold.append(r.draws[:, ::G, :])
+        state, t, b = r.final_state, t + n, b + n // se
+        s = summarize(label_free(cold))
+        print(f"[bench-pt] {t} it, cold R-hat {s['max_rhat']:.4f}",
+              file=sys.stderr, flush=True)
+    w_exec = dm.info()["w_exec"]
+    dm.close()
+    raw = summarize(np.concatenate(cold).astype(np.float64))
     gate = s["max_rhat"] < 1.01
-    sh = fit.adaptation["swap_history"]
-    ks = fit.kernel_ms / 1e3
+    swaps = np.concatenate(sh)
+    evals = C * wl.n_obs * a.pt_iters
+    rate = evals / (first_ms / 1e3)
+    issue = PT_PROFILED_MIX["valu_insts_per_64_evals"]
+    roof = {"bound": "valu", "kernel": "gpum_jit_kernel (fused PT, compiled "
+                                       "mixture program)",
+            "w_exec": w_exec, "achieved": w_exec * rate / 1e12,
+            "peak": VALU_PEAK / 1e12, "unit": "T VALU-slot/s",
+            "frac": w_exec * rate / VALU_PEAK,
+            "avg_launch_ms": first_ms,
+            # what the SIMDs issued: every VALU instruction (transcendentals
+            # included) as one slot per lane
+            "issued_slots_per_eval": issue,
+            "issue_frac": issue * rate / VALU_PEAK,
+            "trans_insts_per_64_evals":
+                PT_PROFILED_MIX["trans_insts_per_64_evals"]}
     return {"workload": f"bimodal mixture, {wl.n_obs} FP32 rows, {G} temperatures "
-                        f"x {R} replicas, {a.pt_iters} iterations, swaps every "
-                        f"{wl.meta['swap_every']}",
-            "obs_evals_per_s": evals / ks,
-            "kernel_ms": fit.kernel_ms, "wall_s": wall,
-            "swap_accept": float(np.nanmean(sh)),
-            "cold_draws_in_upper_mode": upper,
+                        f"x {R} replicas, {a.pt_iters} iterations ({warm} "
+                        f"adaptive warmup), swaps every {se}; continued in "
+                        f"chunks of {a.pt_chunk} until the cold R-hat < 1.01",
+            "obs_evals_per_s": rate,
+            "roofline": roof,
+            "kernel_ms_config_launch": first_ms,
+            "iterations_total": t, "kernel_ms_total": kms,
+            "swap_accept": float(np.mean(swaps[swaps != 255])),
+            "cold_draws_in_upper_mode": upper0,
             "cold_summary": "label-invariant (min(mu1, mu2), max(mu1, mu2))",
             "cold_ess_min": s["min_ess"],
-            "cold_ess_per_s": s["min_ess"] / ks if gate else None,
+            "cold_ess_per_s": s["min_ess"] / (kms / 1e3) if gate else None,
             "cold_rhat_max": s["max_rhat"],
             "cold_rhat_max_raw_labels": raw["max_rhat"],
             "cold_mean": [p["mean"] for p in s["params"]],
             "cold_sd": [p["sd"] for p in s["params"]],
-            "engine": "compiled per model" if fit.adaptation.get("compiled")
-                      else "interpreter"}
+            "engine": "compiled per model" if a.jit else "interpreter"}
 
 
 def batch_section(a):
@@ -632,6 +693,11 @@ def main():
 
     if not _native.available():
         raise RuntimeError("bench needs a gfx950 GPU: " + _native.last_error())
+    if a.only:
+        fn = {"pt": pt_section, "batch": batch_section, "da": da_section,
+              "eval": eval_section, "gaussian": gaussian_mean_section}[a.only]
+        print(json.dumps({a.only: fn(a)}), flush=True)
+        return
 
     w = workloads.gamma(n=a.n_obs, n_chains=a.chains)
     m = w.model
@@ -741,7 +807,7 @@ def main():
         "rhat_max": summ["max_rhat"],
         "ess_min": ess_min,
         "sampling_accept": float(np.mean(accs)),
-        "warmup": {"iterations": sum(plan), "batches": len(plan),
+        "warmup_adaptation": {"iterations": sum(plan), "batches": len(plan),
                    "last_batch_accept": warm_acc,
                    "target_accept": ad.target_accept(P), "wall_s": warm_s},
         "posterior_check": posterior_check,
@@ -783,7 +849,8 @@ def main():
             "valu_frac": w_exec * evals_per_launch / (vm_avg / 1e3) / VALU_PEAK}
     out["engine"] = "compiled per model" if a.jit else "interpreter"
     dm.close()
-    out["gaussian_mean"] = gaussian_mean_section(a)
+    if not a.no_gaussian:
+        out["gaussian_mean"] = gaussian_mean_section(a)
     if not a.no_da:
         out["surrogate_da"] = da_section(a)
     if not a.no_pt:
