@@ -5,7 +5,8 @@
 // hundreds of models is a host loop of calls, each uploading its data
 // (src/rust/src/chain_kernel.rs:393-413).  Here every model's program and
 // data stay on the device, one launch of the batched chain kernel advances
-// every chain of every model through one batch (workgroups ordered longest
+// every chain of every model through one batch (a model's workgroups
+// grouped on one XCD so they share its data in L2, each XCD's list longest
 // model first, so the dispatcher's in-order placement is a
 // longest-processing-time schedule), a small kernel does the between-batch
 // step of the reference's orchestrator, and the whole sequence of batches
@@ -241,11 +242,40 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     in.accept_hist = acc_hist.p + (size_t)nb * plan[k].chain_off;
     for (uint32_t c0 = 0; c0 < C; c0 += t.cpb) work.push_back(BatchWork{k, c0});
   }
-  // Longest models first: the in-order dispatcher then places the long
-  // workgroups early and fills in with the short ones.
-  std::stable_sort(work.begin(), work.end(), [&](const BatchWork& x, const BatchWork& y) {
-    return models[x.model].model->n_obs > models[y.model].model->n_obs;
-  });
+  // XCD-aware order.  The dispatcher hands workgroup w to XCD w mod 8, and
+  // each of MI355X's 8 XCDs has its own L2.  A model's workgroups (its
+  // chain groups) stream the same data tiles every iteration, so they get
+  // ids congruent mod 8 and adjacent in that XCD's order: they start side
+  // by side on one XCD and the tiles one of them fetches serve the others
+  // from L2 (with consecutive ids they ran on different XCDs and every
+  // workgroup streamed the model from HBM).  Models are assigned largest
+  // first to the XCD with the least work so far, and within an XCD the
+  // longest run first, so the in-order dispatcher fills in with short ones
+  // at the end; the shorter XCD lists are padded with empty items.
+  {
+    std::vector<uint32_t> by_size(n_models);
+    for (uint32_t k = 0; k < n_models; ++k) by_size[k] = k;
+    auto work_of = [&](uint32_t k) {
+      return (double)models[k].model->n_obs * models[k].n_chains;
+    };
+    std::stable_sort(by_size.begin(), by_size.end(), [&](uint32_t x, uint32_t y) {
+      return models[x].model->n_obs > models[y].model->n_obs;
+    });
+    std::vector<std::vector<BatchWork>> xq(kXcds);
+    std::vector<double> load(kXcds, 0.0);
+    for (uint32_t k : by_size) {
+      const uint32_t x = (uint32_t)(std::min_element(load.begin(), load.end()) -
+                                    load.begin());
+      for (const BatchWork& bw : work)
+        if (bw.model == k) xq[x].push_back(bw);
+      load[x] += work_of(k);
+    }
+    size_t longest = 0;
+    for (const auto& q : xq) longest = std::max(longest, q.size());
+    work.assign(longest * kXcds, BatchWork{kNoBatchModel, 0});
+    for (uint32_t x = 0; x < kXcds; ++x)
+      for (size_t i = 0; i < xq[x].size(); ++i) work[i * kXcds + x] = xq[x][i];
+  }
   if (!d_args.upload(args.data(), n_models, "upload launch arguments") ||
       !d_hots.upload(hots.data(), n_models, "upload launch arguments") ||
       !d_info.upload(info.data(), n_models, "upload launch arguments") ||

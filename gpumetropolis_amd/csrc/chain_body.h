@@ -598,6 +598,8 @@ __device__ __forceinline__ void batch_body(const BatchWork* __restrict__ work,
                                            BatchStep step) {
   __shared__ ChainArgs a_sh;
   const BatchWork w = work[blockIdx.x];
+  // Padding of the XCD-aware order: the whole workgroup leaves at once.
+  if (w.model == kNoBatchModel) return;
   ChainHot H = hots[w.model];
   H.n_iter = step.n_iter;
   if (threadIdx.x == 0) {

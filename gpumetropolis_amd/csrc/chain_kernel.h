@@ -112,9 +112,12 @@ constexpr uint32_t kTileIssues = (kTileF4 + 1023) / 1024;
 
 // Batched launches: one workgroup per (model, first chain) item.
 struct BatchWork {
-  uint32_t model;
+  uint32_t model;   // kNoBatchModel: an empty item (the workgroup exits)
   uint32_t chain0;
 };
+constexpr uint32_t kNoBatchModel = 0xFFFFFFFFu;
+// XCDs of an MI355X; the dispatcher hands workgroup w to XCD w mod kXcds.
+constexpr uint32_t kXcds = 8;
 // Per-launch values of a batched run (one batch of iterations).
 struct BatchStep {
   uint32_t seed_add;   // added to each model's seed
