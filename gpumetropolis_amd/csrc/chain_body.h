@@ -5,7 +5,8 @@
 //                   observation it reads from LDS (chain j's uniform table
 //                   is U + j * H.n_uniform_pad);
 //   E::run(H, U, tile, ld, lane, T)
-//                   evaluate the program on one chunk into T[kCw][kOpt];
+//                   evaluate the program on one chunk into T[kCw][kOpt]
+//                   (`tile`: the chunk, `ld`: its column stride, 64*kOpt);
 //   E::mode(H)      how a chunk is summed (SumMode);
 //   E::acc_state(H, U), E::run_acc(H, state, U, tile, ld, lane, s)
 //                   SUM_AFFINE: what a tile pass keeps (e.g. the uniforms),
@@ -310,30 +311,67 @@ __device__ __noinline__ void chain_finish(const ChainArgs* __restrict__ A,
 __shared__ float4 g_tile_a[kTileF4];
 __shared__ float4 g_tile_b[kTileF4];
 
+// LDS tile layout, chunk-major: the tile's observations in chunks of
+// 64*OPT, each chunk holding its n_cols columns one after another (64*OPT
+// floats each, consecutive observations, as in HBM).  Within a chunk every
+// column is then a constant offset from one base, so a compiled program's
+// reads of its columns carry immediate offsets, and a chunk costs one
+// address step however many columns it reads.  A (chunk, column) segment
+// is 16*OPT float4s, a multiple of 64 (OPT is 4, 8 or 16).
+// HBM float4 of LDS float4 i of tile ti (n4 float4s per column of a tile,
+// ld4 float4s per column in HBM).
+template <int OPT>
+__device__ __forceinline__ size_t tile_src4(uint32_t i, uint32_t n_cols,
+                                            uint32_t ld4, uint32_t n4,
+                                            uint32_t ti) {
+  constexpr uint32_t seg4 = 16u * OPT;
+  const uint32_t seg = i / seg4, sub = i - seg * seg4;
+  const uint32_t chunk = seg / n_cols, col = seg - chunk * n_cols;
+  return (size_t)col * ld4 + (size_t)ti * n4 + (size_t)chunk * seg4 + sub;
+}
+
 // Asynchronous tile copy: global_load_lds moves 16 bytes per lane from
 // HBM straight into LDS (no VGPR staging); lane l of wave w writes float4
 // i0 + l where i0 = k*blockDim + 64*w is wave-uniform.  A tile holds a
-// multiple of 64 float4s per column, so a wave's 64 slots are all inside
-// the tile or all outside.  Streaming launches use 1024 threads, so a tile
-// takes at most kTileIssues copies per wave; the loop is unrolled so that
-// every copy in flight keeps its own address registers.
-__device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
-                                               uint32_t ld4, uint32_t n4,
-                                               uint32_t tile4,
-                                               uint32_t wave_base, int lane,
-                                               uint32_t ti, float4* dst) {
+// multiple of 64 float4s, and 64 consecutive LDS float4s lie in one
+// (chunk, column) segment, so a wave's slots are all inside the tile or all
+// outside and read 64 consecutive HBM float4s.  Streaming launches use 1024
+// threads, so a tile takes at most kTileIssues copies per wave.  Slots past
+// the tile repeat its last 64 float4s (identical bytes to identical
+// addresses), which keeps the sequence free of branches.
+struct TileCopy {
+  uint32_t lds[kTileIssues];  // LDS float4 index of each copy slot
+  size_t src[kTileIssues];    // its HBM float4 offset within tile 0
+};
+
+// The wave's copy slots, computed once per launch: they are wave-uniform
+// (scalar registers), and a tile's copy only adds ti * n4.
+template <int OPT>
+__device__ __forceinline__ TileCopy tile_copy_plan(uint32_t n_cols,
+                                                   uint32_t ld4, uint32_t n4,
+                                                   uint32_t tile4,
+                                                   uint32_t wave_base) {
+  TileCopy c;
 #pragma unroll
   for (uint32_t k = 0; k < kTileIssues; ++k) {
-    // Slots past the tile repeat its last 64 float4s (identical bytes to
-    // identical addresses), which keeps the sequence free of branches.
-    // n4 is a multiple of 64, so the column is uniform across the wave and
-    // the address is a scalar base plus the lane's 16-byte offset.
-    const uint32_t i0 = min(wave_base + k * 1024u, tile4 - 64u);
-    const uint32_t col = i0 / n4, off0 = i0 - col * n4;
-    const float4* base = src + ((size_t)col * ld4 + (size_t)ti * n4 + off0);
+    c.lds[k] = min(wave_base + k * 1024u, tile4 - 64u);
+    c.src[k] = tile_src4<OPT>(c.lds[k], n_cols, ld4, n4, 0);
+  }
+  return c;
+}
+
+// The loop is unrolled so that every copy in flight keeps its own address
+// registers.
+__device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
+                                               const TileCopy& c, uint32_t n4,
+                                               int lane, uint32_t ti,
+                                               float4* dst) {
+#pragma unroll
+  for (uint32_t k = 0; k < kTileIssues; ++k) {
+    const float4* base = src + (c.src[k] + (size_t)ti * n4);
     __builtin_amdgcn_global_load_lds(
         (const void*)(base + lane),
-        (__attribute__((address_space(3))) void*)(dst + i0), 16, 0, 0);
+        (__attribute__((address_space(3))) void*)(dst + c.lds[k]), 16, 0, 0);
   }
 }
 
@@ -349,8 +387,9 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
                                                int lane,
                                                double (&acc)[E::kCw]) {
   constexpr int OPT = E::kOpt, CW = E::kCw;
-  const uint32_t chunk = 64u * OPT;
+  constexpr uint32_t chunk = 64u * OPT;
   const uint32_t nchunks = H.tile_len / chunk;
+  const uint32_t cstride = H.n_cols * chunk;  // floats per chunk of the tile
   // Chunks wholly inside the data, and chunks holding any observation.  The
   // counts are computed once, so the loops below carry no per-chunk test
   // and their counters stay scalar.
@@ -362,13 +401,13 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
   uint32_t ck = part;
 #pragma unroll 2
   for (; ck < nfull; ck += wpc) {
-    const uint32_t base = ck * chunk;
+    const float* cb = tile + ck * cstride;
     float s[CW];
     if constexpr (MODE == SUM_AFFINE) {
-      E::run_acc(H, ust, U, tile + base, H.tile_len, lane, s);
+      E::run_acc(H, ust, U, cb, chunk, lane, s);
     } else {
       float T[CW][OPT];
-      E::run(H, U, tile + base, H.tile_len, lane, T);
+      E::run(H, U, cb, chunk, lane, T);
 #pragma unroll
       for (int j = 0; j < CW; ++j) {
         if constexpr (MODE == SUM_SQUARE) {
@@ -386,12 +425,11 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
     for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
   }
   for (; ck < nlive; ck += wpc) {
-    const uint32_t base = ck * chunk;
-    const uint32_t first = t0 + base;
+    const uint32_t first = t0 + ck * chunk;
     // The last chunk of the data: observations past n_obs count zero.
     float s[CW];
     float T[CW][OPT];
-    E::run(H, U, tile + base, H.tile_len, lane, T);
+    E::run(H, U, tile + ck * cstride, chunk, lane, T);
 #pragma unroll
     for (int j = 0; j < CW; ++j) {
       s[j] = 0.0f;
@@ -473,24 +511,23 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
 
   // The copy of issue_tile_dma written out here: as a call it raises the
   // scalar register pressure of the iteration loop.
+  constexpr int OPT = E::kOpt;
+  const TileCopy cp = tile_copy_plan<OPT>(H.n_cols, ld4, n4, tile4, wave_base);
   auto issue_tile = [&](uint32_t ti, float4* dst) {
 #pragma unroll
     for (uint32_t k = 0; k < kTileIssues; ++k) {
-      const uint32_t i0 = min(wave_base + k * 1024u, tile4 - 64u);
-      const uint32_t col = i0 / n4, off0 = i0 - col * n4;
-      const float4* base = src + ((size_t)col * ld4 + (size_t)ti * n4 + off0);
+      const float4* base = src + (cp.src[k] + (size_t)ti * n4);
       __builtin_amdgcn_global_load_lds(
           (const void*)(base + lane),
-          (__attribute__((address_space(3))) void*)(dst + i0), 16, 0, 0);
+          (__attribute__((address_space(3))) void*)(dst + cp.lds[k]), 16, 0,
+          0);
     }
   };
 
   if constexpr (!STREAM) {
     // The whole (padded) data set stays in LDS for every iteration.
-    for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-      uint32_t col = i / n4, off = i - col * n4;
-      res[i] = src[(size_t)col * ld4 + off];
-    }
+    for (uint32_t i = tid; i < tile4; i += blockDim.x)
+      res[i] = src[tile_src4<OPT>(i, H.n_cols, ld4, n4, 0)];
   } else {
     if (H.stream_mode == 1 && H.n_vm > 0) issue_tile(0, g_tile_a);
   }
@@ -530,10 +567,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
         // Rows too wide for two buffers: synchronous copy into A.
         for (uint32_t ti = 0; ti < n_tiles; ++ti) {
           __syncthreads();
-          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-            uint32_t col = i / n4, off = i - col * n4;
-            g_tile_a[i] = src[(size_t)col * ld4 + (size_t)ti * n4 + off];
-          }
+          for (uint32_t i = tid; i < tile4; i += blockDim.x)
+            g_tile_a[i] = src[tile_src4<OPT>(i, H.n_cols, ld4, n4, ti)];
           __syncthreads();
           if (active)
             tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
@@ -653,7 +688,7 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t item_sh;
   __shared__ PointsPlan plan_sh;
-  constexpr int CW = E::kCw;
+  constexpr int CW = E::kCw, OPT = E::kOpt;
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63;
   // Wave w evaluates slots w*CW .. w*CW + CW-1 of the item's group.
@@ -667,6 +702,12 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
   const int NU = (int)H.n_uniform_pad;
   float* Uall = reinterpret_cast<float*>(smem);
   float* Uw = Uall + wave * CW * NU;
+  const uint32_t n4 = tile_len / 4;
+  const uint32_t tile4 = H.n_cols * n4;
+  const uint32_t ld4 = H.ld / 4;
+  const float4* src = reinterpret_cast<const float4*>(H.data);
+  const TileCopy cp =
+      tile_copy_plan<OPT>(H.n_cols, ld4, n4, tile4, (uint32_t)(wave * 64));
   bool loaded = false;  // resident layout: data loaded with the first item
   for (;;) {
     __syncthreads();  // previous item's reads of U and the tiles are done
@@ -687,11 +728,6 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
       points_hoist<typename E::Params>(P, kl, Uw + lane * NU);
     const uint32_t ti0 = min(sl * tps, n_tiles);
     const uint32_t ti1 = min(ti0 + tps, n_tiles);
-    const uint32_t n4 = tile_len / 4;
-    const uint32_t tile4 = H.n_cols * n4;
-    const uint32_t ld4 = H.ld / 4;
-    const float4* src = reinterpret_cast<const float4*>(H.data);
-    const uint32_t wave_base = (uint32_t)(wave * 64);
     double acc[CW];
 #pragma unroll
     for (int j = 0; j < CW; ++j) acc[j] = 0.0;
@@ -700,10 +736,8 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
         // One tile holds the whole data set; load it once per workgroup.
         if (!loaded) {
           float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
-          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-            uint32_t col = i / n4, off = i - col * n4;
-            res[i] = src[(size_t)col * ld4 + off];
-          }
+          for (uint32_t i = tid; i < tile4; i += blockDim.x)
+            res[i] = src[tile_src4<OPT>(i, H.n_cols, ld4, n4, 0)];
           loaded = true;
         }
         __syncthreads();
@@ -712,18 +746,18 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
                        0, 0, 1, Uw, lane, acc);
       } else if (H.stream_mode == 1) {
         if (ti0 < ti1)
-          issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti0, g_tile_a);
+          issue_tile_dma(src, cp, n4, lane, ti0, g_tile_a);
         __syncthreads();
         for (uint32_t ti = ti0; ti < ti1; ti += 2) {
           const bool two = ti + 1 < ti1;
           if (two)
-            issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 1, g_tile_b);
+            issue_tile_dma(src, cp, n4, lane, ti + 1, g_tile_b);
           if (active)
             tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
                          ti * tile_len, 0, 1, Uw, lane, acc);
           __syncthreads();
           if (ti + 2 < ti1)
-            issue_tile_dma(src, ld4, n4, tile4, wave_base, lane, ti + 2, g_tile_a);
+            issue_tile_dma(src, cp, n4, lane, ti + 2, g_tile_a);
           if (two && active)
             tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_b),
                          (ti + 1) * tile_len, 0, 1, Uw, lane, acc);
@@ -732,10 +766,8 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
       } else {
         for (uint32_t ti = ti0; ti < ti1; ++ti) {
           __syncthreads();
-          for (uint32_t i = tid; i < tile4; i += blockDim.x) {
-            uint32_t col = i / n4, off = i - col * n4;
-            g_tile_a[i] = src[(size_t)col * ld4 + (size_t)ti * n4 + off];
-          }
+          for (uint32_t i = tid; i < tile4; i += blockDim.x)
+            g_tile_a[i] = src[tile_src4<OPT>(i, H.n_cols, ld4, n4, ti)];
           __syncthreads();
           if (active)
             tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
