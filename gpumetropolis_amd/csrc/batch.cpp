@@ -300,10 +300,11 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
       if (failed) return 1;
     }
   }
-  if (!jf && cw > 2) {
-    // The interpreter's batched kernels hold one or two chains per wave.
-    set_err("chains_per_wave %u runs on compiled kernels only (every model "
-            "compiled, one program)", cw);
+  if (!jf && !interp_has_variant(force.v)) {
+    // The interpreter's batched kernels hold one or two chains per wave,
+    // four for affine programs.
+    set_err("chains_per_wave %u runs on compiled kernels (every model "
+            "compiled, one program) or for affine programs only", cw);
     return 1;
   }
   auto enqueue = [&]() -> hipError_t {

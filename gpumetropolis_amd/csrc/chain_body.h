@@ -17,7 +17,8 @@
 //                   consts, p)
 //                   the FP64 parameter programs (hoisted values into the
 //                   uniform table; the prior).
-// The interpreter (VmEval) and code generated per model share this file.
+// The interpreters (VmEval, and VmAffineEval for affine programs) and code
+// generated per model share this file.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -855,6 +856,69 @@ struct VmEval {
       vm_tail_step(tail[i].op, (double)U[tail[i].a], (double)U[tail[i].b],
                    alpha, beta);
     return vm_tail_apply(alpha, beta, s, H.n_obs, with_const);
+  }
+};
+
+// The interpreter of affine programs (SUM_AFFINE: a zero load, then one
+// FMA_UD per product U[a] * D[b], read from the instruction stream).  It
+// carries no stack machine, so several chains per wave keep their few
+// registers (the general interpreter holds every chain's stack registers
+// through each opcode dispatch and spills), and each LDS read of a column
+// feeds kCw chains' FMAs.  Its arithmetic is the general interpreter's on
+// such a program: run() starts from U[a0] and applies fmaf(U[a], D[k], T)
+// per product, run_acc() is vm_run_acc.
+template <int OPT, int CW>
+struct VmAffineEval {
+  static constexpr int kOpt = OPT;
+  static constexpr int kCw = CW;
+  using Params = VmParams;
+  static __device__ __forceinline__ void run(const ChainHot& H, const float* U,
+                                             const float* tile, uint32_t ld,
+                                             int lane, float (&T)[CW][OPT]) {
+    const vm_prog_t prog = (vm_prog_t)H.vm;
+    const uint32_t nu = H.n_uniform_pad;
+    VmIns4 nxt = vm_fetch(prog, 0);
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+      const float u0 = U[j * nu + nxt.a];
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) T[j][k] = u0;
+    }
+    nxt = vm_fetch(prog, 1);
+    for (uint32_t pc = 1; pc < H.n_vm; ++pc) {
+      const uint32_t a = nxt.a, b = nxt.b;
+      nxt = vm_fetch(prog, pc + 1);
+      float D[OPT];
+      vm_load_col<OPT>(tile + b * ld, lane, D);
+#pragma unroll
+      for (int j = 0; j < CW; ++j) {
+        const float ua = U[j * nu + a];
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) T[j][k] = fmaf(ua, D[k], T[j][k]);
+      }
+    }
+  }
+  static __device__ __forceinline__ uint32_t mode(const ChainHot&) {
+    return SUM_AFFINE;
+  }
+  struct AccState {};
+  static __device__ __forceinline__ AccState acc_state(const ChainHot&,
+                                                       const float*) {
+    return AccState{};
+  }
+  static __device__ __forceinline__ void run_acc(const ChainHot& H,
+                                                 const AccState&,
+                                                 const float* U,
+                                                 const float* tile,
+                                                 uint32_t ld, int lane,
+                                                 float (&s)[CW]) {
+    vm_run_acc<OPT, CW>((vm_prog_t)H.vm, H.n_vm, U, H.n_uniform_pad, tile, ld,
+                        lane, s);
+  }
+  static __device__ __forceinline__ double finish(const ChainHot& H,
+                                                  const float* U, double s,
+                                                  bool with_const) {
+    return VmEval<0, OPT, CW>::finish(H, U, s, with_const);
   }
 };
 

@@ -48,12 +48,54 @@ points_kernel(const PointsArgs* __restrict__ P, ChainHot H) {
   points_body<VmEval<NS, OPT, CW>, STREAM>(P, H);
 }
 
-// The interpreter's variants: (spill slots, observations per lane, chains
-// per wave).  Two chains per wave only for programs without a stack; 8
-// observations per lane for those that are not affine.
+// The affine-program interpreter (VmAffineEval).
+template <int OPT, int CW, bool STREAM>
+__global__ void __launch_bounds__(1024)
+chain_kernel_affine(const ChainArgs* __restrict__ Ap, ChainHot H) {
+  chain_body<VmAffineEval<OPT, CW>, STREAM>(Ap, H, blockIdx.x * H.cpb);
+}
+
+template <int OPT, int CW>
+__global__ void __launch_bounds__(1024)
+chain_kernel_batch_affine(const BatchWork* __restrict__ work,
+                          const ChainArgs* __restrict__ args,
+                          const ChainHot* __restrict__ hots, BatchStep step) {
+  batch_body<VmAffineEval<OPT, CW>>(work, args, hots, step);
+}
+
+template <int OPT, int CW, bool STREAM>
+__global__ void __launch_bounds__(1024)
+points_kernel_affine(const PointsArgs* __restrict__ P, ChainHot H) {
+  points_body<VmAffineEval<OPT, CW>, STREAM>(P, H);
+}
+
+// Affine programs run with 16 observations per lane (variant (0, 16, cw):
+// no stack, the only programs the host gives 16 observations per lane
+// without one); one, two or four chains per wave.
+#define GPUM_AFFINE_VARIANTS(X) \
+  X(16, 1)                      \
+  X(16, 2)                      \
+  X(16, 4)
+
+#define GPUM_AFFINE_KERNELS(PRE, OPT, CW)                                     \
+  PRE template __global__ void chain_kernel_affine<OPT, CW, false>(          \
+      const ChainArgs*, ChainHot);                                            \
+  PRE template __global__ void chain_kernel_affine<OPT, CW, true>(           \
+      const ChainArgs*, ChainHot);                                            \
+  PRE template __global__ void points_kernel_affine<OPT, CW, false>(         \
+      const PointsArgs*, ChainHot);                                           \
+  PRE template __global__ void points_kernel_affine<OPT, CW, true>(          \
+      const PointsArgs*, ChainHot);                                           \
+  PRE template __global__ void chain_kernel_batch_affine<OPT, CW>(           \
+      const BatchWork*, const ChainArgs*, const ChainHot*, BatchStep);
+#define GPUM_AFFINE_INSTANTIATE(OPT, CW) GPUM_AFFINE_KERNELS(, OPT, CW)
+#define GPUM_AFFINE_DECLARE(OPT, CW) GPUM_AFFINE_KERNELS(extern, OPT, CW)
+
+// The general interpreter's variants: (spill slots, observations per lane,
+// chains per wave).  Two chains per wave only for programs without a stack;
+// 8 or 4 observations per lane for those that are not affine (variant
+// (0, 16, cw) runs on the affine kernels above).
 #define GPUM_VARIANTS(X) \
-  X(0, 16, 1)            \
-  X(0, 16, 2)            \
   X(0, 8, 1)             \
   X(0, 8, 2)             \
   X(0, 4, 1)             \

@@ -892,7 +892,7 @@ static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
       m, t.stream_mode ? JIT_POINTS_STREAM : JIT_POINTS_RESIDENT, t.v,
       t.lds_bytes + kPointsExtraLds, &jit_failed);
   if (jit_failed) return 1;
-  if (!jf && t.v.cw > 2) {
+  if (!jf && !interp_has_variant(t.v)) {
     set_err("%d chains per wave need a compiled kernel, which this launch "
             "(%zu B of LDS) cannot use", t.v.cw, (size_t)t.lds_bytes);
     return 1;

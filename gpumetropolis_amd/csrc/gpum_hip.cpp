@@ -255,9 +255,9 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
       cw_req = ext->chains_per_wave;
     // Automatic: several chains per wave when the program keeps no stack,
     // the chains fill every CU with such groups and the launch runs a
-    // compiled kernel (the interpreter's dispatch holds every chain's
-    // registers at each opcode merge and spills).  A ladder or a pinned
-    // workgroup must hold whole groups.
+    // compiled kernel or the affine interpreter (the general interpreter's
+    // dispatch holds every chain's registers at each opcode merge and
+    // spills).  A ladder or a pinned workgroup must hold whole groups.
     if (cw_req == 0) {
       const bool has_tiling =
           ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4;
@@ -271,11 +271,11 @@ bool pick_tiling(const gpum_hip_model* m, uint32_t n_chains,
                                   ext->swap_group > 1 && ext->swap_group % cw;
         const bool split_pin = has_tiling && ext->chains_per_block > 0 &&
                                (uint32_t)ext->chains_per_block % cw;
-        if (m->jit && m->L.vm_depth <= 1 &&
+        if ((m->jit || model_affine(m)) && m->L.vm_depth <= 1 &&
             n_chains >= cw * (uint32_t)g_dev.n_cu && !split_ladder &&
             !split_pin &&
             pick_tiling_cw(m, n_chains, ext, t, points, (int)cw, nullptr) &&
-            t->lds_bytes <= kJitMaxDynamicLds)
+            (!m->jit || t->lds_bytes <= kJitMaxDynamicLds))
           return true;
       }
       cw_req = 1;
@@ -307,9 +307,9 @@ static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
       set_err("chains_per_wave must be 1, or 2 or 4 for a program of depth 1");
       return false;
     }
-    if (cw_in == 4 && !m->jit) {
-      set_err("four chains per wave run on compiled kernels only "
-              "(gpum_hip_model_set_jit)");
+    if (cw_in == 4 && !m->jit && !model_affine(m)) {
+      set_err("four chains per wave run on compiled kernels "
+              "(gpum_hip_model_set_jit) or for affine programs only");
       return false;
     }
     t->v.cw = cw_in;
@@ -884,7 +884,7 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
       m, t.stream_mode ? JIT_CHAIN_STREAM : JIT_CHAIN_RESIDENT, t.v,
       t.lds_bytes, &jit_failed);
   if (jit_failed) return 1;
-  if (!jf && t.v.cw > 2) {
+  if (!jf && !interp_has_variant(t.v)) {
     set_err("%d chains per wave need a compiled kernel, which this launch "
             "(%zu B of LDS) cannot use", t.v.cw, (size_t)t.lds_bytes);
     return 1;

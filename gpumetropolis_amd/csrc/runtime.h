@@ -171,6 +171,13 @@ inline bool model_affine(const gpum_hip_model* m) {
   return !m->L.empty && m->L.sum_mode == SUM_AFFINE;
 }
 
+// Whether the interpreter has kernels of variant v: every variant with one
+// or two chains per wave; four for affine programs (variant (0, 16, 4), the
+// affine interpreter).  Other four-chain launches need a compiled kernel.
+inline bool interp_has_variant(const Variant& v) {
+  return v.cw <= 2 || (v.ns == 0 && v.opt == 16);
+}
+
 inline uint32_t model_sum_mode(const gpum_hip_model* m) {
   return m->L.empty ? (uint32_t)SUM_VALUE : m->L.sum_mode;
 }

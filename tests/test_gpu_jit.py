@@ -116,8 +116,10 @@ def test_two_chains_per_wave_equal_one(hip, case, jit):
                          wl.n_obs, jit=jit)
     C = init.shape[0]
     sd = np.full_like(init, step)
-    # (waves per chain group, chains per wave); four only compiled
-    pairs = [(2, 2), (4, 2)] + ([(4, 4)] if jit else [])
+    # (waves per chain group, chains per wave); four on compiled kernels
+    # and, for affine programs, on the affine interpreter
+    quads = jit or name == "gamma"
+    pairs = [(2, 2), (4, 2)] + ([(4, 4)] if quads else [])
     for wpc, cw in pairs:
         a = dm.run(init, sd, np.ones(C), 40, 3, trace=True, tiling=(16 // wpc, wpc, 1))
         b = dm.run(init, sd, np.ones(C), 40, 3, trace=True,
@@ -127,7 +129,7 @@ def test_two_chains_per_wave_equal_one(hip, case, jit):
     # the points kernel (one slice, so the sums are ordered alike)
     c = dm.run_da(None, init, np.ones(C), 20, 7, proposal_sd=sd, trace=True,
                   work_items=1, chains_per_wave=1)
-    for cw in (2, 4) if jit else (2,):
+    for cw in (2, 4) if quads else (2,):
         d = dm.run_da(None, init, np.ones(C), 20, 7, proposal_sd=sd, trace=True,
                       work_items=1, chains_per_wave=cw)
         assert np.array_equal(c.trace_plp, d.trace_plp, equal_nan=True), cw
@@ -151,9 +153,10 @@ def test_two_chains_per_wave_batch(hip):
 
 
 def test_four_chains_per_wave_batch(hip):
-    """Affine programs (gamma) batched with four chains per wave on the
-    compiled kernel: the draws of one chain per wave, bit for bit; the
-    interpreter's batched kernels refuse four."""
+    """Affine programs (gamma) batched with four chains per wave, on the
+    compiled kernel and on the affine interpreter: the draws of one chain
+    per wave, bit for bit.  The general interpreter's batched kernels
+    refuse four (a square-sum program)."""
     print("[quads] batch", flush=True)
     wls = [workloads.gamma(n=n, seed=40 + k) for k, n in enumerate((3000, 7000))]
     rng = np.random.default_rng(5)
@@ -163,11 +166,15 @@ def test_four_chains_per_wave_batch(hip):
               proposal_sd=0.01, seed=4)
     ms, ds = [w.model for w in wls], [w.data for w in wls]
     a = gpu_metropolis_batch(ms, ds, tiling=(16, 1), **kw)
-    b = gpu_metropolis_batch(ms, ds, tiling=(16, 4, 4), jit=True, **kw)
-    for fa, fb in zip(a.fits, b.fits):
-        assert np.array_equal(fa.draws, fb.draws)
-    with pytest.raises(hip.BackendError, match="compiled kernels only"):
-        gpu_metropolis_batch(ms, ds, tiling=(16, 4, 4), jit=False, **kw)
+    for jit in (True, False):
+        b = gpu_metropolis_batch(ms, ds, tiling=(16, 4, 4), jit=jit, **kw)
+        for fa, fb in zip(a.fits, b.fits):
+            assert np.array_equal(fa.draws, fb.draws), jit
+    rg = workloads.regressions(n_models=1, seed=5, n_min=3000, n_max=3000)[0]
+    kw.update(init=[rg.meta["true"] + 0.01 * rng.standard_normal((16, 10))])
+    with pytest.raises(hip.BackendError, match="affine programs only"):
+        gpu_metropolis_batch([rg.model], [rg.data], tiling=(16, 4, 4),
+                             jit=False, **kw)
 
 
 def test_four_chains_per_wave_square_batch(hip):
