@@ -40,11 +40,15 @@ VALU_PEAK = 256 * 64 * 2.4e9  # FP32 lane-ops/s, non-packed (one slot per op)
 # HBM bytes per launch of the headline kernel, from two rocprofv3 --pmc
 # passes over this script's default launches (FETCH_SIZE scaled x2.0 and
 # WRITE_SIZE x1.0 by a 1 GiB copy); the summary and commands are in
-# profiles/compiled_chain_kernel_gamma_1e6obs_4096chains.md.  Reported only
-# for the launch shape it was measured on.
+# profiles/compiled_chain_kernel_gamma_1e6obs_4096chains.md (compiled) and
+# profiles/chain_kernel_gamma_1e6obs_4096chains.md (interpreter, --no-jit).
+# Reported only for the launch shape it was measured on.
 PROFILED_TRAFFIC = {
     ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 17_180_000 * 1024 * 2.0,
                                          "write_bytes": 70_260 * 1024 * 1.0},
+    ("interpreter", 4096, 1_000_000, 250): {
+        "fetch_bytes": 27_762_326 * 1024 * 2.0,
+        "write_bytes": 871_438 * 1024 * 1.0},
 }
 # Instruction mix of the parallel-tempering launch (4000 iterations, 4096
 # chains, 2e5 rows; SQ_INSTS_VALU and SQ_INSTS_VALU_TRANS_F32 over 64
@@ -826,7 +830,8 @@ def main():
             "w_exec": w_exec,
             "w_bytecode": info["w_bytecode"],
             "kernel": ("gpum_jit_kernel (chain_body, compiled gamma program)"
-                       if a.jit else "gpum::chain_kernel<0,16,1,true>"),
+                       if a.jit else "gpum::chain_kernel_affine<16,4,true> "
+                       "(interpreter of affine programs)"),
             "avg_launch_ms": kms.mean(),
             "stream_bytes_per_launch": bytes_per_launch,
             "stream_GBps": bytes_per_launch / avg_launch_s / 1e9,
