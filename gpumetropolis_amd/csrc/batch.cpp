@@ -224,6 +224,7 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     h.cpb = t.cpb;
     h.wpc = t.wpc;
     h.tile_len = t.tile_len;
+    h.unit_chunks = t.unit;
     h.stream_mode = 1;
     set_sum_fields(m, &h);
     h.lds_tile_offset = t.lds_tile_offset;

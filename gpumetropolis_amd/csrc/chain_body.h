@@ -4,13 +4,16 @@
 //   E::kCw          chains per wave: one wave evaluates kCw chains on every
 //                   observation it reads from LDS (chain j's uniform table
 //                   is U + j * H.n_uniform_pad);
+//   E::kPairs       whether the evaluator sums two chunks per FP32 partial
+//                   when H.unit_chunks is 2 (programs without spill slots);
 //   E::run(H, U, tile, ld, lane, T)
 //                   evaluate the program on one chunk into T[kCw][kOpt]
 //                   (`tile`: the chunk, `ld`: its column stride, 64*kOpt);
 //   E::mode(H)      how a chunk is summed (SumMode);
 //   E::acc_state(H, U), E::run_acc(H, state, U, tile, ld, lane, s)
 //                   SUM_AFFINE: what a tile pass keeps (e.g. the uniforms),
-//                   and the chunk's sums s[kCw], accumulated directly;
+//                   and the chunk's products added straight to the sums
+//                   s[kCw] (the caller starts them);
 //   E::finish(H, U, s, with_const)
 //                   apply the sum tail to one chain's data-pass sum;
 //   E::Params::hoist(code, n, consts, p, U), E::Params::prior(code, n,
@@ -376,9 +379,75 @@ __device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
   }
 }
 
-// The wave's share of one tile for its kCw chains: chunks part, part + wpc,
-// ... of 64*OPT observations; FP32 sum per chunk, FP64 across chunks, added
-// to acc[j].  One loop per summation mode, so the interpreter's loop carries
+// One full chunk of 64*OPT observations added to the FP32 partials s[j] of
+// the wave's kCw chains (FIRST: s starts here).  The affine form
+// accumulates its products straight into s; the others evaluate the chunk
+// and add its values (or their squares).
+template <class E, uint32_t MODE, bool FIRST>
+__device__ __forceinline__ void chunk_sum(const ChainHot& H,
+                                          const typename E::AccState& ust,
+                                          const float* U,
+                                          const float* __restrict__ cb,
+                                          uint32_t chunk, int lane,
+                                          float (&s)[E::kCw]) {
+  constexpr int OPT = E::kOpt, CW = E::kCw;
+  if constexpr (MODE == SUM_AFFINE) {
+    if constexpr (FIRST) {
+#pragma unroll
+      for (int j = 0; j < CW; ++j) s[j] = 0.0f;
+    }
+    E::run_acc(H, ust, U, cb, chunk, lane, s);
+  } else {
+    float T[CW][OPT];
+    E::run(H, U, cb, chunk, lane, T);
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+      if constexpr (MODE == SUM_SQUARE) {
+        if constexpr (FIRST) s[j] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < OPT; ++k) s[j] = fmaf(T[j][k], T[j][k], s[j]);
+      } else {
+        if constexpr (FIRST) s[j] = T[j][0];
+        else s[j] += T[j][0];
+#pragma unroll
+        for (int k = 1; k < OPT; ++k) s[j] += T[j][k];
+      }
+    }
+  }
+}
+
+// A chunk of the unit that holds the end of the data, evaluated per
+// observation: observations past n_obs count zero (`start`: s starts here).
+template <class E, uint32_t MODE>
+__device__ __forceinline__ void chunk_sum_masked(const ChainHot& H,
+                                                 const float* U,
+                                                 const float* __restrict__ cb,
+                                                 uint32_t chunk, uint32_t first,
+                                                 bool start, int lane,
+                                                 float (&s)[E::kCw]) {
+  constexpr int OPT = E::kOpt, CW = E::kCw;
+  float T[CW][OPT];
+  E::run(H, U, cb, chunk, lane, T);
+#pragma unroll
+  for (int j = 0; j < CW; ++j) {
+    if (start) s[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < OPT; ++k) {
+      uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
+      const float v = obs < H.n_obs ? T[j][k] : 0.0f;
+      if constexpr (MODE == SUM_SQUARE)
+        s[j] = fmaf(v, v, s[j]);
+      else
+        s[j] = s[j] + v;
+    }
+  }
+}
+
+// The wave's share of one tile for its kCw chains.  The tile's chunks of
+// 64*OPT observations form units of H.unit_chunks consecutive chunks (the
+// tile starts on a unit boundary); the wave takes units part, part + wpc,
+// ....  A unit's chunks share one FP32 partial per chain, added in FP64 to
+// acc[j].  One loop per summation mode, so the interpreter's loop carries
 // only its mode's registers.
 template <class E, uint32_t MODE>
 __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
@@ -392,58 +461,51 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
   const uint32_t nchunks = H.tile_len / chunk;
   const uint32_t cstride = H.n_cols * chunk;  // floats per chunk of the tile
   // Chunks wholly inside the data, and chunks holding any observation.  The
-  // counts are computed once, so the loops below carry no per-chunk test
-  // and their counters stay scalar.
+  // counts are computed once, so the loops below carry no per-chunk test.
   const uint32_t left = H.n_obs > t0 ? H.n_obs - t0 : 0u;
   const uint32_t nfull = min(nchunks, left / chunk);
   const uint32_t nlive = min(nchunks, (left + chunk - 1) / chunk);
+  // Only evaluators without spill slots pair chunks (the host sets
+  // unit_chunks = 1 for the others, whose chunks are long anyway).
+  const uint32_t uc = E::kPairs ? H.unit_chunks : 1u;
+  const uint32_t ufull = nfull / uc;  // units wholly inside the data
   // The affine form's uniforms, read once per tile pass.
   const typename E::AccState ust = E::acc_state(H, U);
-  uint32_t ck = part;
+  // The unit counter is wave-uniform: kept in a scalar register, the loop
+  // tests and steps it on the scalar unit.
+  uint32_t u = __builtin_amdgcn_readfirstlane(part);
+  if constexpr (E::kPairs) {
+    const uint32_t ustride = uc * cstride;
+    for (; u < ufull; u += wpc) {
+      float s[CW];
+      const float* cb = tile + u * ustride;
+      chunk_sum<E, MODE, true>(H, ust, U, cb, chunk, lane, s);
+      if (uc == 2)
+        chunk_sum<E, MODE, false>(H, ust, U, cb + cstride, chunk, lane, s);
+#pragma unroll
+      for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
+    }
+  } else {
+    // Unrolled by two, so one chunk's LDS reads overlap the previous
+    // chunk's arithmetic.
 #pragma unroll 2
-  for (; ck < nfull; ck += wpc) {
-    const float* cb = tile + ck * cstride;
-    float s[CW];
-    if constexpr (MODE == SUM_AFFINE) {
-      E::run_acc(H, ust, U, cb, chunk, lane, s);
-    } else {
-      float T[CW][OPT];
-      E::run(H, U, cb, chunk, lane, T);
+    for (; u < ufull; u += wpc) {
+      float s[CW];
+      chunk_sum<E, MODE, true>(H, ust, U, tile + u * cstride, chunk, lane, s);
 #pragma unroll
-      for (int j = 0; j < CW; ++j) {
-        if constexpr (MODE == SUM_SQUARE) {
-          s[j] = 0.0f;
-#pragma unroll
-          for (int k = 0; k < OPT; ++k) s[j] = fmaf(T[j][k], T[j][k], s[j]);
-        } else {
-          s[j] = T[j][0];
-#pragma unroll
-          for (int k = 1; k < OPT; ++k) s[j] += T[j][k];
-        }
-      }
+      for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
     }
-#pragma unroll
-    for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
   }
-  for (; ck < nlive; ck += wpc) {
-    const uint32_t first = t0 + ck * chunk;
-    // The last chunk of the data: observations past n_obs count zero.
+  // The unit holding the end of the data (one wave at most), evaluated per
+  // observation with the observations past n_obs masked.  Which unit that
+  // is depends on n_obs alone, so every tiling sums it alike.
+  const uint32_t c0 = u * uc;
+  if (u == ufull && c0 < nlive) {
     float s[CW];
-    float T[CW][OPT];
-    E::run(H, U, tile + ck * cstride, chunk, lane, T);
-#pragma unroll
-    for (int j = 0; j < CW; ++j) {
-      s[j] = 0.0f;
-#pragma unroll
-      for (int k = 0; k < OPT; ++k) {
-        uint32_t obs = first + (uint32_t)(4 * lane + 256 * (k >> 2) + (k & 3));
-        const float v = obs < H.n_obs ? T[j][k] : 0.0f;
-        if constexpr (MODE == SUM_SQUARE)
-          s[j] = fmaf(v, v, s[j]);
-        else
-          s[j] = s[j] + v;
-      }
-    }
+    const uint32_t c1 = min(c0 + uc, nlive);
+    for (uint32_t c = c0; c < c1; ++c)
+      chunk_sum_masked<E, MODE>(H, U, tile + c * cstride, chunk,
+                                t0 + c * chunk, c == c0, lane, s);
 #pragma unroll
     for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
   }
@@ -821,6 +883,7 @@ template <int NS, int OPT, int CW>
 struct VmEval {
   static constexpr int kOpt = OPT;
   static constexpr int kCw = CW;
+  static constexpr bool kPairs = NS == 0;
   using Params = VmParams;
   static __device__ __forceinline__ void run(const ChainHot& H, const float* U,
                                              const float* tile, uint32_t ld,
@@ -871,6 +934,7 @@ template <int OPT, int CW>
 struct VmAffineEval {
   static constexpr int kOpt = OPT;
   static constexpr int kCw = CW;
+  static constexpr bool kPairs = true;
   using Params = VmParams;
   static __device__ __forceinline__ void run(const ChainHot& H, const float* U,
                                              const float* tile, uint32_t ld,

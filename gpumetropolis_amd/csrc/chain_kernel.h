@@ -102,6 +102,11 @@ struct ChainHot {
   // stored in `vm` after the program's padding instruction.
   uint32_t sum_mode;
   uint32_t n_sum_tail;
+  // Chunks per FP32 partial (1 or 2): the unit of observations one FP32
+  // sum covers before its FP64 add.  Fixed by the model (its columns and
+  // observations per lane), so every tiling of a model groups the same
+  // observations and rounds them alike.
+  uint32_t unit_chunks;
 };
 
 // Streamed tiles: two static LDS buffers of kTileF4 float4s each.

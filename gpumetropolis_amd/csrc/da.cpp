@@ -852,6 +852,7 @@ static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
   h.cpb = t.cpb;
   h.wpc = 1;
   h.tile_len = t.tile_len;
+  h.unit_chunks = t.unit;
   h.stream_mode = t.stream_mode;
   set_sum_fields(m, &h);
   h.lds_tile_offset = t.lds_tile_offset;

@@ -408,8 +408,9 @@ __device__ __forceinline__ void vm_acc_fma_u(const float (&ua)[CW],
   }
 }
 
-// The chunk sums of an affine program: its FMA_UD instructions (pc >= 1;
-// instruction 0 loads the zero the per-observation form starts from).
+// A chunk of an affine program added to the sums s[j]: its FMA_UD
+// instructions (pc >= 1; instruction 0 loads the zero the per-observation
+// form starts from).  The caller starts s.
 template <int OPT, int CW>
 __device__ __forceinline__ void vm_run_acc(vm_prog_t prog, uint32_t n_instr,
                                            const float* __restrict__ U,
@@ -417,8 +418,6 @@ __device__ __forceinline__ void vm_run_acc(vm_prog_t prog, uint32_t n_instr,
                                            const float* __restrict__ tile,
                                            uint32_t ld, int lane,
                                            float (&s)[CW]) {
-#pragma unroll
-  for (int j = 0; j < CW; ++j) s[j] = 0.0f;
   VmIns4 nxt = vm_fetch(prog, 1);
   for (uint32_t pc = 1; pc < n_instr; ++pc) {
     const uint32_t a = nxt.a, b = nxt.b;

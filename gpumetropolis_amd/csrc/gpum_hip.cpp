@@ -295,6 +295,15 @@ static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
                                 : (points ? kPointsExtraLds : 0));
   const uint32_t opt = (uint32_t)t->v.opt;
   t->chunk = 64u * opt;
+  // Two chunks per FP32 partial (one conversion and FP64 add per two
+  // chunks) for programs without spill slots whose two chunks of rows fit
+  // one tile buffer.  The choice depends on the model alone (its program
+  // picks the variant), so every tiling sums alike.
+  t->unit = t->v.ns == 0 &&
+                    (size_t)2 * t->chunk * std::max(1u, m->n_cols_total) * 4 <=
+                        (size_t)kTileF4 * 16
+                ? 2u
+                : 1u;
   const uint32_t ncu = (uint32_t)g_dev.n_cu;
   const bool has_tiling =
       ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4;
@@ -335,11 +344,12 @@ static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
     if (ext->waves_per_chain > 0) wpc = (uint32_t)ext->waves_per_chain;
     if (ext->chains_per_block > 0) cpb = (uint32_t)ext->chains_per_block;
   }
-  // Wide rows: fewer waves per chain so one chunk per wave fits in LDS.
+  // Wide rows: fewer waves per chain so one unit per wave fits in LDS.
   {
     const uint32_t ncol0 = std::max(1u, m->n_cols_total);
     const size_t tile_cap = (size_t)kTileF4 * 16;
-    while (wpc > 1 && (size_t)t->chunk * wpc * ncol0 * 4 > tile_cap) {
+    while (wpc > 1 &&
+           (size_t)t->chunk * t->unit * wpc * ncol0 * 4 > tile_cap) {
       wpc /= 2;
       if (!pinned) cpb = fill(wpc);
     }
@@ -382,7 +392,7 @@ static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
   t->cpb = cpb;
   size_t head = head_bytes(cpb, wpc);
   t->lds_tile_offset = (uint32_t)head;
-  const uint32_t step = t->chunk * wpc;  // one chunk per wave of a chain
+  const uint32_t step = t->chunk * t->unit * wpc;  // one unit per wave
   const size_t tile_floats = (size_t)kTileF4 * 4;  // one static buffer
   uint32_t n_res = round_up(std::max(1u, m->n_obs), step);
   if (!force && (size_t)n_res * ncol * 4 <= (size_t)96 * 1024 &&
@@ -871,6 +881,7 @@ static int model_run_locked(gpum_hip_model* m, const double* init,
   h.cpb = t.cpb;
   h.wpc = t.wpc;
   h.tile_len = t.tile_len;
+  h.unit_chunks = t.unit;
   h.stream_mode = t.stream_mode;
   h.lds_tile_offset = t.lds_tile_offset;
   h.state_words = 6 * np + 6;

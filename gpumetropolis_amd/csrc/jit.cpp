@@ -379,6 +379,8 @@ std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
     << "struct JitEval {\n"
     << "  static constexpr int kOpt = " << opt << ";\n"
     << "  static constexpr int kCw = " << cw << ";\n"
+    << "  static constexpr bool kPairs = " << (ns == 0 ? "true" : "false")
+    << ";\n"
     << "  static __device__ __forceinline__ void run(const ChainHot& H,\n"
     << "      const float* __restrict__ U, const float* __restrict__ tile,\n"
     << "      uint32_t ld, int lane, float (&T)[" << cw << "][" << opt << "]) {\n"
@@ -407,9 +409,9 @@ std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
     << "  static __device__ __forceinline__ void run_acc(const ChainHot& H,\n"
     << "      const AccState& st, const float* __restrict__ U,\n"
     << "      const float* __restrict__ tile, uint32_t ld, int lane,\n"
-    << "      float (&s)[" << cw << "]) {\n"
-    << "    for (int j = 0; j < " << cw << "; ++j) s[j] = 0.0f;\n";
-  // The interpreter's vm_run_acc order: instructions 1.. of the program.
+    << "      float (&s)[" << cw << "]) {\n";
+  // The interpreter's vm_run_acc order: instructions 1.. of the program,
+  // added to the caller's sums.
   if (L.sum_mode == SUM_AFFINE)
     for (size_t pc = 1; pc < L.vm.size(); ++pc)
       o << "    vm_acc_fma_u<" << opt << ", " << cw << ">(st.u[" << pc

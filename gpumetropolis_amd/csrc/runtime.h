@@ -88,7 +88,8 @@ uint32_t round_up(uint32_t x, uint32_t m);
 
 struct Tiling {
   Variant v;
-  uint32_t cpb, wpc, chunk, tile_len, n_pad, resident, stream_mode,
+  // unit: chunks per FP32 partial (ChainHot::unit_chunks).
+  uint32_t cpb, wpc, chunk, unit, tile_len, n_pad, resident, stream_mode,
       lds_tile_offset;
   size_t lds_bytes;
 };
