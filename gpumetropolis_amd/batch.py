@@ -3,8 +3,9 @@
 ``gpu_metropolis_batch`` runs the reference's per-model orchestration
 (``gpu_metropolis`` with method "de", path A, R/api.R:442-527 and
 R/adaptive.R:427-524; or "rwm" with the same between-batch rule) for a list
-of models at once: every batch of iterations of every model is one launch,
-the between-batch step a second, and all batches one HIP graph.  Models may
+of models at once: one persistent launch runs every batch of every model,
+each model's next batch starting as soon as its own previous batch and
+between-batch step are done.  Models may
 differ in program, data size, parameter count and chain count.  Each model
 draws its initial values and uniforms from its own Philox stream (stream
 id = its index), so a model's draws do not depend on the rest of the batch.

@@ -1,16 +1,18 @@
-// batch.cpp -- many independent models in one launch per batch of
-// iterations, the batches captured in one HIP graph.
+// batch.cpp -- many independent models, every batch of iterations, in one
+// persistent launch.
 //
 // The reference runs one model per call (R/api.R:247-599); a workload of
 // hundreds of models is a host loop of calls, each uploading its data
-// (src/rust/src/chain_kernel.rs:393-413).  Here every model's program and
-// data stay on the device, one launch of the batched chain kernel advances
-// every chain of every model through one batch (a model's workgroups
-// grouped on one XCD so they share its data in L2, each XCD's list longest
-// model first, so the dispatcher's in-order placement is a
-// longest-processing-time schedule), a small kernel does the between-batch
-// step of the reference's orchestrator, and the whole sequence of batches
-// is one graph launch.
+// (src/rust/src/chain_kernel.rs:393-413) and each batch of iterations
+// another call (R/adaptive.R:427-524).  Here every model's program and data
+// stay on the device, and one launch of the batched chain kernel advances
+// every chain of every model through every batch: a workgroup takes work
+// items (model, chain group, batch) from its XCD's queue, an item starts as
+// soon as its own model has ended the previous batch, and the workgroup
+// that ends a model's batch runs the reference's between-batch step for it.
+// A model's items sit in one XCD's queue (they share its data in that L2),
+// each queue batch-major and longest model first, so no batch waits for the
+// slowest model of the previous one.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -129,7 +131,6 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
   }
   // Flat per-chain buffers; model k's chains are a contiguous slice.
   std::vector<double> h_snap, h_sd, h_temps(n_chains, 1.0);
-  std::vector<uint32_t> h_chain_model(n_chains);
   std::vector<size_t> state_off(n_models), welford_off(n_models);
   size_t so = 0, wo = 0;
   for (uint32_t k = 0; k < n_models; ++k) {
@@ -139,17 +140,17 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     h_snap.insert(h_snap.end(), models[k].init, models[k].init + (size_t)C * np);
     h_sd.insert(h_sd.end(), models[k].proposal_sd,
                 models[k].proposal_sd + (size_t)C * np);
-    for (uint32_t c = 0; c < C; ++c) h_chain_model[plan[k].chain_off + c] = k;
     so += (size_t)C * np;
     wo += (size_t)C * (1 + 2 * np);
   }
   DevBuf<double> snap, sd, sd0, welford, temps, lp;
   DevBuf<float> draws, accept, acc_hist;
-  DevBuf<uint32_t> chain_model;
+  DevBuf<uint32_t> sync;
   DevBuf<ChainArgs> d_args;
   DevBuf<ChainHot> d_hots;
   DevBuf<BatchModelInfo> d_info;
   DevBuf<BatchWork> d_work;
+  DevBuf<BatchQueue> d_queue;
   std::vector<double> zeros(n_welford, 0.0);
   if (!snap.upload(h_snap.data(), n_state, "upload init") ||
       !sd.upload(h_sd.data(), n_state, "upload proposal_sd") ||
@@ -160,7 +161,7 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
       !draws.alloc(n_draws, "allocate draws") ||
       !accept.alloc(n_chains, "allocate state") ||
       !acc_hist.alloc((size_t)nb * n_chains, "allocate accept history") ||
-      !chain_model.upload(h_chain_model.data(), n_chains, "upload chain map"))
+      !sync.alloc(kBatchSyncHead + 2 * (size_t)n_models, "allocate queue"))
     return 1;
   std::vector<ChainArgs> args(n_models);
   std::vector<ChainHot> hots(n_models);
@@ -233,6 +234,7 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     in.chain_off = plan[k].chain_off;
     in.n_chains = C;
     in.n_params = np;
+    in.n_groups = (C + t.cpb - 1) / t.cpb;
     in.draws_ld = o.n_iter;
     in.draws = a.draws;
     in.snapshot = snap.p + state_off[k];
@@ -243,16 +245,15 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     in.accept_hist = acc_hist.p + (size_t)nb * plan[k].chain_off;
     for (uint32_t c0 = 0; c0 < C; c0 += t.cpb) work.push_back(BatchWork{k, c0});
   }
-  // XCD-aware order.  The dispatcher hands workgroup w to XCD w mod 8, and
-  // each of MI355X's 8 XCDs has its own L2.  A model's workgroups (its
-  // chain groups) stream the same data tiles every iteration, so they get
-  // ids congruent mod 8 and adjacent in that XCD's order: they start side
-  // by side on one XCD and the tiles one of them fetches serve the others
-  // from L2 (with consecutive ids they ran on different XCDs and every
-  // workgroup streamed the model from HBM).  Models are assigned largest
-  // first to the XCD with the least work so far, and within an XCD the
-  // longest run first, so the in-order dispatcher fills in with short ones
-  // at the end; the shorter XCD lists are padded with empty items.
+  // Per-XCD queues.  The dispatcher hands workgroup w to XCD w mod 8, and
+  // each of MI355X's 8 XCDs has its own L2.  A model's chain groups stream
+  // the same data tiles every iteration, so all of a model's items go to
+  // one XCD's queue and the tiles one of them fetches serve the others from
+  // that L2.  Models are assigned largest first to the XCD with the least
+  // work so far; within a queue the longest items come first in every
+  // batch, so short ones fill in at the end.
+  BatchQueue q;
+  memset(&q, 0, sizeof q);
   {
     std::vector<uint32_t> by_size(n_models);
     for (uint32_t k = 0; k < n_models; ++k) by_size[k] = k;
@@ -271,20 +272,49 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
         if (bw.model == k) xq[x].push_back(bw);
       load[x] += work_of(k);
     }
-    size_t longest = 0;
-    for (const auto& q : xq) longest = std::max(longest, q.size());
-    work.assign(longest * kXcds, BatchWork{kNoBatchModel, 0});
-    for (uint32_t x = 0; x < kXcds; ++x)
-      for (size_t i = 0; i < xq[x].size(); ++i) work[i * kXcds + x] = xq[x][i];
+    work.clear();
+    // Workgroups per queue: a CU's share of LDS allows one or two, and a
+    // queue needs no more workgroups than it has items per batch (more
+    // would only take items of later batches and wait).
+    const size_t wg_lds = lds + kStreamStaticLds + kBatchArgsLds;
+    const uint32_t per_cu = std::min<uint32_t>(
+        2u, std::max<uint32_t>(1u, (uint32_t)(g_dev.lds_max / wg_lds)));
+    const uint32_t cu_per_xcd = std::max(1, g_dev.n_cu / (int)kXcds);
+    for (uint32_t x = 0; x < kXcds; ++x) {
+      q.off[x] = (uint32_t)work.size();
+      q.len[x] = (uint32_t)xq[x].size();
+      q.slots[x] = std::min(q.len[x], cu_per_xcd * per_cu);
+      work.insert(work.end(), xq[x].begin(), xq[x].end());
+    }
   }
+  uint32_t max_slots = 0;
+  for (uint32_t x = 0; x < kXcds; ++x) max_slots = std::max(max_slots, q.slots[x]);
+  const uint32_t n_wg = kXcds * max_slots;
+  int clock_khz = 0;
+  if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate,
+                            g_dev.id) != hipSuccess || clock_khz <= 0)
+    clock_khz = 100000;
+  q.n_models = n_models;
+  q.n_batches = nb;
+  q.every = every;
+  q.n_iter = o.n_iter;
+  q.warmup_batches = o.adapt ? o.warmup_batches : 0u;
+  // A dependency is another workgroup's batch of one chain group: far below
+  // this bound, which only ends a run that cannot progress.
+  q.wait_ticks = (uint64_t)clock_khz * 1000ull * kBatchWaitSeconds;
+  if (!d_work.upload(work.data(), work.size(), "upload launch arguments") ||
+      !d_info.upload(info.data(), n_models, "upload launch arguments"))
+    return 1;
+  q.items = d_work.p;
+  q.models = d_info.p;
+  q.sync = sync.p;
   if (!d_args.upload(args.data(), n_models, "upload launch arguments") ||
       !d_hots.upload(hots.data(), n_models, "upload launch arguments") ||
-      !d_info.upload(info.data(), n_models, "upload launch arguments") ||
-      !d_work.upload(work.data(), work.size(), "upload launch arguments"))
+      !d_queue.upload(&q, 1, "upload launch arguments"))
     return 1;
+  const uint32_t n_sync = kBatchSyncHead + 2 * n_models;
 
   hipStream_t st = g_dev.stream;
-  const uint32_t n_wg = (uint32_t)work.size();
   // One compiled kernel serves the launch when every model runs the same
   // compiled program with the launch's variant.
   hipFunction_t jf = nullptr;
@@ -309,32 +339,18 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
     return 1;
   }
   auto enqueue = [&]() -> hipError_t {
-    uint32_t t0 = 0;
-    for (uint32_t b = 1; b <= nb; ++b) {
-      BatchStep step;
-      step.seed_add = b - 1;
-      step.n_iter = sizes[b - 1];
-      step.t0 = t0;
-      step.draws_ld = o.n_iter;
-      hipError_t e;
-      if (jf) {
-        const BatchWork* wp = d_work.p;
-        const ChainArgs* ap = d_args.p;
-        const ChainHot* hp = d_hots.p;
-        void* kargs[] = {(void*)&wp, (void*)&ap, (void*)&hp, (void*)&step};
-        e = launch_module(jf, dim3(n_wg), dim3(1024), lds, st, kargs);
-      } else {
-        e = launch_chain_kernel_batch(d_work.p, n_wg, d_args.p, d_hots.p, step,
-                                      force.v.ns, force.v.opt, force.v.cw, lds,
-                                      st);
-      }
-      if (e != hipSuccess) return e;
-      e = launch_batch_end(d_info.p, chain_model.p, n_chains, t0, sizes[b - 1], b,
-                           (o.adapt && b <= o.warmup_batches) ? 1 : 0, st);
-      if (e != hipSuccess) return e;
-      t0 += sizes[b - 1];
+    hipError_t e = launch_batch_sync_reset(sync.p, n_sync, st);
+    if (e != hipSuccess) return e;
+    if (jf) {
+      const BatchQueue* qp = d_queue.p;
+      const ChainArgs* ap = d_args.p;
+      const ChainHot* hp = d_hots.p;
+      void* kargs[] = {(void*)&qp, (void*)&ap, (void*)&hp};
+      return launch_module(jf, dim3(n_wg), dim3(1024), lds, st, kargs);
     }
-    return hipSuccess;
+    return launch_chain_kernel_batch(d_queue.p, n_wg, d_args.p, d_hots.p,
+                                     force.v.ns, force.v.opt, force.v.cw, lds,
+                                     st);
   };
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -367,6 +383,17 @@ extern "C" int gpum_hip_batch_run(gpum_batch_model* models, uint32_t n_models,
   hipEventDestroy(e1);
   if (e != hipSuccess) {
     set_hip_err("batched run", e);
+    return 1;
+  }
+  uint32_t err_word = 0;
+  if (hipMemcpy(&err_word, sync.p + kXcds, sizeof err_word,
+                hipMemcpyDeviceToHost) != hipSuccess) {
+    set_err("read results: device error");
+    return 1;
+  }
+  if (err_word != 0) {
+    set_err("batched run: a work item waited more than %u s for its model's "
+            "previous batch; the run was abandoned", kBatchWaitSeconds);
     return 1;
   }
   // Results per model.

@@ -6,14 +6,19 @@
 //                   is U + j * H.n_uniform_pad);
 //   E::kPairs       whether the evaluator sums two chunks per FP32 partial
 //                   when H.unit_chunks is 2 (programs without spill slots);
-//   E::run(H, U, tile, ld, lane, T)
+//   E::kUnit        chunks per FP32 partial fixed at compile time (1 or 2;
+//                   0: H.unit_chunks);
+//   E::acc_state(H, U)
+//                   what a data pass keeps in registers from the uniform
+//                   table (e.g. the uniforms a compiled program reads), read
+//                   once per pass and held across its tiles;
+//   E::run(H, state, U, tile, ld, lane, T)
 //                   evaluate the program on one chunk into T[kCw][kOpt]
 //                   (`tile`: the chunk, `ld`: its column stride, 64*kOpt);
 //   E::mode(H)      how a chunk is summed (SumMode);
-//   E::acc_state(H, U), E::run_acc(H, state, U, tile, ld, lane, s)
-//                   SUM_AFFINE: what a tile pass keeps (e.g. the uniforms),
-//                   and the chunk's products added straight to the sums
-//                   s[kCw] (the caller starts them);
+//   E::run_acc(H, state, U, tile, ld, lane, s)
+//                   SUM_AFFINE: the chunk's products added straight to the
+//                   sums s[kCw] (the caller starts them);
 //   E::finish(H, U, s, with_const)
 //                   apply the sum tail to one chain's data-pass sum;
 //   E::Params::hoist(code, n, consts, p, U), E::Params::prior(code, n,
@@ -334,9 +339,9 @@ __device__ __forceinline__ size_t tile_src4(uint32_t i, uint32_t n_cols,
   return (size_t)col * ld4 + (size_t)ti * n4 + (size_t)chunk * seg4 + sub;
 }
 
-// Asynchronous tile copy: global_load_lds moves 16 bytes per lane from
-// HBM straight into LDS (no VGPR staging); lane l of wave w writes float4
-// i0 + l where i0 = k*blockDim + 64*w is wave-uniform.  A tile holds a
+// Asynchronous tile copy: a buffer load with the LDS flag moves 16 bytes
+// per lane from HBM straight into LDS (no VGPR staging); lane l of wave w
+// writes float4 i0 + l where i0 = k*blockDim + 64*w is wave-uniform.  A tile holds a
 // multiple of 64 float4s, and 64 consecutive LDS float4s lie in one
 // (chunk, column) segment, so a wave's slots are all inside the tile or all
 // outside and read 64 consecutive HBM float4s.  Streaming launches use 1024
@@ -344,8 +349,14 @@ __device__ __forceinline__ size_t tile_src4(uint32_t i, uint32_t n_cols,
 // the tile repeat its last 64 float4s (identical bytes to identical
 // addresses), which keeps the sequence free of branches.
 struct TileCopy {
-  uint32_t lds[kTileIssues];  // LDS float4 index of each copy slot
-  size_t src[kTileIssues];    // its HBM float4 offset within tile 0
+  uint32_t base;            // LDS float4 index of slot 0 (64 * wave)
+  uint32_t last;            // the tile's last slot (tile4 - 64)
+  size_t src[kTileIssues];  // each slot's HBM float4 offset within tile 0
+  // LDS float4 index of slot k, recomputed per copy on the scalar unit
+  // (fewer scalar registers held across the loop).
+  __device__ __forceinline__ uint32_t lds(uint32_t k) const {
+    return min(base + k * 1024u, last);
+  }
 };
 
 // The wave's copy slots, computed once per launch: they are wave-uniform
@@ -356,26 +367,43 @@ __device__ __forceinline__ TileCopy tile_copy_plan(uint32_t n_cols,
                                                    uint32_t tile4,
                                                    uint32_t wave_base) {
   TileCopy c;
+  c.base = wave_base;
+  c.last = tile4 - 64u;
 #pragma unroll
-  for (uint32_t k = 0; k < kTileIssues; ++k) {
-    c.lds[k] = min(wave_base + k * 1024u, tile4 - 64u);
-    c.src[k] = tile_src4<OPT>(c.lds[k], n_cols, ld4, n4, 0);
-  }
+  for (uint32_t k = 0; k < kTileIssues; ++k)
+    c.src[k] = tile_src4<OPT>(c.lds(k), n_cols, ld4, n4, 0);
   return c;
 }
 
-// The loop is unrolled so that every copy in flight keeps its own address
-// registers.
+// Word 3 of a raw (untyped, unswizzled) buffer descriptor on gfx9-family
+// GPUs, gfx950 included.
+constexpr int kRawBufferWord3 = 0x00020000;
+
+// One tile's copy.  Each copy slot reads 64 consecutive float4s (one per
+// lane) through a buffer descriptor whose base is the slot's first float4
+// of tile ti: the base is computed on the scalar unit, every lane's offset
+// is lane * 16 bytes, so a tile costs no vector address arithmetic and no
+// data set is too large for 32-bit offsets.  The descriptor's range is the
+// slot's 1024 bytes.  The loop is unrolled so that every copy in flight
+// keeps its own descriptor.
 __device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
                                                const TileCopy& c, uint32_t n4,
                                                int lane, uint32_t ti,
                                                float4* dst) {
+  const uint32_t voff = (uint32_t)lane * 16u;
+  // The slots' LDS indices are recomputed here on the scalar unit: the
+  // empty asm keeps the compiler from hoisting them out of the tile loop
+  // into scalar registers it would then have to spill.
+  TileCopy cc = c;
+  asm volatile("" : "+s"(cc.base));
 #pragma unroll
   for (uint32_t k = 0; k < kTileIssues; ++k) {
     const float4* base = src + (c.src[k] + (size_t)ti * n4);
-    __builtin_amdgcn_global_load_lds(
-        (const void*)(base + lane),
-        (__attribute__((address_space(3))) void*)(dst + c.lds[k]), 16, 0, 0);
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float4*>(base), (short)0, 1024, kRawBufferWord3);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        r, (__attribute__((address_space(3))) void*)(dst + cc.lds(k)), 16,
+        voff, 0, 0, 0);
   }
 }
 
@@ -399,7 +427,7 @@ __device__ __forceinline__ void chunk_sum(const ChainHot& H,
     E::run_acc(H, ust, U, cb, chunk, lane, s);
   } else {
     float T[CW][OPT];
-    E::run(H, U, cb, chunk, lane, T);
+    E::run(H, ust, U, cb, chunk, lane, T);
 #pragma unroll
     for (int j = 0; j < CW; ++j) {
       if constexpr (MODE == SUM_SQUARE) {
@@ -420,6 +448,7 @@ __device__ __forceinline__ void chunk_sum(const ChainHot& H,
 // observation: observations past n_obs count zero (`start`: s starts here).
 template <class E, uint32_t MODE>
 __device__ __forceinline__ void chunk_sum_masked(const ChainHot& H,
+                                                 const typename E::AccState& ust,
                                                  const float* U,
                                                  const float* __restrict__ cb,
                                                  uint32_t chunk, uint32_t first,
@@ -427,7 +456,7 @@ __device__ __forceinline__ void chunk_sum_masked(const ChainHot& H,
                                                  float (&s)[E::kCw]) {
   constexpr int OPT = E::kOpt, CW = E::kCw;
   float T[CW][OPT];
-  E::run(H, U, cb, chunk, lane, T);
+  E::run(H, ust, U, cb, chunk, lane, T);
 #pragma unroll
   for (int j = 0; j < CW; ++j) {
     if (start) s[j] = 0.0f;
@@ -444,13 +473,15 @@ __device__ __forceinline__ void chunk_sum_masked(const ChainHot& H,
 }
 
 // The wave's share of one tile for its kCw chains.  The tile's chunks of
-// 64*OPT observations form units of H.unit_chunks consecutive chunks (the
-// tile starts on a unit boundary); the wave takes units part, part + wpc,
-// ....  A unit's chunks share one FP32 partial per chain, added in FP64 to
-// acc[j].  One loop per summation mode, so the interpreter's loop carries
-// only its mode's registers.
+// 64*OPT observations form units of `uc` consecutive chunks (E::kUnit, or
+// H.unit_chunks; the tile starts on a unit boundary); the wave takes units
+// part, part + wpc, ....  A unit's chunks share one FP32 partial per chain,
+// added in FP64 to acc[j].  One loop per summation mode, so the
+// interpreter's loop carries only its mode's registers.  `ust` holds what
+// the evaluator keeps in registers for the whole data pass.
 template <class E, uint32_t MODE>
 __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
+                                               const typename E::AccState& ust,
                                                const float* __restrict__ tile,
                                                uint32_t t0, uint32_t part,
                                                uint32_t wpc, const float* U,
@@ -460,17 +491,25 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
   constexpr uint32_t chunk = 64u * OPT;
   const uint32_t nchunks = H.tile_len / chunk;
   const uint32_t cstride = H.n_cols * chunk;  // floats per chunk of the tile
-  // Chunks wholly inside the data, and chunks holding any observation.  The
-  // counts are computed once, so the loops below carry no per-chunk test.
-  const uint32_t left = H.n_obs > t0 ? H.n_obs - t0 : 0u;
-  const uint32_t nfull = min(nchunks, left / chunk);
-  const uint32_t nlive = min(nchunks, (left + chunk - 1) / chunk);
+  // Chunks wholly inside the data, and chunks holding any observation: all
+  // of them for every tile but the last ones, so the common case costs one
+  // scalar compare.  The loops below carry no per-chunk test.
+  uint32_t nfull = nchunks, nlive = nchunks;
+  if (t0 + H.tile_len > H.n_obs) {
+    // t0 clamped to n_obs in a scalar register; the empty asm keeps the
+    // compiler from folding the clamp and subtraction into a saturating
+    // vector subtract.
+    uint32_t t0c = min(t0, H.n_obs);
+    asm volatile("" : "+s"(t0c));
+    const uint32_t left = H.n_obs - t0c;
+    nfull = min(nchunks, left / chunk);
+    nlive = min(nchunks, (left + chunk - 1) / chunk);
+  }
   // Only evaluators without spill slots pair chunks (the host sets
   // unit_chunks = 1 for the others, whose chunks are long anyway).
-  const uint32_t uc = E::kPairs ? H.unit_chunks : 1u;
+  const uint32_t uc =
+      E::kUnit ? (uint32_t)E::kUnit : (E::kPairs ? H.unit_chunks : 1u);
   const uint32_t ufull = nfull / uc;  // units wholly inside the data
-  // The affine form's uniforms, read once per tile pass.
-  const typename E::AccState ust = E::acc_state(H, U);
   // The unit counter is wave-uniform: kept in a scalar register, the loop
   // tests and steps it on the scalar unit.
   uint32_t u = __builtin_amdgcn_readfirstlane(part);
@@ -504,7 +543,7 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
     float s[CW];
     const uint32_t c1 = min(c0 + uc, nlive);
     for (uint32_t c = c0; c < c1; ++c)
-      chunk_sum_masked<E, MODE>(H, U, tile + c * cstride, chunk,
+      chunk_sum_masked<E, MODE>(H, ust, U, tile + c * cstride, chunk,
                                 t0 + c * chunk, c == c0, lane, s);
 #pragma unroll
     for (int j = 0; j < CW; ++j) acc[j] += (double)s[j];
@@ -513,17 +552,18 @@ __device__ __forceinline__ void tile_pass_mode(const ChainHot& H,
 
 template <class E>
 __device__ __forceinline__ void tile_pass(const ChainHot& H,
+                                          const typename E::AccState& ust,
                                           const float* __restrict__ tile,
                                           uint32_t t0, uint32_t part,
                                           uint32_t wpc, const float* U,
                                           int lane, double (&acc)[E::kCw]) {
   const uint32_t mode = E::mode(H);
   if (mode == SUM_AFFINE)
-    tile_pass_mode<E, SUM_AFFINE>(H, tile, t0, part, wpc, U, lane, acc);
+    tile_pass_mode<E, SUM_AFFINE>(H, ust, tile, t0, part, wpc, U, lane, acc);
   else if (mode == SUM_SQUARE)
-    tile_pass_mode<E, SUM_SQUARE>(H, tile, t0, part, wpc, U, lane, acc);
+    tile_pass_mode<E, SUM_SQUARE>(H, ust, tile, t0, part, wpc, U, lane, acc);
   else
-    tile_pass_mode<E, SUM_VALUE>(H, tile, t0, part, wpc, U, lane, acc);
+    tile_pass_mode<E, SUM_VALUE>(H, ust, tile, t0, part, wpc, U, lane, acc);
 }
 
 // One workgroup's chains chain0 .. chain0 + cpb - 1 through the whole run.
@@ -572,19 +612,10 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
   const uint32_t wave_base = (uint32_t)(wave * 64);
   float4* res = reinterpret_cast<float4*>(smem + H.lds_tile_offset);
 
-  // The copy of issue_tile_dma written out here: as a call it raises the
-  // scalar register pressure of the iteration loop.
   constexpr int OPT = E::kOpt;
   const TileCopy cp = tile_copy_plan<OPT>(H.n_cols, ld4, n4, tile4, wave_base);
   auto issue_tile = [&](uint32_t ti, float4* dst) {
-#pragma unroll
-    for (uint32_t k = 0; k < kTileIssues; ++k) {
-      const float4* base = src + (cp.src[k] + (size_t)ti * n4);
-      __builtin_amdgcn_global_load_lds(
-          (const void*)(base + lane),
-          (__attribute__((address_space(3))) void*)(dst + cp.lds[k]), 16, 0,
-          0);
-    }
+    issue_tile_dma(src, cp, n4, lane, ti, dst);
   };
 
   if constexpr (!STREAM) {
@@ -602,10 +633,12 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
 #pragma unroll
     for (int j = 0; j < CW; ++j) acc[j] = 0.0;
     if (H.n_vm > 0) {
+      // The uniforms of this pass's proposals, held across its tiles.
+      const typename E::AccState ust = E::acc_state(H, Uw);
       if constexpr (!STREAM) {
         if (active)
-          tile_pass<E>(H, reinterpret_cast<const float*>(res), 0, part, wpc,
-                       Uw, lane, acc);
+          tile_pass<E>(H, ust, reinterpret_cast<const float*>(res), 0, part,
+                       wpc, Uw, lane, acc);
       } else if (H.stream_mode == 1) {
         // Double buffering over an even number of tiles: tile ti is in A
         // when the pair starts; the copy of ti+1 into B overlaps the pass
@@ -614,7 +647,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
         for (uint32_t ti = 0; ti < n_tiles; ti += 2) {
           issue_tile(ti + 1, g_tile_b);
           if (active)
-            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
+            tile_pass<E>(H, ust, reinterpret_cast<const float*>(g_tile_a),
                          ti * tile_len, part, wpc, Uw, lane, acc);
           __syncthreads();
           if (ti + 2 < n_tiles)
@@ -622,7 +655,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
           else if (it < H.n_iter)
             issue_tile(0, g_tile_a);
           if (active)
-            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_b),
+            tile_pass<E>(H, ust, reinterpret_cast<const float*>(g_tile_b),
                          (ti + 1) * tile_len, part, wpc, Uw, lane, acc);
           __syncthreads();
         }
@@ -634,7 +667,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
             g_tile_a[i] = src[tile_src4<OPT>(i, H.n_cols, ld4, n4, ti)];
           __syncthreads();
           if (active)
-            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
+            tile_pass<E>(H, ust, reinterpret_cast<const float*>(g_tile_a),
                          ti * tile_len, part, wpc, Uw, lane, acc);
         }
       }
@@ -686,29 +719,163 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
 }
 
 
-// Many models in one launch: workgroup w runs chains work[w].chain0 .. of
-// model work[w].model, with that model's arguments (copied to LDS and
-// patched with this batch's seed, length and draw offset) and hot values.
-template <class E>
-__device__ __forceinline__ void batch_body(const BatchWork* __restrict__ work,
-                                           const ChainArgs* __restrict__ args,
-                                           const ChainHot* __restrict__ hots,
-                                           BatchStep step) {
-  __shared__ ChainArgs a_sh;
-  const BatchWork w = work[blockIdx.x];
-  // Padding of the XCD-aware order: the whole workgroup leaves at once.
-  if (w.model == kNoBatchModel) return;
-  ChainHot H = hots[w.model];
-  H.n_iter = step.n_iter;
-  if (threadIdx.x == 0) {
-    a_sh = args[w.model];
-    a_sh.seed += step.seed_add;
-    a_sh.n_iter = step.n_iter;
-    a_sh.draws += step.t0;
-    a_sh.draws_ld = step.draws_ld;
+// End of batch b (1-based) of one model, rows t0 .. t0 + bsize - 1 of its
+// draws, by the whole workgroup (a thread per chain): the batch acceptance
+// joins the history; with `adapt` the chain's Welford moments absorb the
+// batch's f32 draws and sd = sqrt(max(diag cov, 1e-12)), the seed
+// covariance (initial sd^2) until two draws (R/adaptive.R:487-497); the
+// population snapshot becomes the last f32 draws, which the next batch
+// starts from and draws its DE pairs from (R/adaptive.R:500-502).
+__device__ __noinline__ void batch_end_model(const BatchModelInfo* __restrict__ Mp,
+                                             uint32_t t0, uint32_t bsize,
+                                             uint32_t b, bool adapt) {
+  // Separate roundings, as the host orchestrator computes these moments.
+#pragma clang fp contract(off)
+  const BatchModelInfo M = *Mp;
+  const uint32_t np = M.n_params, C = M.n_chains;
+  const size_t ld = M.draws_ld;
+  for (uint32_t c = threadIdx.x; c < C; c += blockDim.x) {
+    if (M.accept_hist) M.accept_hist[(size_t)(b - 1) * C + c] = M.accept[c];
+    if (adapt) {
+      double* w = M.welford + (size_t)c * (1 + 2 * np);
+      double* mean = w + 1;
+      double* m2 = w + 1 + np;
+      double n = w[0];
+      for (uint32_t t = t0; t < t0 + bsize; ++t) {
+        n += 1.0;
+        for (uint32_t j = 0; j < np; ++j) {
+          const double x = (double)M.draws[t + ld * (c + (size_t)C * j)];
+          const double d1 = x - mean[j];
+          mean[j] += d1 / n;
+          m2[j] += d1 * (x - mean[j]);
+        }
+      }
+      w[0] = n;
+      for (uint32_t j = 0; j < np; ++j) {
+        const double s0 = M.sd0[(size_t)c * np + j];
+        const double var = n >= 2.0 ? m2[j] / (n - 1.0) : s0 * s0;
+        M.sd[(size_t)c * np + j] = sqrt(fmax(var, 1e-12));
+      }
+    }
+    const uint32_t last = t0 + bsize - 1;
+    for (uint32_t j = 0; j < np; ++j)
+      M.snapshot[(size_t)c * np + j] =
+          (double)M.draws[last + ld * (c + (size_t)C * j)];
   }
-  __syncthreads();
-  chain_body<E, true>(&a_sh, H, w.chain0);
+}
+
+// Thread 0 waits until *word >= v.  The wait is bounded: past `ticks` of
+// the wall clock, or once another workgroup has given up, it raises the
+// error word and returns false, so every waiting workgroup ends.
+__device__ __noinline__ bool batch_wait_at_least(uint32_t* word, uint32_t v,
+                                                 uint32_t* err, uint64_t ticks) {
+  if (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= v)
+    return true;
+  const uint64_t t_start = wall_clock64();
+  for (;;) {
+    __builtin_amdgcn_s_sleep(8);
+    if (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= v)
+      return true;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+      return false;
+    if (wall_clock64() - t_start > ticks) {
+      __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+  }
+}
+
+// Many models, every batch, one persistent launch (BatchQueue).  A
+// workgroup takes the next position of its XCD's queue, waits until the
+// item's model has ended the previous batch, and runs the item's chain
+// group through the batch with the model's arguments (copied to LDS and
+// patched with the batch's seed, length and draw offset).  The workgroup
+// completing the model's batch runs its between-batch step.
+//
+// No wait can close a cycle: an item waits only for items of the same
+// model's previous batch, which sit earlier in the same queue and were
+// taken by workgroups already running (a workgroup takes a position only
+// after finishing its previous item), and items of batch 1 wait for
+// nothing.  Writes are published with agent-scope release fences before the
+// counters move and read after agent-scope acquire fences, so the order
+// holds across CUs and XCDs alike.
+template <class E>
+__device__ __forceinline__ void batch_body(const BatchQueue* __restrict__ Q,
+                                           const ChainArgs* __restrict__ args,
+                                           const ChainHot* __restrict__ hots) {
+  struct Item {
+    ChainArgs a;
+    uint32_t pos, model, chain0, last;
+  };
+  __shared__ Item it_sh;
+  const uint32_t x = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+  if (slot >= Q->slots[x]) return;  // the whole workgroup leaves at once
+  uint32_t* next = Q->sync;
+  uint32_t* err = Q->sync + kXcds;
+  uint32_t* done = Q->sync + kBatchSyncHead;
+  uint32_t* ready = done + Q->n_models;
+  const uint32_t len = Q->len[x];
+  const uint32_t total = len * Q->n_batches;
+  const uint32_t nb = Q->n_batches, every = Q->every;
+  const int tid = (int)threadIdx.x;
+  for (;;) {
+    __syncthreads();  // the previous item is done with it_sh
+    if (tid == 0) {
+      uint32_t pos = total;
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+        pos = atomicAdd(&next[x], 1u);
+      if (pos < total) {
+        const uint32_t b = pos / len + 1;
+        const BatchWork w = Q->items[Q->off[x] + (pos - (b - 1) * len)];
+        if (!batch_wait_at_least(&ready[w.model], b - 1, err, Q->wait_ticks))
+          pos = total;
+        it_sh.model = w.model;
+        it_sh.chain0 = w.chain0;
+      }
+      it_sh.pos = pos;
+    }
+    __syncthreads();
+    // Workgroup-uniform values read from LDS: readfirstlane keeps them (and
+    // the model's hot values loaded with them) in scalar registers.
+    const uint32_t pos = __builtin_amdgcn_readfirstlane(it_sh.pos);
+    if (pos >= total) break;  // uniform: every wave leaves here
+    // The previous batch's snapshot, scales and Welford moments, written by
+    // other workgroups, are read after this fence.
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const uint32_t b = pos / len + 1;
+    const uint32_t m = __builtin_amdgcn_readfirstlane(it_sh.model);
+    const uint32_t chain0 = __builtin_amdgcn_readfirstlane(it_sh.chain0);
+    const uint32_t n_it = b == nb ? Q->n_iter - (nb - 1) * every : every;
+    const uint32_t t0 = (b - 1) * every;
+    if (tid == 0) {
+      it_sh.a = args[m];
+      it_sh.a.seed += b - 1;
+      it_sh.a.n_iter = n_it;
+      it_sh.a.draws += t0;
+      it_sh.a.draws_ld = Q->n_iter;
+    }
+    __syncthreads();
+    ChainHot H = hots[m];
+    H.n_iter = n_it;
+    chain_body<E, true>(&it_sh.a, H, chain0);
+    // Publish the group's draws and acceptance, then count the item.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t d = atomicAdd(&done[m], 1u) + 1u;
+      it_sh.last = d == b * Q->models[m].n_groups ? 1u : 0u;
+    }
+    __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(it_sh.last)) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      batch_end_model(&Q->models[m], t0, n_it, b, b <= Q->warmup_batches);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (tid == 0)
+        __hip_atomic_store(&ready[m], b, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // Data term of a list of parameter points (no chain state).  A persistent
@@ -771,6 +938,9 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
   const float4* src = reinterpret_cast<const float4*>(H.data);
   const TileCopy cp =
       tile_copy_plan<OPT>(H.n_cols, ld4, n4, tile4, (uint32_t)(wave * 64));
+  auto issue_tile = [&](uint32_t ti, float4* dst) {
+    issue_tile_dma(src, cp, n4, lane, ti, dst);
+  };
   bool loaded = false;  // resident layout: data loaded with the first item
   for (;;) {
     __syncthreads();  // previous item's reads of U and the tiles are done
@@ -795,6 +965,8 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
 #pragma unroll
     for (int j = 0; j < CW; ++j) acc[j] = 0.0;
     if (H.n_vm > 0) {
+      // The wave's own lead lanes wrote these uniforms just above.
+      const typename E::AccState ust = E::acc_state(H, Uw);
       if constexpr (!STREAM) {
         // One tile holds the whole data set; load it once per workgroup.
         if (!loaded) {
@@ -805,24 +977,22 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
         }
         __syncthreads();
         if (active && ti0 < ti1)
-          tile_pass<E>(H, reinterpret_cast<const float*>(smem + H.lds_tile_offset),
+          tile_pass<E>(H, ust,
+                       reinterpret_cast<const float*>(smem + H.lds_tile_offset),
                        0, 0, 1, Uw, lane, acc);
       } else if (H.stream_mode == 1) {
-        if (ti0 < ti1)
-          issue_tile_dma(src, cp, n4, lane, ti0, g_tile_a);
+        if (ti0 < ti1) issue_tile(ti0, g_tile_a);
         __syncthreads();
         for (uint32_t ti = ti0; ti < ti1; ti += 2) {
           const bool two = ti + 1 < ti1;
-          if (two)
-            issue_tile_dma(src, cp, n4, lane, ti + 1, g_tile_b);
+          if (two) issue_tile(ti + 1, g_tile_b);
           if (active)
-            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
+            tile_pass<E>(H, ust, reinterpret_cast<const float*>(g_tile_a),
                          ti * tile_len, 0, 1, Uw, lane, acc);
           __syncthreads();
-          if (ti + 2 < ti1)
-            issue_tile_dma(src, cp, n4, lane, ti + 2, g_tile_a);
+          if (ti + 2 < ti1) issue_tile(ti + 2, g_tile_a);
           if (two && active)
-            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_b),
+            tile_pass<E>(H, ust, reinterpret_cast<const float*>(g_tile_b),
                          (ti + 1) * tile_len, 0, 1, Uw, lane, acc);
           __syncthreads();
         }
@@ -833,7 +1003,7 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
             g_tile_a[i] = src[tile_src4<OPT>(i, H.n_cols, ld4, n4, ti)];
           __syncthreads();
           if (active)
-            tile_pass<E>(H, reinterpret_cast<const float*>(g_tile_a),
+            tile_pass<E>(H, ust, reinterpret_cast<const float*>(g_tile_a),
                          ti * tile_len, 0, 1, Uw, lane, acc);
         }
       }
@@ -884,8 +1054,12 @@ struct VmEval {
   static constexpr int kOpt = OPT;
   static constexpr int kCw = CW;
   static constexpr bool kPairs = NS == 0;
+  static constexpr int kUnit = 0;
   using Params = VmParams;
-  static __device__ __forceinline__ void run(const ChainHot& H, const float* U,
+  // The interpreter reads the uniforms as it goes.
+  struct AccState {};
+  static __device__ __forceinline__ void run(const ChainHot& H,
+                                             const AccState&, const float* U,
                                              const float* tile, uint32_t ld,
                                              int lane, float (&T)[CW][OPT]) {
     vm_run<NS, OPT, CW>((vm_prog_t)H.vm, H.n_vm, U, H.n_uniform_pad, tile, ld,
@@ -894,8 +1068,6 @@ struct VmEval {
   static __device__ __forceinline__ uint32_t mode(const ChainHot& H) {
     return H.sum_mode;
   }
-  // The interpreter reads the uniforms as it goes.
-  struct AccState {};
   static __device__ __forceinline__ AccState acc_state(const ChainHot&,
                                                        const float*) {
     return AccState{};
@@ -935,8 +1107,11 @@ struct VmAffineEval {
   static constexpr int kOpt = OPT;
   static constexpr int kCw = CW;
   static constexpr bool kPairs = true;
+  static constexpr int kUnit = 0;
   using Params = VmParams;
-  static __device__ __forceinline__ void run(const ChainHot& H, const float* U,
+  struct AccState {};
+  static __device__ __forceinline__ void run(const ChainHot& H,
+                                             const AccState&, const float* U,
                                              const float* tile, uint32_t ld,
                                              int lane, float (&T)[CW][OPT]) {
     const vm_prog_t prog = (vm_prog_t)H.vm;
@@ -965,7 +1140,6 @@ struct VmAffineEval {
   static __device__ __forceinline__ uint32_t mode(const ChainHot&) {
     return SUM_AFFINE;
   }
-  struct AccState {};
   static __device__ __forceinline__ AccState acc_state(const ChainHot&,
                                                        const float*) {
     return AccState{};

@@ -115,27 +115,62 @@ constexpr size_t kStreamStaticLds = 2 * (size_t)kTileF4 * 16;
 // Copies per wave that fill one buffer with 1024 threads.
 constexpr uint32_t kTileIssues = (kTileF4 + 1023) / 1024;
 
-// Batched launches: one workgroup per (model, first chain) item.
+// Batched runs: a work item is one chain group (model, first chain) for one
+// batch of iterations.
 struct BatchWork {
-  uint32_t model;   // kNoBatchModel: an empty item (the workgroup exits)
+  uint32_t model;
   uint32_t chain0;
 };
-constexpr uint32_t kNoBatchModel = 0xFFFFFFFFu;
 // XCDs of an MI355X; the dispatcher hands workgroup w to XCD w mod kXcds.
 constexpr uint32_t kXcds = 8;
-// Per-launch values of a batched run (one batch of iterations).
-struct BatchStep {
-  uint32_t seed_add;   // added to each model's seed
-  uint32_t n_iter;     // iterations of this batch
-  uint32_t t0;         // first draw row of this batch
-  uint32_t draws_ld;   // iteration stride of the models' draws
+
+// Per-model view of the flat per-chain buffers of a batched run.
+struct BatchModelInfo {
+  uint32_t chain_off;  // first global chain of the model
+  uint32_t n_chains;
+  uint32_t n_params;
+  uint32_t n_groups;     // chain groups (work items) per batch
+  uint32_t draws_ld;     // iterations of the whole run
+  const float* draws;    // reference layout t + draws_ld (c + C j)
+  double* snapshot;      // C x np: start states / DE population
+  double* sd;            // C x np proposal scale
+  const double* sd0;     // C x np initial proposal scale (Welford seed)
+  double* welford;       // C x (1 + 2 np): count, mean, M2
+  const float* accept;   // C: acceptance of the batch just run
+  float* accept_hist;    // n_batches x C
 };
+
+// One persistent launch runs every batch of every model.  Workgroup w
+// serves queue x = w mod kXcds (its XCD) as slot w / kXcds; queue x holds
+// items[off[x] .. off[x] + len[x]) once per batch, batch-major, so the
+// queue position i is item i mod len[x] of batch i / len[x] + 1.  An item
+// of batch b starts once batch b-1 of its model has ended (ready[model] >=
+// b-1); the workgroup that completes a model's batch runs its between-batch
+// step and then publishes ready[model] = b.
+// sync: next[kXcds] (queue heads), error word, done[n_models] (items
+// completed), ready[n_models] (batches ended); all zero at launch.
+struct BatchQueue {
+  const BatchWork* items;
+  BatchModelInfo* models;
+  uint32_t* sync;
+  uint32_t off[kXcds];
+  uint32_t len[kXcds];
+  uint32_t slots[kXcds];    // workgroups serving each queue
+  uint32_t n_models;
+  uint32_t n_batches;
+  uint32_t every;           // iterations per batch; the last absorbs the rest
+  uint32_t n_iter;          // iterations of the whole run (draw row stride)
+  uint32_t warmup_batches;  // leading batches whose step adapts the scale
+  uint32_t pad;
+  uint64_t wait_ticks;      // longest wait for a dependency, wall-clock ticks
+};
+constexpr uint32_t kBatchSyncHead = kXcds + 1;  // words before done[]
 // Static LDS of the batched kernel beyond the tile buffers (its copy of
-// the model's arguments).
+// the model's arguments and the item being run).
 constexpr size_t kBatchArgsLds = 512;
 // Static LDS of the points kernel beyond the tile buffers (item and plan).
 constexpr size_t kPointsExtraLds = 64;
-static_assert(sizeof(ChainArgs) <= kBatchArgsLds, "ChainArgs too large");
+static_assert(sizeof(ChainArgs) + 16 <= kBatchArgsLds, "ChainArgs too large");
 
 // The data term of a list of parameter points, summed per observation
 // slice by a persistent grid of n_workgroups that takes (slot group, slice)
@@ -162,11 +197,11 @@ struct PointsArgs {
 hipError_t launch_chain_kernel(const ChainArgs* dev_args, const ChainHot& h,
                                int ns, int opt, int cw, size_t lds_bytes,
                                hipStream_t s);
-hipError_t launch_chain_kernel_batch(const BatchWork* work, uint32_t n_wg,
+// `q` is a device copy of the queue; n_wg = kXcds * the most slots.
+hipError_t launch_chain_kernel_batch(const BatchQueue* q, uint32_t n_wg,
                                      const ChainArgs* args,
-                                     const ChainHot* hots, BatchStep step,
-                                     int ns, int opt, int cw,
-                                     size_t lds_bytes, hipStream_t s);
+                                     const ChainHot* hots, int ns, int opt,
+                                     int cw, size_t lds_bytes, hipStream_t s);
 // `p` is a device copy of the arguments.
 hipError_t launch_points_kernel(const PointsArgs* p, uint32_t n_workgroups,
                                 const ChainHot& h, int ns, int opt, int cw,

@@ -76,15 +76,14 @@ hipError_t launch_points_kernel(const PointsArgs* p, uint32_t n_workgroups,
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_chain_kernel_batch(const BatchWork* work, uint32_t n_wg,
+hipError_t launch_chain_kernel_batch(const BatchQueue* q, uint32_t n_wg,
                                      const ChainArgs* args,
-                                     const ChainHot* hots, BatchStep step,
-                                     int ns, int opt, int cw,
-                                     size_t lds_bytes, hipStream_t s) {
+                                     const ChainHot* hots, int ns, int opt,
+                                     int cw, size_t lds_bytes, hipStream_t s) {
 #define GPUM_AFFINE_CASE(OPT_, CW_)                                          \
   if (ns == 0 && opt == OPT_ && cw == CW_) {                                 \
     hipLaunchKernelGGL((chain_kernel_batch_affine<OPT_, CW_>), dim3(n_wg),   \
-                       dim3(1024), lds_bytes, s, work, args, hots, step);    \
+                       dim3(1024), lds_bytes, s, q, args, hots);             \
     return hipGetLastError();                                                \
   }
   GPUM_AFFINE_VARIANTS(GPUM_AFFINE_CASE)
@@ -92,7 +91,7 @@ hipError_t launch_chain_kernel_batch(const BatchWork* work, uint32_t n_wg,
 #define GPUM_CASE(NS_, OPT_, CW_)                                            \
   if (ns == NS_ && opt == OPT_ && cw == CW_) {                               \
     hipLaunchKernelGGL((chain_kernel_batch<NS_, OPT_, CW_>), dim3(n_wg),     \
-                       dim3(1024), lds_bytes, s, work, args, hots, step);    \
+                       dim3(1024), lds_bytes, s, q, args, hots);             \
     return hipGetLastError();                                                \
   }
   GPUM_VARIANTS(GPUM_CASE)

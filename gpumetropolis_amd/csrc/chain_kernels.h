@@ -36,10 +36,10 @@ chain_kernel(const ChainArgs* __restrict__ Ap, ChainHot H) {
 
 template <int NS, int OPT, int CW>
 __global__ void __launch_bounds__(1024)
-chain_kernel_batch(const BatchWork* __restrict__ work,
+chain_kernel_batch(const BatchQueue* __restrict__ q,
                    const ChainArgs* __restrict__ args,
-                   const ChainHot* __restrict__ hots, BatchStep step) {
-  batch_body<VmEval<NS, OPT, CW>>(work, args, hots, step);
+                   const ChainHot* __restrict__ hots) {
+  batch_body<VmEval<NS, OPT, CW>>(q, args, hots);
 }
 
 template <int NS, int OPT, int CW, bool STREAM>
@@ -57,10 +57,10 @@ chain_kernel_affine(const ChainArgs* __restrict__ Ap, ChainHot H) {
 
 template <int OPT, int CW>
 __global__ void __launch_bounds__(1024)
-chain_kernel_batch_affine(const BatchWork* __restrict__ work,
+chain_kernel_batch_affine(const BatchQueue* __restrict__ q,
                           const ChainArgs* __restrict__ args,
-                          const ChainHot* __restrict__ hots, BatchStep step) {
-  batch_body<VmAffineEval<OPT, CW>>(work, args, hots, step);
+                          const ChainHot* __restrict__ hots) {
+  batch_body<VmAffineEval<OPT, CW>>(q, args, hots);
 }
 
 template <int OPT, int CW, bool STREAM>
@@ -87,7 +87,7 @@ points_kernel_affine(const PointsArgs* __restrict__ P, ChainHot H) {
   PRE template __global__ void points_kernel_affine<OPT, CW, true>(          \
       const PointsArgs*, ChainHot);                                           \
   PRE template __global__ void chain_kernel_batch_affine<OPT, CW>(           \
-      const BatchWork*, const ChainArgs*, const ChainHot*, BatchStep);
+      const BatchQueue*, const ChainArgs*, const ChainHot*);
 #define GPUM_AFFINE_INSTANTIATE(OPT, CW) GPUM_AFFINE_KERNELS(, OPT, CW)
 #define GPUM_AFFINE_DECLARE(OPT, CW) GPUM_AFFINE_KERNELS(extern, OPT, CW)
 
@@ -118,7 +118,7 @@ points_kernel_affine(const PointsArgs* __restrict__ P, ChainHot H) {
   PRE template __global__ void points_kernel<NS, OPT, CW, true>(             \
       const PointsArgs*, ChainHot);                                           \
   PRE template __global__ void chain_kernel_batch<NS, OPT, CW>(              \
-      const BatchWork*, const ChainArgs*, const ChainHot*, BatchStep);
+      const BatchQueue*, const ChainArgs*, const ChainHot*);
 #define GPUM_INSTANTIATE(NS, OPT, CW) GPUM_KERNELS(, NS, OPT, CW)
 #define GPUM_DECLARE(NS, OPT, CW) GPUM_KERNELS(extern, NS, OPT, CW)
 

@@ -299,11 +299,7 @@ static bool pick_tiling_cw(const gpum_hip_model* m, uint32_t n_chains,
   // chunks) for programs without spill slots whose two chunks of rows fit
   // one tile buffer.  The choice depends on the model alone (its program
   // picks the variant), so every tiling sums alike.
-  t->unit = t->v.ns == 0 &&
-                    (size_t)2 * t->chunk * std::max(1u, m->n_cols_total) * 4 <=
-                        (size_t)kTileF4 * 16
-                ? 2u
-                : 1u;
+  t->unit = unit_chunks_for(t->v.ns, t->v.opt, m->n_cols_total);
   const uint32_t ncu = (uint32_t)g_dev.n_cu;
   const bool has_tiling =
       ext && ext->struct_size >= offsetof(gpum_run_ext, waves_per_chain) + 4;

@@ -14,6 +14,7 @@
 #include <sstream>
 #include <vector>
 
+#include "chain_kernel.h"
 #include "lower.h"
 #include "vm.h"
 
@@ -53,9 +54,9 @@ const char* kKernels[JIT_NUM_KINDS] = {
     "gpum_jit_kernel(const gpum::PointsArgs* P, gpum::ChainHot H) {\n"
     "  gpum::points_body<gpum::JitEval, false>(P, H);\n}\n",
     "extern \"C\" __global__ void __launch_bounds__(1024)\n"
-    "gpum_jit_kernel(const gpum::BatchWork* w, const gpum::ChainArgs* a,\n"
-    "                const gpum::ChainHot* h, gpum::BatchStep s) {\n"
-    "  gpum::batch_body<gpum::JitEval>(w, a, h, s);\n}\n",
+    "gpum_jit_kernel(const gpum::BatchQueue* q, const gpum::ChainArgs* a,\n"
+    "                const gpum::ChainHot* h) {\n"
+    "  gpum::batch_body<gpum::JitEval>(q, a, h);\n}\n",
     "extern \"C\" __global__ void __launch_bounds__(256)\n"
     "gpum_jit_kernel(gpum::Eval64Args a) {\n"
     "  gpum::pointwise64_body<gpum::JitEval64>(a);\n}\n",
@@ -369,11 +370,26 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
   return o.str();
 }
 
+uint32_t unit_chunks_for(int ns, int opt, uint32_t n_cols) {
+  const size_t chunk = 64u * (size_t)opt;
+  return ns == 0 && 2 * chunk * std::max(1u, n_cols) * 4 <= (size_t)kTileF4 * 16
+             ? 2u
+             : 1u;
+}
+
 std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
                                  size_t n_prior, const double* prior_consts,
                                  int ns, int opt, int cw) {
   std::ostringstream o;
   const int nsd = ns > 0 ? ns : 1;
+  const uint32_t n_cols =
+      L.empty ? 0u : (uint32_t)(L.n_cols_in + L.extra_cols.size());
+  // Programs without spill slots whose uniforms fit a register budget keep
+  // the wave's chains' uniform table in registers for a whole data pass
+  // (read once per pass, not once per tile); the program then indexes that
+  // copy with constant offsets.
+  const int nup = std::max(1, L.n_uniform);
+  const bool held = ns == 0 && L.sum_mode != SUM_AFFINE && nup * cw <= 64;
   o << "#include \"chain_body.h\"\n"
     << "namespace gpum {\n"
     << "struct JitEval {\n"
@@ -381,27 +397,43 @@ std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
     << "  static constexpr int kCw = " << cw << ";\n"
     << "  static constexpr bool kPairs = " << (ns == 0 ? "true" : "false")
     << ";\n"
-    << "  static __device__ __forceinline__ void run(const ChainHot& H,\n"
-    << "      const float* __restrict__ U, const float* __restrict__ tile,\n"
+    << "  static constexpr int kUnit = " << unit_chunks_for(ns, opt, n_cols)
+    << ";\n";
+  if (held)
+    o << "  struct AccState { float u[" << cw * nup << "]; };\n";
+  else
+    o << "  struct AccState { float u[" << std::max<size_t>(1, L.vm.size())
+      << "][" << cw << "]; };\n";
+  o << "  static __device__ __forceinline__ void run(const ChainHot& H,\n"
+    << "      const AccState& st, const float* __restrict__ U,\n"
+    << "      const float* __restrict__ tile,\n"
     << "      uint32_t ld, int lane, float (&T)[" << cw << "][" << opt << "]) {\n"
     << "    float S[" << cw << "][" << nsd << "][" << opt << "];\n"
-    << "    float D[" << opt << "];\n"
-    << "    const uint32_t nu = H.n_uniform_pad;\n";
+    << "    float D[" << opt << "];\n";
+  if (held)
+    o << "    const float* Ur = st.u;\n"
+      << "    constexpr uint32_t nu = " << nup << "u;\n";
+  else
+    o << "    const float* Ur = U;\n"
+      << "    const uint32_t nu = H.n_uniform_pad;\n";
   for (const VmInstr& ins : L.vm)
     o << "    vm_step<" << ns << ", " << opt << ", " << cw << ">(" << vm_op(ins)
       << "u, " << vm_slot(ins) << "u, " << vm_a(ins) << "u, " << vm_b(ins)
-      << "u, U, nu, tile, ld, lane, T, S, D);\n";
+      << "u, Ur, nu, tile, ld, lane, T, S, D);\n";
   o << "  }\n"
     << "  static __device__ __forceinline__ uint32_t mode(const ChainHot&) {\n"
     << "    return " << L.sum_mode << "u;\n  }\n"
-    << "  struct AccState { float u[" << std::max<size_t>(1, L.vm.size())
-    << "][" << cw << "]; };\n"
     << "  static __device__ __forceinline__ AccState acc_state(const ChainHot& H,\n"
     << "      const float* __restrict__ U) {\n"
     << "    AccState st;\n"
     << "    const uint32_t nu = H.n_uniform_pad;\n";
-  // The uniforms of the affine form's products, read once per tile pass.
-  if (L.sum_mode == SUM_AFFINE)
+  // The uniforms of the affine form's products, or the whole table of the
+  // wave's chains, read once per data pass.
+  if (held)
+    o << "    for (int j = 0; j < " << cw << "; ++j)\n"
+      << "      for (int k = 0; k < " << nup << "; ++k)\n"
+      << "        st.u[j * " << nup << " + k] = U[j * nu + k];\n";
+  else if (L.sum_mode == SUM_AFFINE)
     for (size_t pc = 1; pc < L.vm.size(); ++pc)
       o << "    for (int j = 0; j < " << cw << "; ++j) st.u[" << pc
         << "][j] = U[j * nu + " << vm_a(L.vm[pc]) << "u];\n";

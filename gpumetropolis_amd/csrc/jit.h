@@ -23,6 +23,13 @@ enum JitKind { JIT_CHAIN_STREAM = 0, JIT_CHAIN_RESIDENT, JIT_POINTS_STREAM,
 // from jit_eval64_source.
 constexpr int kJitChainKinds = JIT_POINTWISE64;
 
+// Chunks of 64*opt observations that one FP32 partial covers for a program
+// of `ns` spill slots over rows of n_cols columns: two when the program has
+// no spill slots and two chunks of rows fit one LDS tile buffer (kTileF4
+// float4s), else one.  The tiling and the compiled evaluator both use it, so
+// a model's observations are grouped alike on every path.
+uint32_t unit_chunks_for(int ns, int opt, uint32_t n_cols);
+
 // Source of the evaluator for a lowered program run with variant
 // (ns, opt, cw): ns spill slots, opt observations per lane per chunk, cw
 // chains per wave; with the model's FP64 parameter programs (the lowered

@@ -436,14 +436,19 @@ int gpum_hip_run_de_sync(const uint32_t* code, size_t code_len,
 /* ---- Many models in one run -------------------------------------------
  * No single-call counterpart in the reference, whose host runs one model
  * per rust_gpu_metropolis call (R/api.R:247-599).  Each model keeps its own
- * program, data, chains, seed and Philox stream id; every batch of
- * iterations of every model is one launch, the between-batch step of the
- * reference's orchestrator (R/adaptive.R:427-524 for DE path A: the
- * population snapshot is refreshed from the last f32 draws, and during the
- * warmup batches each chain's sd becomes sqrt(diag cov) of its draws so
- * far) is a second launch, and with use_graph all batches are one HIP
- * graph.  Batch b (1-based) draws its uniforms with seed + b - 1, as the
- * reference reseeds each batch (R/adaptive.R:471). */
+ * program, data, chains, seed and Philox stream id.  One persistent launch
+ * runs every batch of iterations of every model: each chain group of each
+ * model is a work item per batch, an item starts once its model has ended
+ * the previous batch, and the workgroup that ends a model's batch runs the
+ * between-batch step of the reference's orchestrator for it
+ * (R/adaptive.R:427-524 for DE path A: the population snapshot is
+ * refreshed from the last f32 draws, and during the warmup batches each
+ * chain's sd becomes sqrt(diag cov) of its draws so far).  With use_graph
+ * the launch (and the reset of its queue counters) is one HIP graph.
+ * Batch b (1-based) draws its uniforms with seed + b - 1, as the reference
+ * reseeds each batch (R/adaptive.R:471).  A run in which an item waits
+ * more than 30 s for its model's previous batch is abandoned with an
+ * error. */
 typedef struct gpum_batch_model {
   gpum_hip_model* model;
   uint32_t n_chains;
@@ -474,9 +479,9 @@ typedef struct gpum_batch_opts {
                                  chains_per_block / chains_per_wave *
                                  waves_per_chain = 16 */
   int32_t waves_per_chain;
-  int32_t use_graph;          /* capture all batches in one HIP graph */
+  int32_t use_graph;          /* launch through a HIP graph */
   double kernel_ms;           /* out: device time of the run */
-  uint32_t n_workgroups;      /* out: workgroups per batch launch */
+  uint32_t n_workgroups;      /* out: workgroups of the persistent launch */
   int32_t chains_per_wave;    /* 0 or 1: one chain per wave; 2: two (every
                                  model of vm_depth 1); 4: four (also every
                                  model compiled, one program) */

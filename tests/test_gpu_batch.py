@@ -1,4 +1,4 @@
-"""Many models in one batched launch (HIP graph of per-batch launches).
+"""Many models in one persistent batched launch.
 
 The batched run must reproduce, draw for draw, the reference's per-model
 DE orchestration (R/adaptive.R:427-524, restated by the host mirror) run
@@ -75,8 +75,8 @@ def test_batch_reproduces_per_model_orchestration(hip):
         assert np.allclose(got.accept_rate, ref.accept_rate, rtol=1e-6)
         assert 0.02 < float(np.mean(got.accept_rate)) < 0.98
     # ceil(16/16), ceil(24/16), ceil(32/16) chain groups; each model goes
-    # to its own XCD, so the longest XCD list holds 2 and the grid is
-    # padded to 8 XCDs x 2
+    # to its own XCD queue, which gets one workgroup per item of a batch
+    # (at most 2), and the grid is 8 XCDs x 2 slots
     assert bf.n_workgroups == 8 * 2
 
 

@@ -183,17 +183,19 @@ def _logistic_fit(hip, d=6, n=40000, C=256, warm=400, seed=7):
     return wl, dm, w, fit
 
 
-def test_da_replays_oracle(hip):
+@pytest.mark.parametrize("d,n,C", [(6, 40000, 64), (16, 20000, 128)])
+def test_da_replays_oracle(hip, d, n, C):
     """Every stage-1 decision follows from the trace's surrogate values and
     the Philox uniforms; every exact value matches the oracle to 1e-6; every
-    stage-2 decision matches or is a certified near-tie."""
+    stage-2 decision matches or is a certified near-tie.  d = 16 is the
+    dimension of the 16-parameter logistic regression workload."""
     from oracle import da
-    print("[da] replay", flush=True)
-    wl, dm, w, fit = _logistic_fit(hip, C=64)
+    print(f"[da] replay d={d} n={n} C={C}", flush=True)
+    wl, dm, w, fit = _logistic_fit(hip, d=d, n=n, C=C)
     print(fit.summary())
     assert fit.usable
-    d = wl.model.n_params
-    C, T = 64, 120
+    assert wl.model.n_params == d
+    T = 120
     L = w.L.reshape(C, d, d)
     r = dm.run_da(fit.net, w.init, np.ones(C), T, 99, proposal_l=w.L,
                   trace=True)
