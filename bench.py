@@ -692,7 +692,7 @@ def main():
     # torch is not imported: it bundles its own HIP runtime, and the
     # library's calls already return after their stream has drained.
     from gpumetropolis_amd import _native, adaptive as ad, workloads
-    from gpumetropolis_amd.api import adaptive_warmup
+    from gpumetropolis_amd.api import device_warmup
     from gpumetropolis_amd.diagnostics import summarize
 
     if not _native.available():
@@ -727,7 +727,10 @@ def main():
     sd0 = np.full((C, P), 0.1)
     t0 = time.perf_counter()
     plan = ad.warmup_batches(a.warmup * a.iters_per_step)
-    wu = adaptive_warmup(call, P, C, init, sd0, plan, a.seed)
+    # The reference's adaptive warmup (R/adaptive.R:101-254), every batch
+    # and between-batch step on the device in one graph.
+    wu = device_warmup(dm, P, C, init, sd0, a.warmup * a.iters_per_step,
+                       a.seed)
     warm_s = time.perf_counter() - t0
     warm_acc = float(wu.accept_history[:, -1].mean())
     print(f"[bench] warmup {sum(plan)} it in {len(plan)} batches done in "
@@ -812,8 +815,11 @@ def main():
         "ess_min": ess_min,
         "sampling_accept": float(np.mean(accs)),
         "warmup_adaptation": {"iterations": sum(plan), "batches": len(plan),
+                   "engine": "device: one HIP graph of every batch and "
+                             "between-batch step",
                    "last_batch_accept": warm_acc,
-                   "target_accept": ad.target_accept(P), "wall_s": warm_s},
+                   "target_accept": ad.target_accept(P),
+                   "kernel_ms": wu.kernel_ms, "wall_s": warm_s},
         "posterior_check": posterior_check,
         "posterior": [{"mean": p["mean"], "sd": p["sd"], "mcse": p["mcse"]}
                       for p in summ["params"]],

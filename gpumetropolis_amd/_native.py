@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 from dataclasses import dataclass
-from typing import Optional
+from typing import Dict, Optional
 
 import numpy as np
 
@@ -88,6 +88,20 @@ class BatchOpts(C.Structure):
                 ("waves_per_chain", C.c_int32), ("use_graph", C.c_int32),
                 ("kernel_ms", C.c_double), ("n_workgroups", C.c_uint32),
                 ("chains_per_wave", C.c_int32)]
+
+
+class WarmupOpts(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("warmup", C.c_uint32),
+                ("seed", C.c_uint32), ("auto_stop", C.c_int32),
+                ("use_graph", C.c_int32),
+                ("final_init", C.POINTER(C.c_double)),
+                ("final_sd", C.POINTER(C.c_double)),
+                ("final_scale", C.POINTER(C.c_double)),
+                ("final_l", C.POINTER(C.c_double)),
+                ("accept_hist", C.POINTER(C.c_float)),
+                ("n_batches_planned", C.c_uint32), ("n_batches", C.c_uint32),
+                ("warmup_used", C.c_uint32), ("has_l", C.c_uint32),
+                ("kernel_ms", C.c_double)]
 
 
 class TrainOpts(C.Structure):
@@ -211,6 +225,9 @@ def lib():
                                         C.POINTER(DaExt),
                                         C.POINTER(ChainResult)]
     L.gpum_hip_model_run_da.restype = C.c_int
+    L.gpum_hip_model_warmup.argtypes = [C.c_void_p, _f64p, C.c_uint32, _f64p,
+                                        C.POINTER(WarmupOpts)]
+    L.gpum_hip_model_warmup.restype = C.c_int
     L.gpum_hip_batch_run.argtypes = [C.POINTER(BatchModel), C.c_uint32,
                                      C.POINTER(BatchOpts)]
     L.gpum_hip_batch_run.restype = C.c_int
@@ -572,6 +589,46 @@ class DeviceModel:
         _check(rc)
         return _finish(_collect(res, raw_draws), ext, keep, n_iter, n_chains,
                        np_)
+
+    def warmup(self, init, proposal_sd, warmup, seed, auto_stop=False,
+               use_graph=True) -> Dict:
+        """The adaptive warmup of R/adaptive.R:101-254 on the device
+        (gpum_hip_model_warmup): returns the final states, proposal sd,
+        Robbins-Monro scalars, proposal factors L (n_chains x np x np, or
+        None), the acceptance history (n_chains x batches run), the batch
+        plan and the device time."""
+        np_ = self.n_params
+        init, sd = _f64(init), _f64(proposal_sd)
+        n_chains = init.size // np_
+        nb_max = min(40, int(warmup))
+        out_init = np.empty(n_chains * np_)
+        out_sd = np.empty(n_chains * np_)
+        out_scale = np.empty(n_chains)
+        out_l = np.empty(n_chains * np_ * np_)
+        hist = np.full(nb_max * n_chains, np.nan, np.float32)
+        o = WarmupOpts()
+        o.struct_size = C.sizeof(WarmupOpts)
+        o.warmup = int(warmup)
+        o.seed = int(seed) & 0xFFFFFFFF
+        o.auto_stop = 1 if auto_stop else 0
+        o.use_graph = 1 if use_graph else 0
+        o.final_init = _p(out_init, C.c_double)
+        o.final_sd = _p(out_sd, C.c_double)
+        o.final_scale = _p(out_scale, C.c_double)
+        o.final_l = _p(out_l, C.c_double)
+        o.accept_hist = _p(hist, C.c_float)
+        _check(self._L.gpum_hip_model_warmup(self._h, _p(init, C.c_double),
+                                             n_chains, _p(sd, C.c_double),
+                                             C.byref(o)))
+        nb = int(o.n_batches)
+        return {"init": out_init.reshape(n_chains, np_),
+                "sd": out_sd.reshape(n_chains, np_),
+                "scale": out_scale,
+                "L": (out_l.reshape(n_chains, np_, np_) if o.has_l else None),
+                "accept_hist": hist.reshape(nb_max, n_chains)[:nb].T.copy(),
+                "n_batches_planned": int(o.n_batches_planned),
+                "n_batches": nb, "warmup_used": int(o.warmup_used),
+                "kernel_ms": float(o.kernel_ms)}
 
     def run_da(self, surrogate, init, temperatures, n_iter, seed,
                proposal_sd=None, proposal_l=None, trace=False,

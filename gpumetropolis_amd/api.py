@@ -94,7 +94,8 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
                    de_noise: float = 1e-3, de_every=None, de_sync=False,
                    seed: int = 1, backend: str = "hip",
                    tiling=None, ladder_size=None, surrogate=None,
-                   engine: str = "auto", jit="auto") -> Fit:
+                   engine: str = "auto", jit="auto",
+                   device_warmup: bool = True) -> Fit:
     if backend not in ("hip", "auto"):
         raise ValueError(f"backend '{backend}' is not available here; this "
                          "package is the hip backend")
@@ -155,7 +156,8 @@ def gpu_metropolis(model: Model, data=None, init=None, proposal_sd=0.1,
             fit = _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup,
                                 seed, warmup_auto, n_chains,
                                 surrogate=surrogate, dm=dm,
-                                langevin=method == "mala")
+                                langevin=method == "mala",
+                                on_device=device_warmup)
         elif surrogate:
             raise ValueError("`surrogate` needs the adaptive warmup "
                              "(adapt = TRUE and warmup > 0) for its training "
@@ -257,6 +259,7 @@ class WarmupResult:
     accept_history: np.ndarray
     pairs_theta: Optional[np.ndarray] = None   # (n, d) proposals
     pairs_loglik: Optional[np.ndarray] = None  # (n,) their data terms
+    kernel_ms: float = 0.0                     # device time (device warmup)
 
 
 def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
@@ -321,7 +324,8 @@ def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
         used += sizes[b - 1]
         if auto_stop and b >= max(10, reset_batch + 5):
             drift = float(np.max(np.abs(scales / prev_scales - 1)))
-            if np.mean(np.abs(r.accept_rate - target)) < 0.05 and drift < 0.03:
+            acc = r.accept_rate.astype(np.float64)  # R compares in double
+            if np.mean(np.abs(acc - target)) < 0.05 and drift < 0.03:
                 streak += 1
             else:
                 streak = 0
@@ -335,16 +339,36 @@ def adaptive_warmup(call, n_params, n_chains, init_mat, sd_mat, sizes, seed,
                         np.concatenate(pll) if pll else None)
 
 
+def device_warmup(dm, n_params, n_chains, init_mat, sd_mat, warmup, seed,
+                  auto_stop=False) -> WarmupResult:
+    """.am_orchestrate_warmup (R/adaptive.R:101-254) for random-walk chains
+    with every batch and between-batch step on the device, one HIP graph
+    (gpum_hip_model_warmup); the result as adaptive_warmup returns it."""
+    r = dm.warmup(init_mat, sd_mat, warmup, seed, auto_stop=auto_stop)
+    nb = r["n_batches"]
+    L = None if n_params == 1 or r["L"] is None else r["L"].reshape(-1)
+    return WarmupResult(r["init"], r["sd"], L, r["scale"], r["warmup_used"],
+                        nb, ad.warmup_batches(warmup)[:nb], r["accept_hist"],
+                        kernel_ms=r["kernel_ms"])
+
+
 def _rwm_adaptive(call, model, init_mat, sd_mat, n_iter, warmup, seed,
                   auto_stop, n_chains, surrogate=None, dm=None,
-                  langevin=False) -> Fit:
+                  langevin=False, on_device=True) -> Fit:
     """R/adaptive.R:101-254 then R/api.R:537-571; with `surrogate` the
     sampling batch is two-stage delayed acceptance; with `langevin` every
-    batch is MALA (mode 3) with the warmup's factor as preconditioner."""
+    batch is MALA (mode 3) with the warmup's factor as preconditioner.  The
+    random-walk warmup of the fused engine runs on the device
+    (device_warmup); the others orchestrate batch by batch."""
     np_ = model.n_params
-    w = adaptive_warmup(call, np_, n_chains, init_mat, sd_mat,
-                        ad.warmup_batches(warmup), seed, auto_stop,
-                        collect=bool(surrogate), langevin=langevin)
+    if (on_device and dm is not None and isinstance(call, _Caller)
+            and not surrogate and not langevin):
+        w = device_warmup(dm, np_, n_chains, init_mat, sd_mat, warmup, seed,
+                          auto_stop)
+    else:
+        w = adaptive_warmup(call, np_, n_chains, init_mat, sd_mat,
+                            ad.warmup_batches(warmup), seed, auto_stop,
+                            collect=bool(surrogate), langevin=langevin)
     sampling = max(1, n_iter - w.used)
     use_L = w.L is not None
     adaptation = {"final_proposal_sd": w.sd, "final_scales": w.scales,

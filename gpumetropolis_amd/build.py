@@ -21,12 +21,12 @@ OBJDIR = os.path.join(os.path.dirname(HERE), "_build")
 SOURCES = ["chain_kernel_inst0.hip", "chain_kernel_inst1.hip",
            "chain_kernel_inst2.hip", "chain_kernel_inst3.hip",
            "da_kernels.hip", "batch_kernels.hip", "mlp.hip", "chain_kernel.hip",
-           "eval64.hip", "noalloc.hip", "gpum_hip.cpp", "da.cpp", "batch.cpp", "jit.cpp",
+           "eval64.hip", "noalloc.hip", "warmup_kernels.hip", "gpum_hip.cpp", "da.cpp", "batch.cpp", "warmup.cpp", "jit.cpp",
            "lower.cpp", "eval64.cpp"]
 HEADERS = ["vm.h", "lower.h", "chain_kernel.h", "chain_kernels.h",
            "device_vm.h", "math64.h", "philox.h", "runtime.h", "step_common.h",
            "bf16.h", "mlp.h", "da.h", "batch.h", "chain_body.h", "jit.h", "eval64_device.h", "eval64_kernels.h",
-           "eval64.h"]
+           "eval64.h", "warmup.h"]
 # Device headers embedded into the library for the per-model kernels
 # compiled at run time (csrc/jit.cpp).
 JIT_HEADERS = ["chain_body.h", "chain_kernel.h", "device_vm.h", "math64.h",

@@ -8,6 +8,10 @@
 //                   when H.unit_chunks is 2 (programs without spill slots);
 //   E::kUnit        chunks per FP32 partial fixed at compile time (1 or 2;
 //                   0: H.unit_chunks);
+//   E::kScalarCopy  tile copies addressed on the scalar unit (buffer
+//                   descriptors) rather than with per-lane 64-bit addresses
+//                   (the interpreters, short of scalar registers, keep the
+//                   latter);
 //   E::acc_state(H, U)
 //                   what a data pass keeps in registers from the uniform
 //                   table (e.g. the uniforms a compiled program reads), read
@@ -407,6 +411,32 @@ __device__ __forceinline__ void issue_tile_dma(const float4* __restrict__ src,
   }
 }
 
+// The same copy with each lane's 64-bit address in vector registers: no
+// scalar registers per slot beyond the plan's.
+__device__ __forceinline__ void issue_tile_dma_flat(const float4* __restrict__ src,
+                                                    const TileCopy& c,
+                                                    uint32_t n4, int lane,
+                                                    uint32_t ti, float4* dst) {
+#pragma unroll
+  for (uint32_t k = 0; k < kTileIssues; ++k) {
+    const float4* base = src + (c.src[k] + (size_t)ti * n4);
+    __builtin_amdgcn_global_load_lds(
+        (const void*)(base + lane),
+        (__attribute__((address_space(3))) void*)(dst + c.lds(k)), 16, 0, 0);
+  }
+}
+
+template <class E>
+__device__ __forceinline__ void issue_tile_copy(const float4* __restrict__ src,
+                                                const TileCopy& c, uint32_t n4,
+                                                int lane, uint32_t ti,
+                                                float4* dst) {
+  if constexpr (E::kScalarCopy)
+    issue_tile_dma(src, c, n4, lane, ti, dst);
+  else
+    issue_tile_dma_flat(src, c, n4, lane, ti, dst);
+}
+
 // One full chunk of 64*OPT observations added to the FP32 partials s[j] of
 // the wave's kCw chains (FIRST: s starts here).  The affine form
 // accumulates its products straight into s; the others evaluate the chunk
@@ -573,6 +603,9 @@ template <class E, bool STREAM>
 __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
                                            ChainHot H, uint32_t chain0) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // A frozen device-side warmup: every thread reads the same word, so the
+  // whole workgroup leaves before any barrier.
+  if (Ap->stop && *Ap->stop != 0u) return;
   const int SW = (int)H.state_words;
   const int cpb = (int)H.cpb, wpc = (int)H.wpc;
   const int NU = (int)H.n_uniform_pad;
@@ -615,7 +648,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs* __restrict__ Ap,
   constexpr int OPT = E::kOpt;
   const TileCopy cp = tile_copy_plan<OPT>(H.n_cols, ld4, n4, tile4, wave_base);
   auto issue_tile = [&](uint32_t ti, float4* dst) {
-    issue_tile_dma(src, cp, n4, lane, ti, dst);
+    issue_tile_copy<E>(src, cp, n4, lane, ti, dst);
   };
 
   if constexpr (!STREAM) {
@@ -939,7 +972,7 @@ __device__ __forceinline__ void points_body(const PointsArgs* __restrict__ P,
   const TileCopy cp =
       tile_copy_plan<OPT>(H.n_cols, ld4, n4, tile4, (uint32_t)(wave * 64));
   auto issue_tile = [&](uint32_t ti, float4* dst) {
-    issue_tile_dma(src, cp, n4, lane, ti, dst);
+    issue_tile_copy<E>(src, cp, n4, lane, ti, dst);
   };
   bool loaded = false;  // resident layout: data loaded with the first item
   for (;;) {
@@ -1055,6 +1088,7 @@ struct VmEval {
   static constexpr int kCw = CW;
   static constexpr bool kPairs = NS == 0;
   static constexpr int kUnit = 0;
+  static constexpr bool kScalarCopy = false;
   using Params = VmParams;
   // The interpreter reads the uniforms as it goes.
   struct AccState {};
@@ -1108,6 +1142,7 @@ struct VmAffineEval {
   static constexpr int kCw = CW;
   static constexpr bool kPairs = true;
   static constexpr int kUnit = 0;
+  static constexpr bool kScalarCopy = false;
   using Params = VmParams;
   struct AccState {};
   static __device__ __forceinline__ void run(const ChainHot& H,

@@ -64,6 +64,9 @@ struct ChainArgs {
   uint8_t* trace_accept;
   double* trace_ll;        // data term of each proposal
   uint32_t draws_ld;       // iteration stride of `draws` (0: n_iter)
+  // Non-zero *stop (written by an earlier launch on the stream): the launch
+  // does nothing (a batch of a device-side warmup after its early freeze).
+  const uint32_t* stop;
   // tiling
   uint32_t cpb;  // chains per workgroup
   uint32_t wpc;  // waves per chain

@@ -398,7 +398,8 @@ std::string jit_evaluator_source(const Lowered& L, const uint32_t* prior,
     << "  static constexpr bool kPairs = " << (ns == 0 ? "true" : "false")
     << ";\n"
     << "  static constexpr int kUnit = " << unit_chunks_for(ns, opt, n_cols)
-    << ";\n";
+    << ";\n"
+    << "  static constexpr bool kScalarCopy = true;\n";
   if (held)
     o << "  struct AccState { float u[" << cw * nup << "]; };\n";
   else

@@ -296,6 +296,43 @@ int gpum_hip_model_run(gpum_hip_model* m, const double* init,
                        double de_noise, const double* proposal_l,
                        gpum_run_ext* ext, gpum_chain_result* out);
 
+/* ---- Adaptive warmup on the device -------------------------------------
+ * Replaces the host orchestration .am_orchestrate_warmup (R/adaptive.R:
+ * 101-254, called from R/api.R:531-536) for random-walk chains, with its
+ * batch plan (R/adaptive.R:112-118: 10 to 40 batches of about 50
+ * iterations), per-chain and pooled Welford moments, Robbins-Monro scalar
+ * per chain toward the target acceptance of R/adaptive.R:69-72, restart
+ * of every accumulator at mid-warmup, proposal L_c = scale_c * the
+ * regularised Cholesky factor of the pooled covariance (R/adaptive.R:548-
+ * 558) from the second batch on, and optional early freeze (auto_stop,
+ * R/adaptive.R:224-239).  Every batch and every between-batch step runs on
+ * the device (with use_graph, as one HIP graph); batch b draws with seed
+ * + b - 1.  init and proposal_sd: n_chains * n_params, chain-major. */
+typedef struct gpum_warmup_opts {
+  uint32_t struct_size;
+  uint32_t warmup;            /* iterations, > 0 */
+  uint32_t seed;
+  int32_t auto_stop;
+  int32_t use_graph;
+  /* host outputs, each may be NULL: */
+  double* final_init;         /* n_chains * n_params: the last draws */
+  double* final_sd;           /* n_chains * n_params */
+  double* final_scale;        /* n_chains: Robbins-Monro scalars */
+  double* final_l;            /* n_chains * n_params^2, row-major per chain;
+                                 meaningful when has_l */
+  float* accept_hist;         /* min(40, warmup) * n_chains, batch-major;
+                                 rows past n_batches are unspecified */
+  uint32_t n_batches_planned; /* out */
+  uint32_t n_batches;         /* out: batches run (fewer after a freeze) */
+  uint32_t warmup_used;       /* out: iterations run */
+  uint32_t has_l;             /* out: 1 when final_l holds a factor */
+  double kernel_ms;           /* out: device time of the warmup */
+} gpum_warmup_opts;
+
+int gpum_hip_model_warmup(gpum_hip_model* m, const double* init,
+                          uint32_t n_chains, const double* proposal_sd,
+                          gpum_warmup_opts* opts);
+
 /* Loglik batch on a resident model. */
 int gpum_hip_model_loglik(gpum_hip_model* m, const double* points,
                           uint32_t n_points, double* out);

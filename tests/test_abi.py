@@ -64,6 +64,7 @@ STRUCTS = {
     "gpum_da_ext": _native.DaExt,
     "gpum_batch_model": _native.BatchModel,
     "gpum_batch_opts": _native.BatchOpts,
+    "gpum_warmup_opts": _native.WarmupOpts,
 }
 
 
