@@ -44,8 +44,8 @@ VALU_PEAK = 256 * 64 * 2.4e9  # FP32 lane-ops/s, non-packed (one slot per op)
 # profiles/chain_kernel_gamma_1e6obs_4096chains.md (interpreter, --no-jit).
 # Reported only for the launch shape it was measured on.
 PROFILED_TRAFFIC = {
-    ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 17_180_000 * 1024 * 2.0,
-                                         "write_bytes": 70_260 * 1024 * 1.0},
+    ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 8_133_000 * 1024 * 2.0,
+                                         "write_bytes": 66_160 * 1024 * 1.0},
     ("interpreter", 4096, 1_000_000, 250): {
         "fetch_bytes": 27_762_326 * 1024 * 2.0,
         "write_bytes": 871_438 * 1024 * 1.0},

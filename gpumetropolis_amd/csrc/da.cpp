@@ -85,9 +85,12 @@ bool sur_forward(gpum_hip_surrogate* S, uint32_t rows, bool train,
 }
 
 // Backward pass of the loss over the first `valid` of `rows` rows (a
-// multiple of 64) into S->G; the summed squared error lands in S->loss.
+// multiple of 64) into S->G; the summed squared error lands in S->loss
+// (and, in a training step with `sched`, in losses[k] with the step's rate
+// set and the counters advanced).
 bool sur_backward(gpum_hip_surrogate* S, uint32_t rows, uint32_t valid,
-                  hipStream_t st) {
+                  hipStream_t st, MlpSched* sched = nullptr,
+                  float* losses = nullptr, const MlpTrainConst* tc = nullptr) {
   HIP_TRY("surrogate head backward",
           launch_mlp_head_bwd(S->s.p, S->tb.p, S->H2.p, S->A2.p, S->P.p,
                               S->D2b.p, S->part.p, (int)rows, (int)valid, st));
@@ -120,7 +123,7 @@ bool sur_backward(gpum_hip_surrogate* S, uint32_t rows, uint32_t valid,
   HIP_TRY("surrogate dW1", launch_gemm_bf16(g, st));
   HIP_TRY("surrogate gradient",
           launch_mlp_grad_finish(S->part.p, S->cs_b1.p, (int)rows, S->G.p,
-                                 S->loss.p, st));
+                                 S->loss.p, st, sched, losses, tc));
   return true;
 }
 
@@ -344,40 +347,70 @@ int gpum_hip_surrogate_train(gpum_hip_surrogate* S, const double* theta,
     set_hip_err("surrogate features", fe);
     return 1;
   }
+  // The step counters and the rate live on the device, so every step is
+  // the same sequence of launches: chunks of kGraphSteps steps are captured
+  // as one HIP graph and replayed (a step is ten small launches; launched
+  // one by one, their latency, not the matrix cores, bounds training).
+  DevBuf<MlpSched> sched;
+  {
+    MlpSched s0;
+    memset(&s0, 0, sizeof s0);
+    s0.t = S->adam_t;
+    if (!sched.upload(&s0, 1, "allocate training schedule")) return 1;
+  }
+  MlpTrainConst tc;
+  memset(&tc, 0, sizeof tc);
+  tc.steps = o.steps;
+  tc.beta1 = (float)o.beta1;
+  tc.beta2 = (float)o.beta2;
+  tc.eps = (float)o.eps;
+  tc.lr = o.lr;
+  tc.lr_final = o.lr_final;
+  const uint32_t B = o.batch;
+  const uint32_t valid = std::min(B, n);
+  auto step = [&]() -> bool {
+    return launch_mlp_gather(Xtr.p, Ttr.p, n, o.seed, 0, S->Xb.p, S->tb.p,
+                             (int)B, (int)valid, st, sched.p) == hipSuccess &&
+           sur_forward(S, B, true, st) &&
+           sur_backward(S, B, valid, st, sched.p, losses.p, &tc) &&
+           launch_adam_sched(S->P.p, S->G.p, S->Ma.p, S->Va.p, S->W1b.p,
+                             S->W2b.p, sched.p, (float)o.beta1, (float)o.beta2,
+                             (float)o.eps, st) == hipSuccess;
+  };
+  constexpr uint32_t kGraphSteps = 50;
+  const uint32_t chunk = std::min(kGraphSteps, o.steps);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ok = true;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (uint32_t k = 0; k < chunk && ok; ++k) ok = step();
+    e = hipStreamEndCapture(st, &graph);
+  }
+  if (ok && e == hipSuccess)
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  hipEventRecord(e0, st);
-  const uint32_t B = o.batch;
-  const uint32_t valid = std::min(B, n);
-  bool ok = true;
-  for (uint32_t k = 0; k < o.steps && ok; ++k) {
-    ok = launch_mlp_gather(Xtr.p, Ttr.p, n, o.seed, k, S->Xb.p, S->tb.p, (int)B,
-                           (int)valid, st) == hipSuccess;
-    ok = ok && sur_forward(S, B, true, st) && sur_backward(S, B, valid, st);
-    if (!ok) break;
-    // cosine decay from lr to lr_final over the steps
-    const double frac = o.steps > 1 ? (double)k / (double)(o.steps - 1) : 1.0;
-    const double lr = o.lr_final + 0.5 * (o.lr - o.lr_final) * (1.0 + cos(M_PI * frac));
-    S->adam_t += 1;
-    const double bc1 = 1.0 - pow(o.beta1, (double)S->adam_t);
-    const double bc2 = 1.0 - pow(o.beta2, (double)S->adam_t);
-    const float lr_t = (float)(lr * sqrt(bc2) / bc1);
-    ok = hipMemcpyAsync(losses.p + k, S->loss.p, sizeof(float),
-                        hipMemcpyDeviceToDevice, st) == hipSuccess &&
-         launch_adam(S->P.p, S->G.p, S->Ma.p, S->Va.p, S->W1b.p, S->W2b.p, lr_t,
-                     (float)o.beta1, (float)o.beta2, (float)o.eps, st) == hipSuccess;
-  }
-  hipEventRecord(e1, st);
-  hipError_t e = hipStreamSynchronize(st);
+  if (ok && e == hipSuccess) e = hipEventRecord(e0, st);
+  for (uint32_t k = 0; ok && e == hipSuccess && k + chunk <= o.steps; k += chunk)
+    e = hipGraphLaunch(exec, st);
+  for (uint32_t k = o.steps / chunk * chunk; ok && e == hipSuccess && k < o.steps;
+       ++k)
+    ok = step();
+  if (ok && e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   float ms = 0.0f;
-  hipEventElapsedTime(&ms, e0, e1);
+  if (ok && e == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
   if (!ok || e != hipSuccess) {
     if (ok) set_hip_err("surrogate training", e);
     return 1;
   }
+  S->adam_t += o.steps;
   if (kernel_ms) *kernel_ms = ms;
   if (loss_trace) {
     std::vector<float> l(o.steps);

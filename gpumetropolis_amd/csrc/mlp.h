@@ -18,12 +18,30 @@ constexpr int kOffW3 = kOffB2 + kMlpHid;           // kMlpHid
 constexpr int kOffB3 = kOffW3 + kMlpHid;           // 1
 constexpr int kMlpParams = kOffB3 + 1;
 
+// Training schedule on the device, so a captured graph of training steps
+// needs no per-step arguments: k counts this call's steps (the minibatch
+// draw and the learning-rate schedule), t Adam's steps over the network's
+// life; lr_t is the step's bias-corrected rate.
+struct MlpSched {
+  uint32_t k, t;
+  float lr_t;
+  uint32_t pad;
+};
+
+// The constants of a training call.
+struct MlpTrainConst {
+  uint32_t steps;
+  float beta1, beta2, eps;
+  double lr, lr_final;
+};
+
 // Head-backward partials of one 64-row block: [0, 256) sum g*H2 (w3),
 // [256, 512) sum D2 (b2), 512 sum g (b3), 513 squared error.
 constexpr int kPartStride = 516;
 
 // C = op(A) op(B) over bf16 operands with FP32 accumulation, M and K
-// multiples of 16, N a multiple of 64.  A(m, k) = A[m*sam + k*sak],
+// multiples of 16, N a multiple of 64 (K a multiple of 32 runs the 32-deep
+// matrix instruction).  A(m, k) = A[m*sam + k*sak],
 // B(k, n) = B[k*sbk + n*sbn].  Epilogue per element, in this order: add
 // bias[n]; multiply by silu'(dact[m*ld + n]); store the value (out, FP32;
 // out_b, bf16); store silu(value) (act, FP32; act_b, bf16); colsum gets
@@ -62,16 +80,26 @@ hipError_t launch_mlp_head_bwd(const float* s, const float* t, const float* H2,
                                hipStream_t st);
 
 // Sums the partials into the gradient vector (b1, b2, w3, b3) and `loss`
-// (the sum of squared errors over the valid rows).
+// (the sum of squared errors over the valid rows).  With `sched`: the
+// squared error also goes to losses[sched->k], and the step's rate
+// (cosine decay from lr to lr_final over the call's steps, Adam's bias
+// corrections at step t + 1) to sched->lr_t; then k and t advance.
 hipError_t launch_mlp_grad_finish(const float* part, const float* colsum_b1,
                                   int M, float* grad, float* loss,
-                                  hipStream_t st);
+                                  hipStream_t st, MlpSched* sched = nullptr,
+                                  float* losses = nullptr,
+                                  const MlpTrainConst* tc = nullptr);
 
 // One Adam step over the flat parameters; refreshes the bf16 copies of W1
 // and W2.  lr_t carries the bias corrections.
 hipError_t launch_adam(float* params, const float* grad, float* m, float* v,
                        uint16_t* W1b, uint16_t* W2b, float lr_t, float beta1,
                        float beta2, float eps, hipStream_t st);
+// The same step with the rate read from sched->lr_t.
+hipError_t launch_adam_sched(float* params, const float* grad, float* m,
+                             float* v, uint16_t* W1b, uint16_t* W2b,
+                             const MlpSched* sched, float beta1, float beta2,
+                             float eps, hipStream_t st);
 
 // Rebuild the bf16 copies of W1 and W2 from the FP32 parameters.
 hipError_t launch_mlp_mirror(const float* params, uint16_t* W1b, uint16_t* W2b,
@@ -79,9 +107,10 @@ hipError_t launch_mlp_mirror(const float* params, uint16_t* W1b, uint16_t* W2b,
 
 // Minibatch gather: row b of the batch is training row
 // philox(step, b, 0, TAG_MLP) mod n_rows; rows past `valid` are zero.
+// With `sched`, step is sched->k.
 hipError_t launch_mlp_gather(const uint16_t* Xtr, const float* Ttr,
                              uint32_t n_rows, uint32_t seed, uint32_t step,
                              uint16_t* Xb, float* tb, int M, int valid,
-                             hipStream_t st);
+                             hipStream_t st, const MlpSched* sched = nullptr);
 
 }  // namespace gpum

@@ -1,13 +1,14 @@
 // mlp.hip -- kernels of the surrogate network (16-256-256-1, SiLU).
 //
 // The two wide layers and every weight gradient are products of bf16
-// matrices on the gfx950 matrix cores (v_mfma_f32_16x16x16_bf16, FP32
-// accumulation); the 256 -> 1 head, its backward, the loss and Adam are
-// VALU kernels.  Activations that feed a matrix product are stored in bf16;
+// matrices on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16, or
+// 16x16x16 where K is 16; FP32 accumulation); the 256 -> 1 head, its
+// backward, the loss and Adam are VALU kernels.  Activations that feed a matrix product are stored in bf16;
 // pre-activations, the last hidden layer and all gradients stay FP32.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "bf16.h"
 #include "mlp.h"
@@ -16,6 +17,8 @@
 namespace gpum {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float sigmoidf_(float x) {
@@ -29,9 +32,12 @@ __device__ __forceinline__ float silu_grad(float x) {
 }
 
 // One wave per 16 x 16 output tile, four waves per workgroup along N.
-// Operand fragments of v_mfma_f32_16x16x16_bf16 (lane l, r = l % 16,
-// q = l / 16): A holds A(r, 4q + i), B holds B(4q + i, r), i = 0..3; the
-// result holds D(4q + i, r).
+// Operand fragments (lane l, r = l % 16, q = l / 16): for
+// v_mfma_f32_16x16x32_bf16 A holds A(r, 8q + i) and B holds B(8q + i, r),
+// i = 0..7; for v_mfma_f32_16x16x16_bf16 A(r, 4q + i) and B(4q + i, r),
+// i = 0..3.  A and B take the same k for each (lane, i), so the product
+// sums every k whichever order the instruction adds them in.  The result
+// holds D(4q + i, r), i = 0..3.
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -43,15 +49,50 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
   const size_t arow = (size_t)(m0 + r) * g.sam;
   const size_t bcol = (size_t)(n0 + r) * g.sbn;
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    v4s a, b;
+  if (g.K % 128 == 0) {
+    // Four 32-deep slabs per step, their loads issued together: the
+    // operands are small and cache-resident, and a wave's loop is bound by
+    // load latency, not by the matrix unit.
+    for (int k0 = 0; k0 < g.K; k0 += 128) {
+      v8s a[4], b[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const size_t k = (size_t)(k0 + 4 * q + i);
-      a[i] = (short)A[arow + k * g.sak];
-      b[i] = (short)B[k * g.sbk + bcol];
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const size_t k = (size_t)(k0 + 32 * u + 8 * q + i);
+          a[u][i] = (short)A[arow + k * g.sak];
+          b[u][i] = (short)B[k * g.sbk + bcol];
+        }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(v8bf, a[u]), __builtin_bit_cast(v8bf, b[u]), acc,
+            0, 0, 0);
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  } else if (g.K % 32 == 0) {
+    for (int k0 = 0; k0 < g.K; k0 += 32) {
+      v8s a, b;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const size_t k = (size_t)(k0 + 8 * q + i);
+        a[i] = (short)A[arow + k * g.sak];
+        b[i] = (short)B[k * g.sbk + bcol];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          __builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), acc, 0, 0,
+          0);
+    }
+  } else {
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+      v4s a, b;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const size_t k = (size_t)(k0 + 4 * q + i);
+        a[i] = (short)A[arow + k * g.sak];
+        b[i] = (short)B[k * g.sbk + bcol];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
   }
   const int n = n0 + r;
   float csum = 0.0f;
@@ -175,7 +216,8 @@ hipError_t launch_mlp_head_bwd(const float* s, const float* t, const float* H2,
 
 __global__ void __launch_bounds__(256) mlp_grad_finish_kernel(
     const float* __restrict__ part, const float* __restrict__ cs_b1, int M,
-    float* __restrict__ grad, float* __restrict__ loss) {
+    float* __restrict__ grad, float* __restrict__ loss, MlpSched* sched,
+    float* __restrict__ losses, MlpTrainConst tc) {
   const int j = threadIdx.x;
   float b1 = 0.0f;
   for (int r = 0; r < M / 16; ++r) b1 += cs_b1[(size_t)r * kMlpHid + j];
@@ -195,23 +237,43 @@ __global__ void __launch_bounds__(256) mlp_grad_finish_kernel(
     }
     grad[kOffB3] = b3;
     if (loss) *loss = se;
+    if (sched) {
+      const uint32_t k = sched->k, t = sched->t + 1u;
+      if (losses) losses[k] = se;
+      // cosine decay from lr to lr_final over the call's steps
+      const double frac =
+          tc.steps > 1 ? (double)k / (double)(tc.steps - 1) : 1.0;
+      const double lr = tc.lr_final + 0.5 * (tc.lr - tc.lr_final) *
+                                          (1.0 + cos(3.141592653589793 * frac));
+      const double bc1 = 1.0 - pow((double)tc.beta1, (double)t);
+      const double bc2 = 1.0 - pow((double)tc.beta2, (double)t);
+      sched->lr_t = (float)(lr * sqrt(bc2) / bc1);
+      sched->t = t;
+      sched->k = k + 1u;
+    }
   }
 }
 
 hipError_t launch_mlp_grad_finish(const float* part, const float* colsum_b1,
                                   int M, float* grad, float* loss,
-                                  hipStream_t st) {
+                                  hipStream_t st, MlpSched* sched,
+                                  float* losses, const MlpTrainConst* tc) {
+  MlpTrainConst c;
+  memset(&c, 0, sizeof c);
+  if (tc) c = *tc;
   hipLaunchKernelGGL(mlp_grad_finish_kernel, dim3(1), dim3(kMlpHid), 0, st, part,
-                     colsum_b1, M, grad, loss);
+                     colsum_b1, M, grad, loss, sched, losses, c);
   return hipGetLastError();
 }
 
 __global__ void __launch_bounds__(256) adam_kernel(
     float* __restrict__ P, const float* __restrict__ G, float* __restrict__ Mm,
     float* __restrict__ V, uint16_t* __restrict__ W1b,
-    uint16_t* __restrict__ W2b, float lr_t, float b1, float b2, float eps) {
+    uint16_t* __restrict__ W2b, float lr_t, float b1, float b2, float eps,
+    const MlpSched* __restrict__ sched) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= kMlpParams) return;
+  if (sched) lr_t = sched->lr_t;
   const float g = G[i];
   const float m = b1 * Mm[i] + (1.0f - b1) * g;
   const float v = b2 * V[i] + (1.0f - b2) * g * g;
@@ -227,7 +289,18 @@ hipError_t launch_adam(float* params, const float* grad, float* m, float* v,
                        uint16_t* W1b, uint16_t* W2b, float lr_t, float beta1,
                        float beta2, float eps, hipStream_t st) {
   hipLaunchKernelGGL(adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0,
-                     st, params, grad, m, v, W1b, W2b, lr_t, beta1, beta2, eps);
+                     st, params, grad, m, v, W1b, W2b, lr_t, beta1, beta2, eps,
+                     (const MlpSched*)nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam_sched(float* params, const float* grad, float* m,
+                             float* v, uint16_t* W1b, uint16_t* W2b,
+                             const MlpSched* sched, float beta1, float beta2,
+                             float eps, hipStream_t st) {
+  hipLaunchKernelGGL(adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0,
+                     st, params, grad, m, v, W1b, W2b, 0.0f, beta1, beta2, eps,
+                     sched);
   return hipGetLastError();
 }
 
@@ -249,9 +322,11 @@ hipError_t launch_mlp_mirror(const float* params, uint16_t* W1b, uint16_t* W2b,
 __global__ void __launch_bounds__(256) mlp_gather_kernel(
     const uint16_t* __restrict__ Xtr, const float* __restrict__ Ttr,
     uint32_t n_rows, uint32_t seed, uint32_t step, uint16_t* __restrict__ Xb,
-    float* __restrict__ tb, int M, int valid) {
+    float* __restrict__ tb, int M, int valid,
+    const MlpSched* __restrict__ sched) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= M) return;
+  if (sched) step = sched->k;
   if (b >= valid || n_rows == 0) {
     for (int k = 0; k < kMlpIn; ++k) Xb[(size_t)b * kMlpIn + k] = 0;
     tb[b] = 0.0f;
@@ -267,9 +342,9 @@ __global__ void __launch_bounds__(256) mlp_gather_kernel(
 hipError_t launch_mlp_gather(const uint16_t* Xtr, const float* Ttr,
                              uint32_t n_rows, uint32_t seed, uint32_t step,
                              uint16_t* Xb, float* tb, int M, int valid,
-                             hipStream_t st) {
+                             hipStream_t st, const MlpSched* sched) {
   hipLaunchKernelGGL(mlp_gather_kernel, dim3((M + 255) / 256), dim3(256), 0, st,
-                     Xtr, Ttr, n_rows, seed, step, Xb, tb, M, valid);
+                     Xtr, Ttr, n_rows, seed, step, Xb, tb, M, valid, sched);
   return hipGetLastError();
 }
 
