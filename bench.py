@@ -551,7 +551,7 @@ def batch_section(a):
 def eval_section(a):
     """FP64 batch evaluation at many parameter points: the per-observation
     log-likelihood matrix (64 posterior draws x the 1e6 gamma rows, 512 MB
-    of output in two device chunks) and the gradient of the summed data term
+    of output in one device chunk) and the gradient of the summed data term
     at a Cramer-Rao stencil (33 points = centre +- h e_k, logistic-16, 1e6
     rows).  Kernel time from HIP events; the oracle times a sample of the
     same calls on one host thread."""
@@ -574,9 +574,9 @@ def eval_section(a):
     t0 = time.perf_counter()
     oracle.loglik_pointwise(*args, pts[:2])
     cpu_rate = 2 * wg.n_obs / (time.perf_counter() - t0)
-    # Algorithmic bytes per evaluation: the FP64 output written and the FP64
-    # data read (one column).
-    bpe = 8 + 8 * wg.n_cols
+    # Algorithmic bytes per evaluation: the FP64 output written, and the
+    # FP64 data (one column) read once per row for the 64 points.
+    bpe = 8 + 8 * wg.n_cols / 64
     gbps = bpe * evals / (ms / 1e3) / 1e9
     out["pointwise"] = {
         "workload": f"gamma, {wg.n_obs} rows x 64 points, FP64",

@@ -28,7 +28,7 @@ bool g_eval_jit = true;
 
 // Device buffers bounded per launch: a pointwise chunk's output and a
 // gradient chunk's partial sums.
-constexpr size_t kPointwiseChunkBytes = size_t(256) << 20;
+constexpr size_t kPointwiseChunkBytes = size_t(1) << 30;
 constexpr size_t kGradPartialBytes = size_t(64) << 20;
 
 bool is_leaf(uint32_t op) { return op <= OP_PUSH_DATA; }
@@ -248,8 +248,12 @@ int gpum_hip_loglik_pointwise(const uint32_t* code, size_t code_len,
     a.points = S.points.p + (size_t)p0 * n_params;
     a.n_points = np;
     a.out = d_out.p;
+    // Work items: (point, slice) on the interpreter, (group of kPointGroup
+    // points, slice) on the compiled kernel.
     const uint64_t blocks =
-        (uint64_t)((n_obs + kGradSliceObs - 1) / kGradSliceObs) * np;
+        jf ? (uint64_t)((n_obs + kPointwiseSliceObs - 1) / kPointwiseSliceObs) *
+                 ((np + kPointGroup - 1) / kPointGroup)
+           : (uint64_t)((n_obs + kGradSliceObs - 1) / kGradSliceObs) * np;
     hipError_t e = hipEventRecord(ev.e0, st);
     if (e == hipSuccess)
       e = jf ? launch_compiled(jf, a, grid_for(blocks), st)

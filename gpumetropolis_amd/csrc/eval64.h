@@ -31,6 +31,11 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 // threads, 16 observations each.  It depends on nothing but the kernel, so
 // a point's gradient has the same summation order in any batch.
 constexpr uint32_t kGradSliceObs = 4096;
+// Work items of the compiled pointwise kernel: an observation's
+// parameter-free values are computed once for kPointGroup points, over
+// slices of kPointwiseSliceObs observations (values do not depend on it).
+constexpr uint32_t kPointGroup = 16;
+constexpr uint32_t kPointwiseSliceObs = 1024;
 // Longest program the gradient kernel's tape holds.
 constexpr uint32_t kMaxTapeOps = 1024;
 

@@ -38,6 +38,41 @@ __device__ __forceinline__ void pointwise64_body(const Eval64Args& a) {
   }
 }
 
+// The pointwise matrix for evaluators that split an observation's
+// parameter-free values (E::ObsVals, E::obs(a, i)) from the rest
+// (E::value_o(a, h, o, P), the same operations as E::value): a work item is
+// (group of up to kPointGroup points, slice of kPointwiseSliceObs
+// observations, four per thread: enough items to fill every CU); each observation's parameter-free values (data loads,
+// log y, ...) are computed once and serve every point of the group, whose
+// hoisted values sit in LDS.  Writes stay coalesced: for each point the
+// workgroup's threads write consecutive observations.
+
+template <class E>
+__device__ __forceinline__ void pointwise64_grouped_body(const Eval64Args& a) {
+  __shared__ typename E::Hoisted hs[kPointGroup];
+  const uint32_t ns = (a.n_obs + kPointwiseSliceObs - 1) / kPointwiseSliceObs;
+  const uint32_t ng = (a.n_points + kPointGroup - 1) / kPointGroup;
+  const uint64_t total = (uint64_t)ns * ng;
+  for (uint64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const uint32_t g = (uint32_t)(b / ns), s = (uint32_t)(b % ns);
+    const uint32_t p0 = g * kPointGroup;
+    const uint32_t gc = min(kPointGroup, a.n_points - p0);
+    __syncthreads();  // the previous item is done with hs
+    if (threadIdx.x < gc)
+      hs[threadIdx.x] = E::prepare(a, a.points + (size_t)(p0 + threadIdx.x) * a.n_params);
+    __syncthreads();
+    const uint32_t i0 = s * kPointwiseSliceObs + threadIdx.x;
+    for (uint32_t j = 0; j < kPointwiseSliceObs / 256u; ++j) {
+      const uint32_t i = i0 + j * 256u;
+      if (i >= a.n_obs) break;
+      const typename E::ObsVals o = E::obs(a, i);
+      for (uint32_t p = 0; p < gc; ++p)
+        a.out[(size_t)(p0 + p) * a.n_obs + i] =
+            E::value_o(a, hs[p], o, a.points + (size_t)(p0 + p) * a.n_params);
+    }
+  }
+}
+
 // partials[(p * n_slices + s) * (n_params + 1) + k]: the gradient (k <
 // n_params) and the value (k = n_params) summed over slice s at point p.
 // Lanes, then waves (butterfly), then the four waves in order.

@@ -59,7 +59,7 @@ const char* kKernels[JIT_NUM_KINDS] = {
     "  gpum::batch_body<gpum::JitEval>(q, a, h);\n}\n",
     "extern \"C\" __global__ void __launch_bounds__(256)\n"
     "gpum_jit_kernel(gpum::Eval64Args a) {\n"
-    "  gpum::pointwise64_body<gpum::JitEval64>(a);\n}\n",
+    "  gpum::pointwise64_grouped_body<gpum::JitEval64>(a);\n}\n",
     "extern \"C\" __global__ void __launch_bounds__(256, 2)\n"
     "gpum_jit_kernel(gpum::Eval64Args a) {\n"
     "  gpum::grad64_body<gpum::JitEval64>(a);\n}\n",
@@ -231,7 +231,9 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
   std::vector<uint32_t> lhs(n, 0), rhs(n, 0), st;
   // hoist[k]: an operation whose operands depend on no data (constants,
   // parameters, other such operations): computed once per point.
-  std::vector<char> data_dep(n, 0), hoist(n, 0);
+  // obs_only[k]: a data load or an operation that depends on no parameter:
+  // computed once per observation for a group of points (pointwise).
+  std::vector<char> data_dep(n, 0), hoist(n, 0), param_dep(n, 0), obs_only(n, 0);
   auto v = [](size_t k) { return "v" + std::to_string(k); };
   auto adj = [](size_t k) { return "a" + std::to_string(k); };
   auto is_unary = [](uint32_t op) {
@@ -242,20 +244,25 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
     const uint32_t op = code[2 * pc];
     if (op == OP_PUSH_DATA) {
       data_dep[pc] = 1;
+    } else if (op == OP_PUSH_PARAM) {
+      param_dep[pc] = 1;
     } else if (op > OP_PUSH_DATA) {
       if (is_unary(op)) {
         lhs[pc] = st.back();
         st.pop_back();
         data_dep[pc] = data_dep[lhs[pc]];
+        param_dep[pc] = param_dep[lhs[pc]];
       } else {
         rhs[pc] = st.back();
         st.pop_back();
         lhs[pc] = st.back();
         st.pop_back();
         data_dep[pc] = data_dep[lhs[pc]] | data_dep[rhs[pc]];
+        param_dep[pc] = param_dep[lhs[pc]] | param_dep[rhs[pc]];
       }
       hoist[pc] = !data_dep[pc];
     }
+    obs_only[pc] = data_dep[pc] && !param_dep[pc];
     st.push_back((uint32_t)pc);
   }
   // The value of instruction pc as an expression of its operands' names.
@@ -304,6 +311,26 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
   for (size_t pc = 0; pc < n; ++pc)
     fwd << "    const double " << v(pc) << " = "
         << (hoist[pc] ? "h." + v(pc) : expr(pc)) << ";\n";
+  // Pointwise over a group of points: the observation's parameter-free
+  // values once (obs), then the rest per point (value_o) -- the same
+  // operations as `value`, so the same bits.
+  std::ostringstream ofields, oprep, fwd_o;
+  bool any_obs = false;
+  for (size_t pc = 0; pc < n; ++pc) {
+    const uint32_t op = code[2 * pc];
+    if (op == OP_PUSH_CONST)
+      oprep << "    const double " << v(pc) << " = " << expr(pc) << ";\n";
+    if (!obs_only[pc]) continue;
+    any_obs = true;
+    ofields << " double " << v(pc) << ";";
+    oprep << "    const double " << v(pc) << " = " << expr(pc) << ";\n"
+          << "    o." << v(pc) << " = " << v(pc) << ";\n";
+  }
+  if (!any_obs) ofields << " double unused;";
+  for (size_t pc = 0; pc < n; ++pc)
+    fwd_o << "    const double " << v(pc) << " = "
+          << (hoist[pc] ? "h." + v(pc) : obs_only[pc] ? "o." + v(pc) : expr(pc))
+          << ";\n";
   const std::string root = n ? v(n - 1) : "0.0";
   // Backward: each value has one consumer, which defines its adjoint; a
   // zero adjoint propagates zeros (cpu_native.rs:218-277).
@@ -366,6 +393,15 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
     << "      double (&g)[kMaxParams]) {\n"
     << "#pragma clang fp contract(off)\n"
     << fwd.str() << bwd.str() << "    return " << root << ";\n  }\n"
+    << "  struct ObsVals {" << ofields.str() << " };\n"
+    << "  static __device__ __forceinline__ ObsVals obs(const Eval64Args& a,\n"
+    << "      uint32_t i) {\n"
+    << "#pragma clang fp contract(off)\n"
+    << "    ObsVals o;\n" << oprep.str() << "    return o;\n  }\n"
+    << "  static __device__ __forceinline__ double value_o(const Eval64Args& a,\n"
+    << "      const Hoisted& h, const ObsVals& o, const double* __restrict__ P) {\n"
+    << "#pragma clang fp contract(off)\n"
+    << fwd_o.str() << "    return " << root << ";\n  }\n"
     << "};\n}  // namespace gpum\n";
   return o.str();
 }

@@ -145,6 +145,25 @@ pub struct GpumBatchOpts {
     pub chains_per_wave: i32,
 }
 
+#[repr(C)]
+pub struct GpumWarmupOpts {
+    pub struct_size: u32,
+    pub warmup: u32,
+    pub seed: u32,
+    pub auto_stop: i32,
+    pub use_graph: i32,
+    pub final_init: *mut f64,
+    pub final_sd: *mut f64,
+    pub final_scale: *mut f64,
+    pub final_l: *mut f64,
+    pub accept_hist: *mut f32,
+    pub n_batches_planned: u32,
+    pub n_batches: u32,
+    pub warmup_used: u32,
+    pub has_l: u32,
+    pub kernel_ms: f64,
+}
+
 /// Opaque handles.
 #[repr(C)]
 pub struct GpumHipModel {
@@ -219,6 +238,10 @@ extern "C" {
     ) -> i32;
     pub fn gpum_hip_batch_run(models: *mut GpumBatchModel, n_models: u32,
                               opts: *mut GpumBatchOpts) -> i32;
+    pub fn gpum_hip_model_warmup(
+        m: *mut GpumHipModel, init: *const f64, n_chains: u32,
+        proposal_sd: *const f64, opts: *mut GpumWarmupOpts,
+    ) -> i32;
     pub fn gpum_hip_model_run_da(
         m: *mut GpumHipModel, s: *mut GpumHipSurrogate, init: *const f64,
         n_chains: u32, proposal_sd: *const f64, temperatures: *const f64,

@@ -165,7 +165,7 @@ RUST_STRUCTS = {
     "GpumChainResult": "gpum_chain_result", "GpumRunExt": "gpum_run_ext",
     "GpumModelInfo": "gpum_model_info", "GpumTrainOpts": "gpum_train_opts",
     "GpumDaExt": "gpum_da_ext", "GpumBatchModel": "gpum_batch_model",
-    "GpumBatchOpts": "gpum_batch_opts",
+    "GpumBatchOpts": "gpum_batch_opts", "GpumWarmupOpts": "gpum_warmup_opts",
 }
 C_TO_RUST = {"uint32_t": "u32", "int32_t": "i32", "double": "f64",
              "float": "f32", "uint8_t": "u8"}
@@ -211,5 +211,6 @@ def test_rust_binding_mirrors_the_header():
                 "gpum_hip_result_free", "gpum_hip_available",
                 "gpum_hip_last_error", "gpum_hip_loglik_pointwise",
                 "gpum_hip_grad_batch", "gpum_hip_run_de_sync",
-                "gpum_hip_batch_run", "gpum_hip_model_run_da"):
+                "gpum_hip_batch_run", "gpum_hip_model_run_da",
+                "gpum_hip_model_warmup"):
             assert f"pub fn {name}(" in src, name
