@@ -80,6 +80,34 @@ def test_batch_reproduces_per_model_orchestration(hip):
     assert bf.n_workgroups == 8 * 2
 
 
+def test_batch_queues_with_many_models_match_per_model(hip):
+    """More models than XCDs: several models share each XCD's queue and
+    their items interleave batch by batch, each waiting only for its own
+    model's previous batch; every model still reproduces its per-model
+    orchestration bit for bit."""
+    print("[batch] 12 models, shared queues", flush=True)
+    wls = workloads.regressions(n_models=12, seed=8, n_min=1000, n_max=40000)
+    post = [workloads.regression_posterior(w.data) for w in wls]
+    rng = np.random.default_rng(6)
+    inits = [p["mean"] + 2 * p["sd"] * rng.standard_normal((32, 10))
+             for p in post]
+    sds = [np.tile(p["sd"], (32, 1)) for p in post]
+    n_iter, warmup, de_every, seed = 160, 64, 32, 13
+    bf = gpu_metropolis_batch([w.model for w in wls], [w.data for w in wls],
+                              n_chains=32, n_iter=n_iter, warmup=warmup,
+                              de_every=de_every, init=inits, proposal_sd=sds,
+                              seed=seed, tiling=(16, 1))
+    for k, w in enumerate(wls):
+        ref = _per_model(hip, w, k, inits[k], sds[k], n_iter, warmup,
+                         de_every, seed)
+        got = bf.fits[k]
+        assert np.array_equal(got.draws, ref.draws), k
+        assert np.array_equal(got.adaptation["final_proposal_sd"],
+                              ref.adaptation["final_proposal_sd"]), k
+    # two chain groups of 16 per model, 12 models over 8 queues
+    assert bf.n_workgroups == 8 * 4
+
+
 def test_batch_regressions_match_closed_form(hip):
     """Six uneven regressions, DE-MC with 64 chains each: posterior means
     within 4 MCSE and sds within 15% of the closed form."""
