@@ -47,8 +47,8 @@ PROFILED_TRAFFIC = {
     ("compiled", 4096, 1_000_000, 250): {"fetch_bytes": 8_133_000 * 1024 * 2.0,
                                          "write_bytes": 66_160 * 1024 * 1.0},
     ("interpreter", 4096, 1_000_000, 250): {
-        "fetch_bytes": 27_762_326 * 1024 * 2.0,
-        "write_bytes": 871_438 * 1024 * 1.0},
+        "fetch_bytes": 20_390_000 * 1024 * 2.0,
+        "write_bytes": 819_300 * 1024 * 1.0},
 }
 # Instruction mix of the parallel-tempering launch (4000 iterations, 4096
 # chains, 2e5 rows; SQ_INSTS_VALU and SQ_INSTS_VALU_TRANS_F32 over 64

@@ -4,9 +4,9 @@ restatement of R/adaptive.R:101-254 driving one launch per batch).
 
 Both draw the same Philox uniforms through the same kernel, so every
 accept decision matches unless a chain meets a near-tie; the moments,
-scalars and Cholesky factors then agree to rounding (the device sums the
-pooled statistics in chain order, the host with numpy, and each side
-rounds exp/log in its own library)."""
+scalars and Cholesky factors then agree to rounding (the device merges the
+pooled statistics over groups of chains, the host with numpy, and each
+side rounds exp/log in its own library)."""
 import numpy as np
 import pytest
 
