@@ -559,7 +559,7 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   ea.depth = (uint32_t)m->L.ref_depth;
   ea.n_slots = m->slots_ll;
   const uint32_t grid = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)ns * C, (uint64_t)g_dev.n_cu * 8u));
+      1, std::min<uint64_t>(grad64_items(ns, C), (uint64_t)g_dev.n_cu * 8u));
   // The gradient compiled for the program where the process can, as the
   // batch evaluation's (same values either way).
   const hipFunction_t gjf = eval64_compiled_kernel(

@@ -333,8 +333,8 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
     a.n_points = np;
     a.out = d_part.p;
     a.n_slices = ns;
-    e = jf ? launch_compiled(jf, a, grid_for((uint64_t)ns * np), st)
-           : launch_grad64(a, grid_for((uint64_t)ns * np), st);
+    e = jf ? launch_compiled(jf, a, grid_for(grad64_items(ns, np)), st)
+           : launch_grad64(a, grid_for(grad64_items(ns, np)), st);
     if (e == hipSuccess)
       e = launch_grad64_finish(d_part.p, np, n_params, ns,
                                d_out.p + (size_t)p0 * n_params, nullptr, st);

@@ -41,9 +41,20 @@ __global__ void grad64_finish_kernel(const double* part, uint32_t n_points,
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_points * (np + 1)) return;
   const uint32_t p = t / (np + 1), k = t % (np + 1);
+  const double* q = part + (size_t)p * ns * (np + 1) + k;
+  const size_t ld = np + 1;
+  // Slice order; sixteen slices' partials are loaded before they are added,
+  // so that the loads overlap.
   double s = 0.0;
-  for (uint32_t j = 0; j < ns; ++j)
-    s += part[((size_t)p * ns + j) * (np + 1) + k];
+  uint32_t j = 0;
+  for (; j + 16 <= ns; j += 16) {
+    double v[16];
+#pragma unroll
+    for (uint32_t u = 0; u < 16; ++u) v[u] = q[(j + u) * ld];
+#pragma unroll
+    for (uint32_t u = 0; u < 16; ++u) s += v[u];
+  }
+  for (; j < ns; ++j) s += q[j * ld];
   if (k < np)
     out[(size_t)p * np + k] = s;
   else if (value)

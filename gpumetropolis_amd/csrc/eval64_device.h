@@ -19,6 +19,20 @@ __device__ __forceinline__ double data_at(const Eval64Args& a, uint32_t col,
   return a.data[(size_t)col * a.n_obs + i];
 }
 
+// a^b as the reference's powf: a * a for b = 2 and a for b = 1 (the
+// correctly rounded power, special values included), the library routine
+// for every other exponent.  The exponent of a power and of its derivative
+// (b - 1) is most often the constant 2.  The empty asm hides a constant
+// exponent from the optimiser, so that code generated per program calls
+// the routine the interpreter calls rather than a rewrite (pow(x, -1) as
+// 1 / x, say).
+__device__ __forceinline__ double pow_ref64(double a, double b) {
+  if (b == 2.0) return a * a;
+  if (b == 1.0) return a;
+  __asm__ volatile("" : "+v"(b));
+  return pow(a, b);
+}
+
 // The stack machine, cpu_native.rs:80-107.  The stack pointer follows the
 // program only, so it is the same in every lane.
 template <int MAXD>
@@ -42,7 +56,7 @@ __device__ double eval_obs64(const Eval64Args& a, const double* P,
       case OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
       case OP_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
       case OP_LGAMMA: st[sp - 1] = lgamma64(st[sp - 1]); break;
-      default: st[sp - 2] = pow(st[sp - 2], st[sp - 1]); --sp; break;
+      default: st[sp - 2] = pow_ref64(st[sp - 2], st[sp - 1]); --sp; break;
     }
   }
   return st[0];
@@ -88,7 +102,7 @@ __device__ double grad_obs64(const Eval64Args& a, const double* P, uint32_t i,
       case OP_LOG: st[sp - 1] = log(st[sp - 1]); break;
       case OP_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
       case OP_LGAMMA: st[sp - 1] = lgamma64(st[sp - 1]); break;
-      default: st[sp - 2] = pow(st[sp - 2], st[sp - 1]); --sp; break;
+      default: st[sp - 2] = pow_ref64(st[sp - 2], st[sp - 1]); --sp; break;
     }
     if (o.slot != kNoSlot) kept[o.slot] = st[sp - 1];
   }
@@ -133,7 +147,7 @@ __device__ double grad_obs64(const Eval64Args& a, const double* P, uint32_t i,
       case OP_LGAMMA: dl = ga * digamma64(val(o.l)); break;
       default: {
         const double base = val(o.l), ex = val(o.r);
-        dl = ga * ex * pow(base, ex - 1.0);
+        dl = ga * ex * pow_ref64(base, ex - 1.0);
         dr = base > 0.0 ? ga * kept[o.slot] * log(base) : 0.0;
         break;
       }

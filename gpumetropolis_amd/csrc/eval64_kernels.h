@@ -5,7 +5,11 @@
 //                          work item (generated code: parameter-only
 //                          subexpressions; the interpreter: nothing);
 //   E::value(a, h, P, i)   the program at point P, observation i;
-//   E::grad(a, h, P, i, g) the same value, d/d param k added into g[k].
+//   E::grad(a, h, P, i, g) the same value, d/d param k added into g[k];
+//   E::kFastGrad, E::grad_fast(a, h, P, i, g)
+//                          (when kFastGrad) grad without the zero-adjoint
+//                          rule: equal bits unless a sum of g ends NaN;
+//                          i < 2^29 (an index within a slice).
 // The interpreter (InterpEval64) and code generated per program share them.
 //
 // A work item is (point, slice of kGradSliceObs observations): 256 threads,
@@ -20,6 +24,25 @@
 
 namespace gpum {
 
+// A copy of x whose every 32-bit word is the first active lane's: for a
+// value the whole wave holds alike, so that the compiler keeps it in
+// scalar registers.
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T& x) {
+  if constexpr (sizeof(T) % 4 != 0) {
+    return x;
+  } else {
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &x, sizeof(T));
+#pragma unroll
+    for (uint32_t k = 0; k < sizeof(T) / 4; ++k)
+      w[k] = __builtin_amdgcn_readfirstlane(w[k]);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+  }
+}
+
 // out[p * n_obs + i] = the program at point p, observation i.
 template <class E>
 __device__ __forceinline__ void pointwise64_body(const Eval64Args& a) {
@@ -29,7 +52,7 @@ __device__ __forceinline__ void pointwise64_body(const Eval64Args& a) {
     const uint32_t p = (uint32_t)(b / ns), s = (uint32_t)(b % ns);
     const double* P = a.points + (size_t)p * a.n_params;
     double* out = a.out + (size_t)p * a.n_obs;
-    const typename E::Hoisted h = E::prepare(a, P);
+    const typename E::Hoisted h = wave_uniform(E::prepare(a, P));
     const uint32_t i0 = s * kGradSliceObs + threadIdx.x;
     for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
       const uint32_t i = i0 + j * 256u;
@@ -80,20 +103,49 @@ template <class E>
 __device__ __forceinline__ void grad64_body(const Eval64Args& a) {
   __shared__ double red[4][kMaxParams + 1];
   const uint32_t ns = a.n_slices;
-  const uint64_t total = (uint64_t)ns * a.n_points;
+  const uint64_t total = grad64_items(ns, a.n_points);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (uint64_t b = blockIdx.x; b < total; b += gridDim.x) {
-    const uint32_t p = (uint32_t)(b / ns), s = (uint32_t)(b % ns);
+    const uint64_t r = b / kEvalXcds;
+    const uint32_t s = (uint32_t)(b % kEvalXcds) +
+                       kEvalXcds * (uint32_t)(r / a.n_points);
+    const uint32_t p = (uint32_t)(r % a.n_points);
+    if (s >= ns) continue;  // uniform over the workgroup
     const double* P = a.points + (size_t)p * a.n_params;
     double g[kMaxParams];
 #pragma unroll
     for (int k = 0; k < kMaxParams; ++k) g[k] = 0.0;
     double val = 0.0;
-    const typename E::Hoisted h = E::prepare(a, P);
+    const typename E::Hoisted h = wave_uniform(E::prepare(a, P));
     const uint32_t i0 = s * kGradSliceObs + threadIdx.x;
-    for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
-      const uint32_t i = i0 + j * 256u;
-      if (i < a.n_obs) val += E::grad(a, h, P, i, g);
+    bool exact = true;
+    if constexpr (E::kFastGrad) {
+      // Without the zero-adjoint rule: the same bits unless a sum ends
+      // NaN, and then the whole item again with the rule.  The data are
+      // addressed from the slice's first row, so that a load is a scalar
+      // column base plus a 32-bit offset.
+      Eval64Args as = a;
+      as.data = a.data + (size_t)s * kGradSliceObs;
+      const uint32_t left = a.n_obs - s * kGradSliceObs;
+      for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
+        const uint32_t li = threadIdx.x + j * 256u;
+        if (li < left) val += E::grad_fast(as, h, P, li, g);
+      }
+      bool nan = false;
+#pragma unroll
+      for (int k = 0; k < kMaxParams; ++k) nan = nan || isnan(g[k]);
+      exact = __syncthreads_or(nan) != 0;
+      if (exact) {
+#pragma unroll
+        for (int k = 0; k < kMaxParams; ++k) g[k] = 0.0;
+        val = 0.0;
+      }
+    }
+    if (exact) {
+      for (uint32_t j = 0; j < kGradSliceObs / 256u; ++j) {
+        const uint32_t i = i0 + j * 256u;
+        if (i < a.n_obs) val += E::grad(a, h, P, i, g);
+      }
     }
 #pragma unroll
     for (int k = 0; k <= kMaxParams; ++k) {
@@ -117,6 +169,7 @@ __device__ __forceinline__ void grad64_body(const Eval64Args& a) {
 // gradient keeps.
 template <int MAXD, int MAXS>
 struct InterpEval64 {
+  static constexpr bool kFastGrad = false;
   struct Hoisted {};
   static __device__ __forceinline__ Hoisted prepare(const Eval64Args&,
                                                     const double*) {

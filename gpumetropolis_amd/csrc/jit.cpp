@@ -60,10 +60,19 @@ const char* kKernels[JIT_NUM_KINDS] = {
     "extern \"C\" __global__ void __launch_bounds__(256)\n"
     "gpum_jit_kernel(gpum::Eval64Args a) {\n"
     "  gpum::pointwise64_grouped_body<gpum::JitEval64>(a);\n}\n",
-    "extern \"C\" __global__ void __launch_bounds__(256, 2)\n"
+    "extern \"C\" __global__ void __launch_bounds__(256, 4)\n"
     "gpum_jit_kernel(gpum::Eval64Args a) {\n"
     "  gpum::grad64_body<gpum::JitEval64>(a);\n}\n",
 };
+
+// The gradient kernel above runs four waves per SIMD (at most 128 registers
+// a lane); a program whose kernel then keeps more than kGradSpillBytes a
+// lane in scratch memory is compiled again for two waves per SIMD.
+const char* kGrad64TwoWaves =
+    "extern \"C\" __global__ void __launch_bounds__(256, 2)\n"
+    "gpum_jit_kernel(gpum::Eval64Args a) {\n"
+    "  gpum::grad64_body<gpum::JitEval64>(a);\n}\n";
+constexpr int kGradSpillBytes = 64;
 
 struct Entry {
   hipModule_t mod = nullptr;
@@ -246,6 +255,7 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
       data_dep[pc] = 1;
     } else if (op == OP_PUSH_PARAM) {
       param_dep[pc] = 1;
+      hoist[pc] = 1;  // read once per point, held with the hoisted values
     } else if (op > OP_PUSH_DATA) {
       if (is_unary(op)) {
         lhs[pc] = st.back();
@@ -283,7 +293,7 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
       case OP_LOG: return "log(" + l + ")";
       case OP_SQRT: return "sqrt(" + l + ")";
       case OP_LGAMMA: return "lgamma64(" + l + ")";
-      default: return "pow(" + l + ", opaque64(" + r + "))";
+      default: return "pow_ref64(" + l + ", " + r + ")";
     }
   };
   // Once per point: the hoisted values and the parameter-only factors of
@@ -303,14 +313,28 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
       prep << "    h.dg" << pc << " = digamma64(" << l << ");\n";
     } else if (op == OP_POW) {
       fields << " double pw" << pc << "; double lg" << pc << ";";
-      prep << "    h.pw" << pc << " = pow(" << l << ", opaque64(" << r
-           << " - 1.0));\n    h.lg" << pc << " = log(" << l << ");\n";
+      prep << "    h.pw" << pc << " = pow_ref64(" << l << ", " << r
+           << " - 1.0);\n    h.lg" << pc << " = log(" << l << ");\n";
     }
   }
-  std::ostringstream fwd;
+  std::ostringstream fwd, fwd_slice;
   for (size_t pc = 0; pc < n; ++pc)
     fwd << "    const double " << v(pc) << " = "
         << (hoist[pc] ? "h." + v(pc) : expr(pc)) << ";\n";
+  // The same forward pass for an observation index below 2^29 (grad_fast
+  // gets the index within a slice): a column's base plus a 32-bit byte
+  // offset, which the load instruction takes as it is.
+  for (size_t pc = 0; pc < n; ++pc) {
+    fwd_slice << "    const double " << v(pc) << " = ";
+    if (hoist[pc])
+      fwd_slice << "h." << v(pc);
+    else if (code[2 * pc] == OP_PUSH_DATA)
+      fwd_slice << "*(const double*)((const char*)(a.data + (size_t)"
+                << code[2 * pc + 1] << " * a.n_obs) + (i << 3))";
+    else
+      fwd_slice << expr(pc);
+    fwd_slice << ";\n";
+  }
   // Pointwise over a group of points: the observation's parameter-free
   // values once (obs), then the rest per point (value_o) -- the same
   // operations as `value`, so the same bits.
@@ -333,9 +357,19 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
           << ";\n";
   const std::string root = n ? v(n - 1) : "0.0";
   // Backward: each value has one consumer, which defines its adjoint; a
-  // zero adjoint propagates zeros (cpu_native.rs:218-277).
-  std::ostringstream bwd;
-  if (n) bwd << "    const double " << adj(n - 1) << " = 1.0;\n";
+  // zero adjoint propagates zeros (cpu_native.rs:218-277).  `bwd` applies
+  // that rule with a select per adjoint.  `bwd_fast` leaves it out: every
+  // adjoint is linear in its consumer's, so where the rule would give +0
+  // the plain expression gives +0, -0 or NaN, all through the subtree; a
+  // signed zero added to a gradient sum (which starts at +0) leaves its
+  // bits unchanged, so the fast form equals the exact one bit for bit
+  // unless a gradient sum ends NaN, and then the caller repeats the work
+  // with `bwd`.
+  std::ostringstream bwd, bwd_fast;
+  if (n) {
+    bwd << "    const double " << adj(n - 1) << " = 1.0;\n";
+    bwd_fast << "    const double " << adj(n - 1) << " = 1.0;\n";
+  }
   for (size_t k = n; k-- > 0;) {
     const uint32_t op = code[2 * k], arg = code[2 * k + 1];
     const std::string g = adj(k), live = "(" + adj(k) + " != 0.0)";
@@ -344,12 +378,14 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
     auto def = [&](uint32_t t, const std::string& e) {
       bwd << "    const double " << adj(t) << " = " << live << " ? (" << e
           << ") : 0.0;\n";
+      bwd_fast << "    const double " << adj(t) << " = " << e << ";\n";
     };
     switch (op) {
       case OP_PUSH_CONST:
       case OP_PUSH_DATA: break;
       case OP_PUSH_PARAM:
         bwd << "    if " << live << " g[" << arg << "] += " << g << ";\n";
+        bwd_fast << "    g[" << arg << "] += " << g << ";\n";
         break;
       case OP_ADD: def(lhs[k], g); def(rhs[k], g); break;
       case OP_SUB: def(lhs[k], g); def(rhs[k], "-" + g); break;
@@ -367,7 +403,7 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
         break;
       default: {
         const std::string pw =
-            hoist[k] ? "h.pw" + kk : "pow(" + l + ", opaque64(" + r + " - 1.0))";
+            hoist[k] ? "h.pw" + kk : "pow_ref64(" + l + ", " + r + " - 1.0)";
         const std::string lg = hoist[k] ? "h.lg" + kk : "log(" + l + ")";
         def(lhs[k], g + " * " + r + " * " + pw);
         def(rhs[k], l + " > 0.0 ? " + g + " * " + me + " * " + lg + " : 0.0");
@@ -393,6 +429,12 @@ std::string jit_eval64_source(const uint32_t* code, size_t n_words,
     << "      double (&g)[kMaxParams]) {\n"
     << "#pragma clang fp contract(off)\n"
     << fwd.str() << bwd.str() << "    return " << root << ";\n  }\n"
+    << "  static constexpr bool kFastGrad = true;\n"
+    << "  static __device__ __forceinline__ double grad_fast(const Eval64Args& a,\n"
+    << "      const Hoisted& h, const double* __restrict__ P, uint32_t i,\n"
+    << "      double (&g)[kMaxParams]) {\n"
+    << "#pragma clang fp contract(off)\n"
+    << fwd_slice.str() << bwd_fast.str() << "    return " << root << ";\n  }\n"
     << "  struct ObsVals {" << ofields.str() << " };\n"
     << "  static __device__ __forceinline__ ObsVals obs(const Eval64Args& a,\n"
     << "      uint32_t i) {\n"
@@ -527,6 +569,20 @@ hipFunction_t jit_kernel(const std::string& evaluator, JitKind kind,
   if (!runtime_matches(err)) return nullptr;
   Entry e;
   if (!compile(src, &e, err, compile_ms)) return nullptr;
+  int spill = 0;
+  if (kind == JIT_GRAD64 &&
+      hipFuncGetAttribute(&spill, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, e.fn) ==
+          hipSuccess &&
+      spill > kGradSpillBytes) {
+    Entry e2;
+    double ms2 = 0.0;
+    std::string err2;
+    if (compile(evaluator + kGrad64TwoWaves, &e2, &err2, &ms2)) {
+      hipModuleUnload(e.mod);
+      e = e2;
+      if (compile_ms) *compile_ms += ms2;
+    }
+  }
   g_cache[src] = e;
   return e.fn;
 }

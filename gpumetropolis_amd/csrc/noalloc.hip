@@ -174,7 +174,7 @@ int gpum_hip_grad_batch_noalloc(const uint32_t* code, size_t code_len,
     a.points = d_points + (size_t)p0 * n_params;
     a.n_points = np;
     const uint32_t grid =
-        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ns * np, cap));
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(grad64_items(ns, np), cap));
     e = launch_grad64(a, grid, st);
     if (e == hipSuccess)
       e = launch_grad64_finish(d_part, np, n_params, ns,

@@ -139,6 +139,35 @@ def test_opcodes_and_ragged_sizes(hip, orc, eval_mode, form, n_obs):
     assert err.size == 0 or err.max() <= GRAD_TOL, (formula, err.max())
 
 
+def test_zero_adjoint_rule(hip, orc):
+    """A zero adjoint stops at its operation (cpu_native.rs:218-220): in
+    exp(-exp(a x)) with exp(a x) = inf the value is 0 and the reference's
+    gradient term is 0, where 0 * inf would give NaN.  Only the second
+    4096-observation slice holds such rows, so the compiled gradient takes
+    its exact path there alone; it must equal the interpreter bit for bit
+    and the oracle within the FP64 tolerance."""
+    m = dsl.gpum_model("~ exp(-exp(a * x)) + b * y", ["a", "b"], ["y", "x"])
+    n_obs = 3 * 4096 + 5
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.1, 1.0, n_obs)
+    x[4096 + rng.choice(4096, 7, replace=False)] = 900.0
+    data = np.column_stack([rng.normal(0.3, 1.0, n_obs), x])
+    pts = np.array([[1.0, 0.5], [1.2, -0.3], [0.9, 0.1]])
+    print("[grad] zero-adjoint rule", flush=True)
+    args = (m.loglik.code, m.loglik.consts, 2, data, 2, n_obs, pts)
+    out = {}
+    for jit in (True, False):
+        hip.set_eval_jit(jit)
+        out[jit] = hip.grad_batch(*args)
+        assert hip.last_eval_compiled() == jit
+    hip.set_eval_jit(True)
+    assert np.all(np.isfinite(out[True]))
+    assert np.array_equal(out[True], out[False])
+    ref = orc.grad_batch(*args)
+    scale_k = _grad_scale(orc, m.loglik.code, m.loglik.consts, 2, data, pts)
+    assert np.max(np.abs(out[True] - ref) / scale_k) <= GRAD_TOL
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_compiled_equals_interpreter(hip, name):
     """The kernels compiled per program run the interpreter's operations in
