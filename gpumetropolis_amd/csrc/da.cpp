@@ -580,7 +580,8 @@ static int run_mala_locked(gpum_hip_model* m, const double* init, uint32_t C,
   const hipFunction_t pjf = eval64_compiled_kernel(
       m->prior_ref.data(), m->prior_ref.size(), m->prior_consts_h.data(),
       JIT_GRAD64);
-  const uint32_t pgrid = std::min<uint32_t>(C, (uint32_t)g_dev.n_cu * 8u);
+  const uint32_t pgrid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(grad64_items(1, C), (uint64_t)g_dev.n_cu * 8u));
   auto prior_term = [&](const double* points) -> hipError_t {
     if (n_prior_ops == 0) {
       hipError_t r = hipMemsetAsync(vpr.p, 0, C * sizeof(double), st);

@@ -276,60 +276,85 @@ __global__ void __launch_bounds__(256) da_draws_kernel(const float* __restrict__
 // One thread per chain; n_params <= 16, so each chain's vectors are short.
 // Every expression is rounded as the reference's (no fused multiply-add).
 
+// The Langevin kernels run one wave per chain, lane k holding parameter k
+// (k < d); every sum keeps the order in which the reference's loops add
+// its terms.
+
+// Lane j's x in every lane (j the same in every lane).
+__device__ __forceinline__ double from_lane(double x, uint32_t j) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, (int)j);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), (int)j);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // The preconditioned, truncated drift f * 0.5 * M g with M = Ls Ls' (Ls
 // the chain's Cholesky factor, or the diagonal proposal_sd), the whitened
-// drift norm capped at 2 sqrt(d) (cpu_native.rs:618-651).
-__device__ void mala_drift(const DaArgs* __restrict__ A, uint32_t c,
-                           const double* g, double* out) {
+// drift norm capped at 2 sqrt(d) (cpu_native.rs:618-651): lane k's
+// component from the gradient's component g in lane k.
+__device__ double mala_drift(const DaArgs* __restrict__ A, uint32_t c,
+                             double g, uint32_t lane) {
 #pragma clang fp contract(off)
   const uint32_t np = A->n_params;
+  const bool mine = lane < np;
   const double dmax = 2.0 * sqrt((double)np);
   const double* sd = A->proposal_sd + (size_t)c * np;
   const double* L = A->proposal_l ? A->proposal_l + (size_t)c * np * np : nullptr;
-  double tmp[kMaxParams];
-  for (uint32_t j = 0; j < np; ++j) {
-    if (!L) {
-      tmp[j] = 0.5 * sd[j] * g[j];
-    } else {
-      double acc = 0.0;
-      for (uint32_t k = j; k < np; ++k) acc += L[k * np + j] * g[k];
-      tmp[j] = 0.5 * acc;
+  // tmp_j = 0.5 * (Ls' g)_j, k ascending from j.
+  double tmp = 0.0;
+  if (!L) {
+    if (mine) tmp = 0.5 * sd[lane] * g;
+  } else {
+    double acc = 0.0;
+    for (uint32_t k = 0; k < np; ++k) {
+      const double gk = from_lane(g, k);
+      if (mine && k >= lane) acc += L[k * np + lane] * gk;
     }
+    tmp = 0.5 * acc;
   }
   double nrm = 0.0;
-  for (uint32_t j = 0; j < np; ++j) nrm += tmp[j] * tmp[j];
+  for (uint32_t j = 0; j < np; ++j) {
+    const double tj = from_lane(tmp, j);
+    nrm += tj * tj;
+  }
   nrm = sqrt(nrm);
   const double f = nrm > dmax ? dmax / nrm : 1.0;
-  for (uint32_t k = 0; k < np; ++k) {
-    if (!L) {
-      out[k] = f * sd[k] * tmp[k];
-    } else {
-      double acc = 0.0;
-      for (uint32_t j = 0; j <= k; ++j) acc += L[k * np + j] * tmp[j];
-      out[k] = f * acc;
-    }
+  if (!mine) return 0.0;
+  if (!L) return f * sd[lane] * tmp;
+  double acc = 0.0;
+  for (uint32_t j = 0; j < np; ++j) {
+    const double tj = from_lane(tmp, j);
+    if (j <= lane) acc += L[lane * np + j] * tj;
   }
+  return f * acc;
 }
 
-// -0.5 |Ls^{-1} (v - m)|^2 by forward substitution (cpu_native.rs:655-675).
+// -0.5 |Ls^{-1} (v - m)|^2 (cpu_native.rs:655-675) from lane k's
+// components v and m.  The forward substitution runs by columns: once
+// tmp_j is known every later row subtracts its term, so each row still
+// subtracts in ascending j.
 __device__ double mala_log_q(const DaArgs* __restrict__ A, uint32_t c,
-                             const double* v, const double* m) {
+                             double v, double m, uint32_t lane) {
 #pragma clang fp contract(off)
   const uint32_t np = A->n_params;
+  const bool mine = lane < np;
   const double* sd = A->proposal_sd + (size_t)c * np;
   const double* L = A->proposal_l ? A->proposal_l + (size_t)c * np * np : nullptr;
-  double tmp[kMaxParams];
   double q = 0.0;
-  for (uint32_t k = 0; k < np; ++k) {
-    if (!L) {
-      const double r = (v[k] - m[k]) / sd[k];
-      q -= 0.5 * r * r;
-    } else {
-      double acc = v[k] - m[k];
-      for (uint32_t j = 0; j < k; ++j) acc -= L[k * np + j] * tmp[j];
-      tmp[k] = acc / L[k * np + k];
-      q -= 0.5 * tmp[k] * tmp[k];
+  if (!L) {
+    const double r = mine ? (v - m) / sd[lane] : 0.0;
+    for (uint32_t k = 0; k < np; ++k) {
+      const double rk = from_lane(r, k);
+      q -= 0.5 * rk * rk;
     }
+    return q;
+  }
+  double acc = v - m;
+  for (uint32_t k = 0; k < np; ++k) {
+    const double tk_own = mine && lane == k ? acc / L[k * np + k] : 0.0;
+    const double tk = from_lane(tk_own, k);
+    q -= 0.5 * tk * tk;
+    if (mine && lane > k) acc -= L[lane * np + k] * tk;
   }
   return q;
 }
@@ -359,73 +384,84 @@ __global__ void mala_init_kernel(const DaArgs* __restrict__ A) {
   for (uint32_t k = 0; k < np; ++k) A->gx[(size_t)c * np + k] = g[k];
 }
 
-__global__ void mala_propose_kernel(const DaArgs* __restrict__ A, uint32_t t) {
+constexpr int kMalaWaves = 4;  // chains per 256-thread workgroup
+
+__global__ void __launch_bounds__(256) mala_propose_kernel(
+    const DaArgs* __restrict__ A, uint32_t t) {
 #pragma clang fp contract(off)
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= A->n_chains) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t c = blockIdx.x * kMalaWaves + (threadIdx.x >> 6);
+  if (c >= A->n_chains) return;  // the whole wave
   const uint32_t np = A->n_params;
+  const bool mine = lane < np;
   const double two_pi = 6.283185307179586;
-  const double* th = A->theta + (size_t)c * np;
-  double mx[kMaxParams], z[kMaxParams];
-  mala_drift(A, c, A->gx + (size_t)c * np, mx);
+  const double th = mine ? A->theta[(size_t)c * np + lane] : 0.0;
+  const double gx = mine ? A->gx[(size_t)c * np + lane] : 0.0;
+  const double mx = mala_drift(A, c, gx, lane) + th;
   StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
-  for (uint32_t j = 0; j < np; ++j) {
-    mx[j] += th[j];
-    const double u1 = us.get(2 * j), u2 = us.get(2 * j + 1);
-    z[j] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+  double z = 0.0;
+  if (mine) {
+    const double u1 = us.get(2 * lane), u2 = us.get(2 * lane + 1);
+    z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
   }
   const double* sd = A->proposal_sd + (size_t)c * np;
   const double* L = A->proposal_l ? A->proposal_l + (size_t)c * np * np : nullptr;
-  for (uint32_t k = 0; k < np; ++k) {
-    double p;
-    if (!L) {
-      p = mx[k] + sd[k] * z[k];
-    } else {
-      p = mx[k];
-      for (uint32_t j = 0; j <= k; ++j) p += L[k * np + j] * z[j];
+  double p = mx;
+  if (!L) {
+    if (mine) p = mx + sd[lane] * z;
+  } else {
+    for (uint32_t j = 0; j < np; ++j) {
+      const double zj = from_lane(z, j);
+      if (mine && j <= lane) p += L[lane * np + j] * zj;
     }
-    A->prop[(size_t)c * np + k] = p;
-    A->mean_x[(size_t)c * np + k] = mx[k];
+  }
+  if (mine) {
+    A->prop[(size_t)c * np + lane] = p;
+    A->mean_x[(size_t)c * np + lane] = mx;
   }
 }
 
-__global__ void mala_accept_kernel(const DaArgs* __restrict__ A, uint32_t t) {
+__global__ void __launch_bounds__(256) mala_accept_kernel(
+    const DaArgs* __restrict__ A, uint32_t t) {
 #pragma clang fp contract(off)
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t c = blockIdx.x * kMalaWaves + (threadIdx.x >> 6);
   const uint32_t C = A->n_chains;
-  if (c >= C) return;
+  if (c >= C) return;  // the whole wave
   const uint32_t np = A->n_params;
-  const double* pr = A->prop + (size_t)c * np;
-  double* th = A->theta + (size_t)c * np;
-  double g[kMaxParams], my[kMaxParams];
-  mala_gradient(A, c, g);
+  const bool mine = lane < np;
+  const size_t ck = (size_t)c * np + lane;
+  const double pr = mine ? A->prop[ck] : 0.0;
+  const double th = mine ? A->theta[ck] : 0.0;
+  const double mxo = mine ? A->mean_x[ck] : 0.0;
+  const double g = mine ? A->gy[ck] + A->g_prior[ck] : 0.0;
   const double ll = A->ll_prop[c];
   const double plp = ll + A->v_prior[c];
-  mala_drift(A, c, g, my);
-  for (uint32_t j = 0; j < np; ++j) my[j] += pr[j];
-  const double lq = mala_log_q(A, c, th, my) -
-                    mala_log_q(A, c, pr, A->mean_x + (size_t)c * np);
+  const double my = mala_drift(A, c, g, lane) + pr;
+  const double lq = mala_log_q(A, c, th, my, lane) - mala_log_q(A, c, pr, mxo, lane);
   StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
   const double u = us.get(2 * np);
   const bool acc_ok =
       isfinite(plp) && log(u) < (plp - A->cur[c]) / A->temperatures[c] + lq;
-  if (acc_ok) {
-    for (uint32_t j = 0; j < np; ++j) {
-      th[j] = pr[j];
-      A->gx[(size_t)c * np + j] = g[j];
+  const double now = acc_ok ? pr : th;
+  if (mine) {
+    if (acc_ok) {
+      A->theta[ck] = pr;
+      A->gx[ck] = g;
     }
-    A->cur[c] = plp;
-    A->accepts[c] += 1.0;
+    if (A->trace_prop) A->trace_prop[((size_t)t * C + c) * np + lane] = pr;
+    A->draws_it[((size_t)t * np + lane) * C + c] = (float)now;
   }
-  A->passes[c] += 1.0;
-  if (A->trace_plp) A->trace_plp[(size_t)t * C + c] = plp;
-  if (A->trace_ll) A->trace_ll[(size_t)t * C + c] = ll;
-  if (A->trace_prop)
-    for (uint32_t j = 0; j < np; ++j)
-      A->trace_prop[((size_t)t * C + c) * np + j] = pr[j];
-  if (A->trace_stage) A->trace_stage[(size_t)t * C + c] = acc_ok ? 2 : 1;
-  for (uint32_t j = 0; j < np; ++j)
-    A->draws_it[((size_t)t * np + j) * C + c] = (float)th[j];
+  if (lane == 0) {
+    if (acc_ok) {
+      A->cur[c] = plp;
+      A->accepts[c] += 1.0;
+    }
+    A->passes[c] += 1.0;
+    if (A->trace_plp) A->trace_plp[(size_t)t * C + c] = plp;
+    if (A->trace_ll) A->trace_ll[(size_t)t * C + c] = ll;
+    if (A->trace_stage) A->trace_stage[(size_t)t * C + c] = acc_ok ? 2 : 1;
+  }
 }
 
 static dim3 chain_grid(uint32_t n) { return dim3((n + 255) / 256); }
@@ -436,12 +472,14 @@ hipError_t launch_mala_init(const DaArgs* A, uint32_t n, hipStream_t s) {
 }
 hipError_t launch_mala_propose(const DaArgs* A, uint32_t n, uint32_t t,
                                hipStream_t s) {
-  hipLaunchKernelGGL(mala_propose_kernel, chain_grid(n), dim3(256), 0, s, A, t);
+  hipLaunchKernelGGL(mala_propose_kernel, dim3((n + kMalaWaves - 1) / kMalaWaves),
+                     dim3(256), 0, s, A, t);
   return hipGetLastError();
 }
 hipError_t launch_mala_accept(const DaArgs* A, uint32_t n, uint32_t t,
                               hipStream_t s) {
-  hipLaunchKernelGGL(mala_accept_kernel, chain_grid(n), dim3(256), 0, s, A, t);
+  hipLaunchKernelGGL(mala_accept_kernel, dim3((n + kMalaWaves - 1) / kMalaWaves),
+                     dim3(256), 0, s, A, t);
   return hipGetLastError();
 }
 

@@ -39,18 +39,19 @@ constexpr uint32_t kPointwiseSliceObs = 1024;
 // Longest program the gradient kernel's tape holds.
 constexpr uint32_t kMaxTapeOps = 1024;
 
-// Gradient work items (point, slice of kGradSliceObs observations) are
-// numbered so that the points of a slice share its rows in one XCD's L2:
-// slice s belongs to XCD s mod 8, and an XCD takes one of its slices at
-// every point before its next slice.  Item b runs on XCD b mod 8 (the
-// hardware deals workgroups to the XCDs round-robin, and the grid is a
-// multiple of 8 or covers every item); items whose slice is past the last
-// are empty.
+// Gradient work items (point, slice of kGradSliceObs observations), T =
+// n_slices x n_points of them, taken slice-major (slice, then point) and
+// cut into 8 runs of ceil(T / 8) items (the last run shorter), one per
+// XCD: the points of a slice run together and share its rows in one XCD's
+// L2 (a slice spans at most two XCDs), and the XCDs share the work
+// evenly.  Item b is the (b / 8)-th of run b mod 8 and runs on XCD b mod 8
+// (the hardware deals workgroups to the XCDs round-robin, and the grid is
+// a multiple of 8 or covers every item); items past the last are empty.
 constexpr uint32_t kEvalXcds = 8;
 __host__ __device__ inline uint64_t grad64_items(uint32_t n_slices,
                                                  uint32_t n_points) {
-  return (uint64_t)kEvalXcds * ((n_slices + kEvalXcds - 1) / kEvalXcds) *
-         n_points;
+  const uint64_t t = (uint64_t)n_slices * n_points;
+  return kEvalXcds * ((t + kEvalXcds - 1) / kEvalXcds);
 }
 
 struct Eval64Args {

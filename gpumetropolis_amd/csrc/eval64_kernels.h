@@ -103,14 +103,18 @@ template <class E>
 __device__ __forceinline__ void grad64_body(const Eval64Args& a) {
   __shared__ double red[4][kMaxParams + 1];
   const uint32_t ns = a.n_slices;
-  const uint64_t total = grad64_items(ns, a.n_points);
+  // The items in slice-major runs, one run per XCD (grad64_items), the
+  // mapping in 64 bits (a 32-bit item counter made the compiler spill the
+  // observation loop's registers).
+  const uint64_t t = (uint64_t)ns * a.n_points;
+  const uint64_t run = (t + kEvalXcds - 1) / kEvalXcds;
+  const uint64_t total = kEvalXcds * run;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (uint64_t b = blockIdx.x; b < total; b += gridDim.x) {
-    const uint64_t r = b / kEvalXcds;
-    const uint32_t s = (uint32_t)(b % kEvalXcds) +
-                       kEvalXcds * (uint32_t)(r / a.n_points);
-    const uint32_t p = (uint32_t)(r % a.n_points);
-    if (s >= ns) continue;  // uniform over the workgroup
+    const uint64_t idx = (b % kEvalXcds) * run + b / kEvalXcds;
+    if (idx >= t) continue;  // an empty item: uniform over the workgroup
+    const uint32_t s = (uint32_t)(idx / a.n_points);
+    const uint32_t p = (uint32_t)(idx % a.n_points);
     const double* P = a.points + (size_t)p * a.n_params;
     double g[kMaxParams];
 #pragma unroll
