@@ -953,7 +953,7 @@ static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
       hipEventDestroy(e1);
       return 1;
     }
-    chk(launch_da_stage1(d_args.p, it, st));
+    if (S) chk(launch_da_stage1(d_args.p, it, st));
     chk(points(d_pa.p));
     chk(launch_da_stage2(d_args.p, C, it, st));
   }

@@ -8,7 +8,9 @@
 //   [network forward over all proposals]
 //   stage1    one workgroup: the surrogate's Metropolis test
 //             log u1 < (s(prop) - s(theta)) / T, and an ordered compaction of
-//             the surviving proposals into the exact stage's slot list;
+//             the surviving proposals into the exact stage's slot list
+//             (without a surrogate the propose kernel writes the list, every
+//             chain in order, and this launch is skipped);
 //   points    the exact data term of the listed proposals (chain kernel
 //             file), split over observation slices;
 //   stage2    one thread per chain: the exact test, corrected by the
@@ -151,9 +153,18 @@ __global__ void __launch_bounds__(256) da_propose_kernel(const DaArgs* __restric
   }
   __syncthreads();
   if (!live) return;
-  if (lane == 0)
+  if (lane == 0) {
     A->prop_prior[c] = eval_params64(A->prior, A->n_prior, A->prior_consts,
                                      prop_sh[w], nullptr);
+    if (!A->use_surrogate) {
+      // Every proposal goes to the exact stage, in chain order: the slot
+      // list da_stage1_kernel would write, written here instead.
+      A->list[c] = c;
+      A->pos[c] = (int32_t)c;
+      A->passes[c] += 1.0;
+      if (c == 0) *A->count = A->n_chains;
+    }
+  }
   if (A->use_surrogate && lane < kMlpIn) {
     const uint32_t n_in = A->n_in;
     double x = 0.0;
@@ -226,7 +237,17 @@ __global__ void da_stage2_kernel(const DaArgs* __restrict__ A, uint32_t t) {
   if (p >= 0) {
     ll = 0.0;
     const uint32_t ns = *A->n_slices;
-    for (uint32_t s = 0; s < ns; ++s) ll += A->partial[(size_t)s * C + p];
+    const double* part = A->partial + p;
+    // Slice order; eight slices' partials are loaded before they are added.
+    uint32_t s = 0;
+    for (; s + 8 <= ns; s += 8) {
+      double v[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) v[k] = part[(size_t)(s + k) * C];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) ll += v[k];
+    }
+    for (; s < ns; ++s) ll += part[(size_t)s * C];
     lpp = ll + A->prop_prior[c];
     double la = lpp - A->cur[c];
     if (A->use_surrogate) la -= A->prop_s[c] - A->cur_s[c];
