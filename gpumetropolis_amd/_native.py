@@ -634,13 +634,15 @@ class DeviceModel:
                proposal_sd=None, proposal_l=None, trace=False,
                final_state=False, stream_id=0, iter_offset=0,
                work_items=0, work_groups=0, chains_per_wave=0,
-               de_sync=None, langevin=False) -> RunOutput:
+               de_sync=None, langevin=False, raw_draws=False) -> RunOutput:
         """Stepwise random-walk Metropolis; with `surrogate` (a Surrogate)
         two-stage delayed acceptance, without it plain Metropolis with the
         uniforms of `run`.  Mode 2 when proposal_l is given, else mode 0;
         `de_sync=(gamma, de_noise)` runs synchronous Differential Evolution
         (mode 1) with proposal_sd as the jitter scale; `langevin` runs MALA
-        (mode 3) preconditioned by proposal_l when given, else proposal_sd."""
+        (mode 3) preconditioned by proposal_l when given, else proposal_sd.
+        `raw_draws`: the draws in the library's layout only (draws_pct,
+        parameter x chain x iteration), as `run` gives them."""
         np_ = self.n_params
         init, temps = _f64(init), _f64(temperatures)
         n_chains = init.size // np_
@@ -682,7 +684,7 @@ class DeviceModel:
             _p(temps, C.c_double), n_iter, seed & 0xFFFFFFFF, mode,
             _p(pl, C.c_double), C.byref(ext), C.byref(res))
         _check(rc)
-        out = _collect(res)
+        out = _collect(res, raw=raw_draws)
         out.kernel_ms = ext.kernel_ms
         out.exact_evals = ext.exact_evals
         out.stage1_rate = keep["rate"]

@@ -937,7 +937,10 @@ __device__ __noinline__ PointsPlan points_plan(const PointsArgs* __restrict__ P,
   PointsPlan q;
   q.count = P->count ? *P->count : P->n_points;
   q.groups = (q.count + cpb - 1) / cpb;
-  uint32_t n_sl = q.groups ? (P->target_items + q.groups - 1) / q.groups : 1u;
+  // Slices rounded down: no more than target_items items unless the slot
+  // groups alone are more, so that a grid of target_items workgroups takes
+  // every item in one round (a few items over would run a second round).
+  uint32_t n_sl = q.groups ? P->target_items / q.groups : 1u;
   n_sl = max(1u, min(n_sl, n_tiles));
   q.tps = (n_tiles + n_sl - 1) / n_sl;
   q.n_sl = (n_tiles + q.tps - 1) / q.tps;

@@ -769,16 +769,25 @@ static int run_da_locked(gpum_hip_model* m, gpum_hip_surrogate* S,
     return 1;
   if (!ensure_data(m, t.n_pad)) return 1;
   const uint32_t n_tiles = t.n_pad / t.tile_len;
-  // Exact-stage work items per launch: every CU twice by default.
-  const uint32_t items = X.work_items ? X.work_items : 2u * (uint32_t)g_dev.n_cu;
+  // Exact-stage work items per launch and grid: by default one item per
+  // workgroup the CUs hold at once (160 KB of LDS and 32 waves a CU), so an
+  // item's start (queue, uniforms, first tile) is paid once per resident
+  // workgroup.  On 1024 DE chains over 1e6 rows (one 148 KB workgroup a
+  // CU) 256 items took 0.090 ms a generation, 512 items 0.095 to 0.103.
+  const size_t wg_lds = t.lds_bytes + kPointsExtraLds + 64 +
+                        (t.stream_mode ? kStreamStaticLds : 0);
+  const uint32_t per_cu = std::max<uint32_t>(
+      1u, std::min<uint32_t>((uint32_t)(g_dev.lds_max / wg_lds),
+                             32u / std::max(1u, tiling_waves(t))));
+  const uint32_t resident = per_cu * (uint32_t)g_dev.n_cu;
+  const uint32_t items = X.work_items ? X.work_items : resident;
   const uint32_t max_slices = std::max(1u, std::min(n_tiles, items));
   const uint32_t groups_cap = (C + t.cpb - 1) / t.cpb;
   const uint32_t n_wg =
       X.work_groups
           ? X.work_groups
           : std::max(1u, (uint32_t)std::min<uint64_t>(
-                             (uint64_t)groups_cap * n_tiles,
-                             2u * (uint32_t)g_dev.n_cu));
+                             (uint64_t)groups_cap * n_tiles, resident));
 
   const size_t ncp = (size_t)C * np;
   const size_t n_draws = (size_t)n_iter * ncp;
