@@ -95,6 +95,10 @@ def parse():
     p.add_argument("--no-eval", action="store_true",
                    help="skip the FP64 pointwise / gradient, synchronous "
                         "DE and MALA sections")
+    p.add_argument("--no-m1", action="store_true",
+                   help="skip the reference's M1 benchmark cells")
+    p.add_argument("--m1-reps", type=int, default=3,
+                   help="replications per M1 cell (median reported)")
     p.add_argument("--no-gaussian", action="store_true",
                    help="skip the Gaussian-mean (8 chains) section")
     p.add_argument("--no-batch", action="store_true",
@@ -106,7 +110,7 @@ def parse():
                         "chain, which holds split R-hat near 1.02")
     p.add_argument("--batch-warmup", type=int, default=2000)
     p.add_argument("--only", choices=["pt", "batch", "da", "eval",
-                                      "gaussian"],
+                                      "gaussian", "m1"],
                    help="run only this section and print its JSON (for "
                         "profiling one kernel family); no headline line")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -241,6 +245,75 @@ def gaussian_mean_section(a):
             "closed_form_mean_err_mcse": abs(p["mean"] - w.meta["post_mean"]) / p["mcse"],
             "sd_ratio": p["sd"] / w.meta["post_sd"],
             "sampling_accept": float(np.mean(fit.accept_rate))}
+
+
+# The reference's published M1 cells: gpumetropolis-cuda on an RTX 4090,
+# median ESS/s over replications (benchmark/full_m1_cell_summary.csv:35-43).
+M1_PUBLISHED = {(10000, 1): 3040.54, (10000, 8): 24143.41,
+                (10000, 64): 101775.20, (10000, 512): 207630.28,
+                (10000, 4096): 259145.06, (100000, 1): 53.18,
+                (100000, 8): 398.84, (100000, 64): 2873.47,
+                (100000, 512): 5785.03}
+
+
+def m1_cells_section(a):
+    """The reference's M1 benchmark protocol on every published cell
+    (benchmark/run_cell.R, benchmark/adapters/cpu_adapters.R:25-61,
+    benchmark/models/m1_gaussian_mean.R): y ~ N(0.37, 1.5^2) of N rows,
+    log-likelihood -((y - mu)^2) / 4.5, C chains started at
+    mean(y) + 1.5 * seq(-2, 2) (mean(y) for one chain), proposal sd
+    2.4 * 1.5 / sqrt(N), 4000 iterations with the API's default warmup (the
+    first half), one discarded call first.  ESS is the sum over chains of
+    each chain's AR spectral ESS of the kept half (coda::effectiveSize, as
+    benchmark/harness_util.R:9-12), over the wall time of the whole API
+    call; the median over replications is compared with the published
+    median of the same cell.  The data are drawn with numpy, not R."""
+    from gpumetropolis_amd import dsl
+    from gpumetropolis_amd.api import gpu_metropolis
+    from gpumetropolis_amd.diagnostics import ess_ar, rhat
+
+    sigma, mu0, T = 1.5, 0.37, 4000
+    model = dsl.gpum_model(f"~ -((y - mu)^2) / {2 * sigma ** 2:.10g}",
+                           ["mu"], ["y"])
+    cells = []
+    for k, ((N, C), pub) in enumerate(M1_PUBLISHED.items()):
+        rates, rhats, walls, errs = [], [], [], []
+        for rep in range(1, a.m1_reps + 1):
+            y = np.random.default_rng(20260000 + 1000 * k + rep).normal(
+                mu0, sigma, N).reshape(N, 1)
+            init = (np.full((1, 1), y.mean()) if C == 1 else
+                    (y.mean() + sigma * np.linspace(-2, 2, C)).reshape(C, 1))
+            kw = dict(init=init, proposal_sd=2.4 * sigma / math.sqrt(N),
+                      seed=10000 * (k + 1) + rep, jit=a.jit)
+            if rep == 1:
+                gpu_metropolis(model, y, n_iter=64, **kw)  # discarded
+            t0 = time.perf_counter()
+            fit = gpu_metropolis(model, y, n_iter=T, **kw)
+            wall = time.perf_counter() - t0
+            d = fit.draws[:, :, 0]
+            total = sum(ess_ar(d[:, c]) for c in range(C))
+            rates.append(total / wall)
+            walls.append(wall)
+            if C > 1:
+                rhats.append(rhat(d))
+            # closed form N(mean(y), sigma^2 / N), in Monte Carlo errors
+            mcse = sigma / math.sqrt(N) / math.sqrt(max(total, 1.0))
+            errs.append(abs(float(d.mean()) - float(y.mean())) / mcse)
+        med = float(np.median(rates))
+        cells.append({"N": N, "C": C, "ess_per_s_median": med,
+                      "wall_s_median": float(np.median(walls)),
+                      "rhat_max": max(rhats) if rhats else None,
+                      "closed_form_max_err_mcse": max(errs),
+                      "published_rtx4090": pub, "ratio": med / pub})
+        print(f"[bench-m1] N={N} C={C}: {med:.0f} ESS/s "
+              f"({med / pub:.2f}x the published cell)", file=sys.stderr,
+              flush=True)
+    return {"protocol": "reference M1 harness: 4000 iterations, half "
+                        "warmup, sum of per-chain AR spectral ESS over the "
+                        f"API call's wall time, median of {a.m1_reps} "
+                        "replications",
+            "cells": cells,
+            "min_ratio": min(c["ratio"] for c in cells)}
 
 
 MLP_FLOP_PER_ROW = 2 * (16 * 256 + 256 * 256)  # matrix-core flops, forward
@@ -702,7 +775,8 @@ def main():
         raise RuntimeError("bench needs a gfx950 GPU: " + _native.last_error())
     if a.only:
         fn = {"pt": pt_section, "batch": batch_section, "da": da_section,
-              "eval": eval_section, "gaussian": gaussian_mean_section}[a.only]
+              "eval": eval_section, "gaussian": gaussian_mean_section,
+              "m1": m1_cells_section}[a.only]
         print(json.dumps({a.only: fn(a)}), flush=True)
         return
 
@@ -865,6 +939,8 @@ def main():
     dm.close()
     if not a.no_gaussian:
         out["gaussian_mean"] = gaussian_mean_section(a)
+    if not a.no_m1:
+        out["m1_cells"] = m1_cells_section(a)
     if not a.no_da:
         out["surrogate_da"] = da_section(a)
     if not a.no_pt:

@@ -127,3 +127,44 @@ def summarize(draws: np.ndarray) -> dict:
     return {"params": rows,
             "max_rhat": max(x["rhat"] for x in rows),
             "min_ess": min(x["ess"] for x in rows)}
+
+
+def ess_ar(x: np.ndarray) -> float:
+    """Effective sample size of one chain from its spectral density at
+    frequency zero, estimated by an autoregression: the estimator of
+    coda::effectiveSize, which the reference's M1 benchmark harness sums
+    over chains (benchmark/harness_util.R:9-12).
+
+    The series is centred; autocovariances (divisor n) up to lag
+    p_max = min(n - 1, floor(10 log10 n)) give the Yule-Walker fits of
+    every order by the Levinson-Durbin recursion; the order p minimises
+    n log(v_p) + 2 p (Akaike); the innovation variance is rescaled by
+    n / (n - p - 1), and S(0) = v / (1 - sum(phi))^2.  ESS = n var(x) / S(0)
+    (var with divisor n - 1), 0 for a constant chain."""
+    x = np.asarray(x, dtype=np.float64).ravel()
+    n = x.size
+    if n < 2:
+        return float(n)
+    xc = x - x.mean()
+    r0 = float(np.dot(xc, xc)) / n
+    if r0 == 0.0:
+        return 0.0
+    pmax = int(min(n - 1, np.floor(10.0 * np.log10(n))))
+    r = np.array([np.dot(xc[:n - k], xc[k:]) / n for k in range(pmax + 1)])
+    phi = np.zeros(0)
+    v = r0
+    fits, var = [phi], [v]
+    for k in range(1, pmax + 1):
+        kappa = (r[k] - np.dot(phi, r[k - 1:0:-1])) / v
+        phi = np.concatenate([phi - kappa * phi[::-1], [kappa]])
+        v = v * (1.0 - kappa * kappa)
+        if not v > 0.0:
+            break
+        fits.append(phi)
+        var.append(v)
+    var = np.asarray(var)
+    aic = n * np.log(var) + 2.0 * np.arange(var.size)
+    p = int(np.argmin(aic))
+    v_pred = var[p] * n / (n - (p + 1))
+    spec = v_pred / (1.0 - float(np.sum(fits[p]))) ** 2
+    return 0.0 if spec == 0.0 else n * float(np.var(x, ddof=1)) / spec

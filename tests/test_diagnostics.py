@@ -157,3 +157,28 @@ def test_summarize_equals_the_per_parameter_functions():
         assert row["mean"] == pytest.approx(M.mean(), rel=1e-12)
         assert row["sd"] == pytest.approx(M.std(ddof=1), rel=1e-12)
     assert s["max_rhat"] == pytest.approx(dg.rhat(x[:, :, 1]), rel=1e-12)
+
+
+# The AR spectral ESS (coda::effectiveSize's estimator), pinned by what the
+# mathematics fixes: independent draws give about n, an AR(1) chain with
+# coefficient rho gives n (1 - rho) / (1 + rho), a constant chain 0.
+def test_ess_ar_iid():
+    x = np.random.default_rng(3).standard_normal(20000)
+    assert abs(dg.ess_ar(x) / 20000 - 1.0) < 0.05
+
+
+@pytest.mark.parametrize("rho", [0.5, 0.9, -0.3])
+def test_ess_ar_ar1(rho):
+    rng = np.random.default_rng(7)
+    n = 200000
+    e = rng.standard_normal(n)
+    x = np.empty(n)
+    x[0] = e[0] / np.sqrt(1 - rho * rho)
+    for t in range(1, n):
+        x[t] = rho * x[t - 1] + e[t]
+    want = n * (1 - rho) / (1 + rho)
+    assert abs(dg.ess_ar(x) / want - 1.0) < 0.06
+
+
+def test_ess_ar_constant_chain():
+    assert dg.ess_ar(np.full(500, 2.5)) == 0.0
