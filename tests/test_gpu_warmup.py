@@ -31,6 +31,10 @@ CASES = [
      False),
     ("logistic3_freeze", lambda: workloads.logistic(n=20000, d=3, seed=9), 64,
      4000, True),
+    # d = 16: a whole workgroup per chain's 16 x 16 moments, four merge
+    # groups in the pooled accumulator
+    ("logistic16", lambda: workloads.logistic(n=8000, d=16, seed=4), 24, 800,
+     False),
 ]
 
 
