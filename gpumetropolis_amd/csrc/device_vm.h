@@ -418,11 +418,38 @@ __device__ __forceinline__ void vm_run_acc(vm_prog_t prog, uint32_t n_instr,
                                            const float* __restrict__ tile,
                                            uint32_t ld, int lane,
                                            float (&s)[CW]) {
-  VmIns4 nxt = vm_fetch(prog, 1);
-  for (uint32_t pc = 1; pc < n_instr; ++pc) {
-    const uint32_t a = nxt.a, b = nxt.b;
-    nxt = vm_fetch(prog, pc + 1);
-    vm_acc_fma<OPT, CW>(U, nu, tile, ld, lane, a, b, s);
+  // Software-pipelined: the next product's column and uniforms are read
+  // from LDS before this product's FMAs, and the instruction after it one
+  // step earlier still (the fetch of index n_instr is the one the plain
+  // loop also makes).  Each s[j] takes the same FMAs in the same order.
+  if (n_instr < 2) return;
+  VmIns4 cur = vm_fetch(prog, 1);
+  VmIns4 nxt = vm_fetch(prog, 2);
+  float D[OPT], ua[CW];
+  vm_load_col<OPT>(tile + cur.b * ld, lane, D);
+#pragma unroll
+  for (int j = 0; j < CW; ++j) ua[j] = U[j * nu + cur.a];
+  for (uint32_t pc = 1; pc + 1 < n_instr; ++pc) {
+    const VmIns4 after = vm_fetch(prog, pc + 2);
+    float Dn[OPT], uan[CW];
+    vm_load_col<OPT>(tile + nxt.b * ld, lane, Dn);
+#pragma unroll
+    for (int j = 0; j < CW; ++j) uan[j] = U[j * nu + nxt.a];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+#pragma unroll
+      for (int k = 0; k < OPT; ++k) s[j] = fmaf(ua[j], D[k], s[j]);
+    }
+#pragma unroll
+    for (int k = 0; k < OPT; ++k) D[k] = Dn[k];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) ua[j] = uan[j];
+    nxt = after;
+  }
+#pragma unroll
+  for (int j = 0; j < CW; ++j) {
+#pragma unroll
+    for (int k = 0; k < OPT; ++k) s[j] = fmaf(ua[j], D[k], s[j]);
   }
 }
 
