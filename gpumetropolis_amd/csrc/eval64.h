@@ -28,9 +28,9 @@ struct TapeOp {
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 // Observations per (point, slice) work item of the gradient kernel: 256
-// threads, 16 observations each.  It depends on nothing but the kernel, so
+// threads, 32 observations each.  It depends on nothing but the kernel, so
 // a point's gradient has the same summation order in any batch.
-constexpr uint32_t kGradSliceObs = 4096;
+constexpr uint32_t kGradSliceObs = 8192;
 // Work items of the compiled pointwise kernel: an observation's
 // parameter-free values are computed once for kPointGroup points, over
 // slices of kPointwiseSliceObs observations (values do not depend on it).

@@ -13,7 +13,7 @@
 // The interpreter (InterpEval64) and code generated per program share them.
 //
 // A work item is (point, slice of kGradSliceObs observations): 256 threads,
-// 16 observations each, the point fixed across a thread's loop.
+// 32 observations each, the point fixed across a thread's loop.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -128,7 +128,7 @@ int gpum_hip_grad_batch_noalloc(const uint32_t* code, size_t code_len,
     return 1;
   }
   // One point's partials: every slice's gradient and value (64-bit: a
-  // 32-bit n_obs + 4095 would wrap).
+  // 32-bit n_obs + kGradSliceObs - 1 would wrap).
   const uint64_t ns64 = ((uint64_t)n_obs + kGradSliceObs - 1) / kGradSliceObs;
   const uint64_t per_point = ns64 * (n_params + 1) * sizeof(double);
   if (per_point > kPartialBytes) {

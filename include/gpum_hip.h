@@ -214,7 +214,7 @@ int gpum_hip_grad_batch(const uint32_t* code, size_t code_len,
  * owns; the program's tape, its constants and the partial sums live in an
  * 8 MiB array of the library, one per GPU, held until the process ends.
  * Limits beyond gpum_hip_grad_batch's: at most 4096 constants, and
- * ceil(n_obs / 4096) * (n_params + 1) * 8 bytes of partial sums per point
+ * ceil(n_obs / 8192) * (n_params + 1) * 8 bytes of partial sums per point
  * must fit in the array (about 2.4e8 observations at 16 parameters); larger
  * calls return an error.  Points are processed in chunks that fit.  Results
  * equal gpum_hip_grad_batch's bit for bit (the interpreted kernels; this

@@ -143,14 +143,14 @@ def test_zero_adjoint_rule(hip, orc):
     """A zero adjoint stops at its operation (cpu_native.rs:218-220): in
     exp(-exp(a x)) with exp(a x) = inf the value is 0 and the reference's
     gradient term is 0, where 0 * inf would give NaN.  Only the second
-    4096-observation slice holds such rows, so the compiled gradient takes
+    8192-observation slice holds such rows, so the compiled gradient takes
     its exact path there alone; it must equal the interpreter bit for bit
     and the oracle within the FP64 tolerance."""
     m = dsl.gpum_model("~ exp(-exp(a * x)) + b * y", ["a", "b"], ["y", "x"])
-    n_obs = 3 * 4096 + 5
+    n_obs = 3 * 8192 + 5
     rng = np.random.default_rng(11)
     x = rng.uniform(0.1, 1.0, n_obs)
-    x[4096 + rng.choice(4096, 7, replace=False)] = 900.0
+    x[8192 + rng.choice(8192, 7, replace=False)] = 900.0
     data = np.column_stack([rng.normal(0.3, 1.0, n_obs), x])
     pts = np.array([[1.0, 0.5], [1.2, -0.3], [0.9, 0.1]])
     print("[grad] zero-adjoint rule", flush=True)
