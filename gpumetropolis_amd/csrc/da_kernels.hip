@@ -380,6 +380,36 @@ __device__ double mala_log_q(const DaArgs* __restrict__ A, uint32_t c,
   return q;
 }
 
+// mala_log_q(v1, m1) - mala_log_q(v2, m2): the two substitutions advance in
+// one loop (each with its own operations in its own order), so their
+// serial steps overlap.
+__device__ double mala_log_q_diff(const DaArgs* __restrict__ A, uint32_t c,
+                                  double v1, double m1, double v2, double m2,
+                                  uint32_t lane) {
+#pragma clang fp contract(off)
+  const uint32_t np = A->n_params;
+  const double* L = A->proposal_l ? A->proposal_l + (size_t)c * np * np : nullptr;
+  if (!L)
+    return mala_log_q(A, c, v1, m1, lane) - mala_log_q(A, c, v2, m2, lane);
+  const bool mine = lane < np;
+  double q1 = 0.0, q2 = 0.0;
+  double acc1 = v1 - m1, acc2 = v2 - m2;
+  for (uint32_t k = 0; k < np; ++k) {
+    const double lkk = L[k * np + k];
+    const bool own = mine && lane == k;
+    const double t1 = from_lane(own ? acc1 / lkk : 0.0, k);
+    const double t2 = from_lane(own ? acc2 / lkk : 0.0, k);
+    q1 -= 0.5 * t1 * t1;
+    q2 -= 0.5 * t2 * t2;
+    if (mine && lane > k) {
+      const double lk = L[lane * np + k];
+      acc1 -= lk * t1;
+      acc2 -= lk * t2;
+    }
+  }
+  return q1 - q2;
+}
+
 // The log-posterior's gradient at a point: the data term's, then the
 // prior's added (as the reference adds the prior's adjoints after the data
 // loop, cpu_native.rs:285-298).
@@ -459,7 +489,7 @@ __global__ void __launch_bounds__(256) mala_accept_kernel(
   const double ll = A->ll_prop[c];
   const double plp = ll + A->v_prior[c];
   const double my = mala_drift(A, c, g, lane) + pr;
-  const double lq = mala_log_q(A, c, th, my, lane) - mala_log_q(A, c, pr, mxo, lane);
+  const double lq = mala_log_q_diff(A, c, th, my, pr, mxo, lane);
   StepUniforms us{A->seed, A->stream_id, A->iter_offset + t, c};
   const double u = us.get(2 * np);
   const bool acc_ok =
